@@ -1,0 +1,133 @@
+"""Throughput of the femmd2D step on one MI355X.
+
+Workload (the headline configuration): 1,048,576 LJ particles at density
+0.8 (rc = 2.5, T = 1) plus 1,024 CST elastomer disks of radius 5 (about
+93k nodes, 154k triangles), periodic square box, FP32 state with FP64
+energy sums, dt = 0.005, synthetic seeded inputs (femmd_old_amd.configs.c2).
+
+A step is one pass of the hot path over the whole system: half-kick + drift
++ wrap + cell binning, counting sort, LJ / CST forces, and the second
+half-kick with its kinetic-energy reduction (fused into the next step's
+first kernel).  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line: value = particle-steps/s; roofline for the dominant
+kernel (live HIP-event timing on the engine's stream, algorithmic bytes);
+cpu_baseline = the reference CPU path restated in C++ (oracle/), timed on a
+bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def algorithmic_bytes(n, ncells, ntri, nbond):
+    """Bytes each kernel must move per launch (each input array read once,
+    each output written once; FP32 values, int32 indices)."""
+    forces = 28 * n + 4 * ncells + 100 * ntri + 48 * nbond
+    kick = 60 * n + 36 * ntri + 24 * nbond
+    scan = 36 * n + 12 * ncells
+    return {"forces": forces, "kick": kick, "scan": scan, "step": forces + kick + scan}
+
+
+def cpu_baseline(w, steps, threads):
+    import numpy as np
+    import oracle as O
+    from femmd_old_amd import configs
+    osys = O.from_workload(w)
+    ga = w.global_arrays()
+    vx, vy = configs.node_velocities(w)
+    b = O.Baseline(osys, vx, vy, ga["mass"], w.dt, threads=threads)
+    b.run(2)  # warm-up
+    secs = b.run(steps)
+    return {"value": w.n_total * steps / secs, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the full workload ({w.n_total} particles), reference "
+                      f"DEV=CPU path restated in FP32 (Verlet list, skin 0.4, g++ -O2), "
+                      f"{threads} thread(s), {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (bench.py --gpus 1)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from femmd_old_amd import configs
+
+    t_gen = time.time()
+    w = configs.c2(scale=args.scale)
+    s = configs.build_system(w)
+    print(f"[bench] workload c2: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} L={w.Lx:.2f} "
+          f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
+    ncx, ncy, _, _, _ = s.cells()
+
+    s.run(args.warmup)
+    e0 = s.energies()
+    t0 = time.perf_counter()
+    s.run(args.steps)  # returns after a stream synchronise
+    t1 = time.perf_counter()
+    e1 = s.energies()
+    secs = t1 - t0
+    value = w.n_total * args.steps / secs
+    print(f"[bench] timed {args.steps} steps in {secs:.3f}s", file=sys.stderr, flush=True)
+
+    prof = s.profile(args.profile_steps)
+    ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds)
+    kern = max(("forces", "kick"), key=lambda k: prof[k])
+    achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": f"k_{kern}", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_launch": ab[kern],
+            "launch_ms": round(prof[kern], 5),
+            "step_kernels_ms": {k: round(v, 5) for k, v in prof.items()},
+            "step_frac": round(ab["step"] / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    E0 = e0["ke"] + e0["u_lj"] + e0["u_fea"] + e0["u_bond"]
+    E1 = e1["ke"] + e1["u_lj"] + e1["u_fea"] + e1["u_bond"]
+    out = {
+        "metric": "particle-steps/s @1M 2D LJ+FEA; batched vs serial replicas; adjoint steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator femmd_old_amd.configs.c2)",
+        "config": {"workload": "c2: 1,048,576 LJ + 1,024 CST disks r=5, rho 0.8, rc 2.5, "
+                               "dt 0.005, periodic", "n_particles": w.n_total,
+                   "n_triangles": w.n_tri, "cells": [ncx, ncy], "box": w.Lx,
+                   "reductions": "f64"},
+        "energy_drift": (E1 - E0) / abs(E0),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

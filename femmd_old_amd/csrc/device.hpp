@@ -1,0 +1,126 @@
+// device.hpp -- data layout shared by the host engine and the HIP kernels.
+//
+// All per-particle state is structure-of-arrays in HBM, in the engine's
+// internal order (per replica: particles sorted by their cell at commit time,
+// which keeps every cell's members close in memory).  A batch of replicas is
+// stored back to back: replica r owns particles [p0, p1), cells
+// [cell0, cell0 + ncx*ncy), elements and bonds of its own ranges.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fm {
+
+// Largest tile row (cells) and its slot table: 3 rows of (W + 2) cells.
+constexpr int kMaxTileW = 64;
+constexpr int kMaxSlots = 3 * (kMaxTileW + 2);
+// Cells scanned per workgroup of the counting-sort scan (256 threads x 4).
+constexpr int kScanCells = 1024;
+constexpr int kScanThreads = 256;
+constexpr int kKickThreads = 256;
+constexpr int kElemThreads = 256;
+constexpr int kMaxPairTypes = 4;  // ntypes <= 4 on the typed path
+
+// Per-replica geometry and integration constants.
+struct alignas(16) RepDesc {
+    float Lx, Ly;      // periodic box
+    float hLx, hLy;    // 0.5 * box (exact)
+    float fncx, fncy;  // (float)ncx, (float)ncy for the cell key
+    int32_t ncx, ncy;
+    int32_t cell0;     // first global cell of this replica
+    int32_t p0, p1;    // particle range
+    float dt, hdt;     // step and half step
+    int32_t pt;        // offset of this replica's pair table
+    int32_t ntypes;
+    float rc2_uniform; // ((rc*rc)*s)*s for single-type uniform-sigma systems
+    float s2_uniform;  // sigma^2 for uniform sigma
+    int32_t pad[3];
+};
+
+// One pair-table entry: 48 eps, 4 eps, additive shift, cutoff in sigma units.
+struct alignas(16) PairEntry {
+    float e48, e4, shift, rc;
+};
+
+// Work tables: one workgroup per entry.
+struct alignas(16) Tile {  // force pass, LJ role
+    int32_t rep, cy, cx0;
+    int32_t w_flags;  // W in bits 0..15, bit 16 = needs minimum image
+};
+struct alignas(16) Span {  // kick / element / bond workgroups
+    int32_t rep, begin, end, pad;
+};
+
+// Kernel argument bundles (passed by value).
+struct KickArgs {
+    const RepDesc *reps;
+    const Span *spans;
+    float *x, *y, *vx, *vy;
+    const float *fx, *fy;      // pair forces (internal order)
+    const float2 *cf;          // element / bond corner forces
+    const int32_t *goff;       // gather CSR offsets (n + 1), may be null
+    const int32_t *gent;       // gather CSR entries
+    const float *mass, *inv_m;
+    int32_t *count;            // per-cell counters (zero on entry)
+    int32_t *tmp;              // cell * cap + slot -> particle
+    int32_t cap;
+    double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
+    uint32_t *err;
+};
+
+struct ScanArgs {
+    int32_t ncells;
+    int32_t cap;
+    int32_t n;                 // particles (bound on every perm / sorted write)
+    int32_t *count;            // zeroed here for the next binning
+    const int32_t *tmp;
+    int32_t *bsum;             // per scan workgroup totals
+    int32_t *start;            // ncells + 1
+    int32_t *perm;             // sorted position -> particle
+    const float *x, *y;
+    const int32_t *grp;        // exclusion group per particle
+    const float *sig;          // per-particle sigma (may be null)
+    const int32_t *typ;        // per-particle type (may be null)
+    float4 *sorted;            // sorted position -> {x, y, grp bits, sigma}
+    int32_t *sorted_typ;       // sorted position -> type (may be null)
+};
+
+struct ForceArgs {
+    const RepDesc *reps;
+    const PairEntry *ptab;
+    // pair role
+    const Tile *tiles;
+    int32_t ntiles;
+    int32_t lds_cap;           // staged particles that fit in LDS
+    const int32_t *start;
+    const int32_t *perm;
+    const float4 *sorted;
+    const int32_t *sorted_typ;
+    float *fx, *fy;            // pair forces out, internal order
+    double *lj_part;           // 2 per tile: U, W
+    // element role
+    const Span *espans;
+    int32_t nespans;
+    const int32_t *t0, *t1, *t2;
+    const float4 *dmbm;        // per element: Dm {00,01,10,11}, Bm {00,01,10,11}
+    const float2 *alm;         // A0 * lambda, A0 * mu
+    int32_t model;             // 0 NH, 1 SVK (legacy is SVK with doubled moduli)
+    const float *x, *y;        // current positions, internal order
+    float2 *cf;                // corner forces out
+    double *fea_part;          // 2 per element workgroup: U, W
+    // bond role
+    const Span *bspans;
+    int32_t nbspans;
+    const int32_t *bi, *bj;
+    const float *bk, *br0;
+    int32_t cf_bond0;          // first corner slot of the bonds
+    double *bond_part;         // 2 per bond workgroup: U, W
+    // diagnostics (pair-set dump); null in production
+    const int32_t *uid;        // internal -> caller id
+    int32_t *dbg_cnt;
+    long long *dbg_s1, *dbg_s2;
+    uint32_t *err;
+};
+
+}  // namespace fm

@@ -1,0 +1,935 @@
+// engine.hip -- build of the device image and step orchestration.
+//
+// Replaces the reference's per-step host loop (femmd2D/src/integrator.cpp:
+// 8-57) and its upload / topology setup (femmd2D/src/particles.cpp:105-199,
+// femmd2D/src/bonds.cpp:31-83, femmd2D/src/nn.cpp:192-250).  One step is
+// four kernels on one stream (kick+bin, scan, scatter, forces) with no host
+// synchronisation; the step loop is replayed from a captured HIP graph.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace fm {
+
+namespace {
+
+constexpr int kGraphSteps = 16;  // step bodies per captured graph
+
+float wrap_host(float r, float L) {  // femmd2D/src/def.h:112, FP32
+    float q = r / L;
+    float f = std::floor(q);
+    float p = f * L;
+    return r - p;
+}
+
+int cell_key_host(float x, float y, const RepDesc &R) {  // nnCuda.cu:9-17
+    float tx = x * R.fncx;
+    tx = tx / R.Lx;
+    int ix = (int)tx;
+    if (ix == R.ncx) ix -= 1;
+    float ty = y * R.fncy;
+    ty = ty / R.Ly;
+    int iy = (int)ty;
+    if (iy == R.ncy) iy -= 1;
+    ix = std::min(std::max(ix, 0), R.ncx - 1);
+    iy = std::min(std::max(iy, 0), R.ncy - 1);
+    return ix + iy * R.ncx;
+}
+
+template <class T>
+hipError_t upload(DBuf<T> &b, const std::vector<T> &v, hipStream_t s) {
+    hipError_t e = b.alloc(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+std::string dev_error_text(uint32_t e) {
+    std::string s = "device error:";
+    if (e & FM_DEV_CELL_OVERFLOW) s += " cell capacity exceeded;";
+    if (e & FM_DEV_TILE_OVERFLOW) s += " cell tile exceeded LDS capacity;";
+    if (e & FM_DEV_INVERTED) s += " inverted element (det F <= 0);";
+    if (e & FM_DEV_NONFINITE) s += " non-finite coordinate;";
+    return s;
+}
+
+}  // namespace
+
+Engine::~Engine() {
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_steps_ = 0;
+    return FM_OK;
+}
+
+int Engine::check(hipError_t e, const char *what, std::string &err) {
+    if (e == hipSuccess) return FM_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return FM_ERR_HIP;
+}
+
+double Engine::density_of(int r) const {
+    const RepDesc &R = reps_h_[r];
+    return (double)(R.p1 - R.p0) / ((double)R.Lx * R.Ly);
+}
+
+int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<Engine> &out,
+                  std::string &err) {
+    if (reps.empty()) { err = "no systems"; return FM_ERR_ARG; }
+    std::unique_ptr<Engine> E(new Engine());
+    E->device_ = reps[0]->device;
+    int rc = E->check(hipSetDevice(E->device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = E->check(hipStreamCreateWithFlags(&E->stream_, hipStreamNonBlocking), "stream", err);
+    if (rc) return rc;
+    hipStream_t st = E->stream_;
+    const int R = (int)reps.size();
+    E->R_ = R;
+    E->model_ = reps[0]->model;
+    E->block_ = reps[0]->block;
+    E->use_graph_ = reps[0]->use_graph;
+
+    // ---- pass 1: validation and cell geometry per replica ----
+    std::vector<RepDesc> rd(R);
+    std::vector<float> smin(R), smax(R);
+    double amax = 0.0, rho_cp = 0.0;
+    for (int r = 0; r < R; ++r) {
+        const HostSystem &S = *reps[r];
+        const size_t n = S.x.size();
+        if (n == 0) { err = "system " + std::to_string(r) + " has no particles"; return FM_ERR_ARG; }
+        if (!(S.Lx > 0) || !(S.Ly > 0)) { err = "box must be positive"; return FM_ERR_ARG; }
+        if (S.model != E->model_) { err = "all replicas of a batch must use one material model"; return FM_ERR_ARG; }
+        if (S.ntypes < 1 || S.ntypes > kMaxPairTypes) {
+            err = "ntypes must be in [1, 4]";
+            return FM_ERR_ARG;
+        }
+        for (int32_t t : S.type)
+            if (t < 0 || t >= S.ntypes) { err = "particle type out of range of the pair table"; return FM_ERR_ARG; }
+        float lo = S.sigma[0], hi = S.sigma[0];
+        for (float s : S.sigma) { lo = std::min(lo, s); hi = std::max(hi, s); }
+        if (!(lo > 0)) { err = "sigma must be positive"; return FM_ERR_ARG; }
+        smin[r] = lo;
+        smax[r] = hi;
+        if (lo != hi) E->sigma_ = true;
+        if (S.ntypes > 1) E->types_ = true;
+        double rcmax = 0;
+        for (float v : S.rc) rcmax = std::max(rcmax, (double)v);
+        const double rc_eff = rcmax * hi;
+        // cell side >= the largest cutoff plus the FP32 slop of the key
+        const double cmin = rc_eff * (1.0 + 1e-5) + 1e-6 * std::max(S.Lx, S.Ly);
+        RepDesc &D = rd[r];
+        std::memset(&D, 0, sizeof(D));
+        D.Lx = (float)S.Lx;
+        D.Ly = (float)S.Ly;
+        D.hLx = 0.5f * D.Lx;
+        D.hLy = 0.5f * D.Ly;
+        D.ncx = (int32_t)std::floor(S.Lx / cmin);
+        D.ncy = (int32_t)std::floor(S.Ly / cmin);
+        if (D.ncx < 3 || D.ncy < 3) {
+            err = "box too small: fewer than 3 cells of side >= rc*sigma per axis";
+            return FM_ERR_GEOMETRY;
+        }
+        D.fncx = (float)D.ncx;
+        D.fncy = (float)D.ncy;
+        D.dt = (float)S.dt;
+        D.hdt = 0.5f * D.dt;  // femmd2D/src/integrator.cpp:102 hdt = 0.5*dt
+        D.ntypes = S.ntypes;
+        const float s0 = hi;
+        const float rcf = S.rc[0];
+        D.rc2_uniform = rcf * rcf * s0 * s0;  // reference order ((rc*rc)*s)*s
+        D.s2_uniform = s0 * s0;
+        amax = std::max(amax, (double)(D.Lx / D.ncx) * (D.Ly / D.ncy));
+        rho_cp = std::max(rho_cp, 1.16 / ((double)lo * lo));
+    }
+    // per-cell list capacity: twice the close-packed count of a cell
+    E->rho_cp_ = rho_cp;
+    E->cap_ = std::min(60, std::max(32, (int)std::ceil((2.0 * rho_cp * amax + 4.0) / 8.0) * 8));
+
+    // ---- pass 2: layout ----
+    std::vector<float> hx, hy, hvx, hvy, hm, him, hs;
+    std::vector<int32_t> hg, ht, huid;
+    std::vector<PairEntry> hpt;
+    std::vector<int32_t> ht0, ht1, ht2, hbi, hbj;
+    std::vector<float4> hdmbm;
+    std::vector<float2> halm;
+    std::vector<float> hbk, hbr0;
+    std::vector<Span> ks, es, bs;
+    std::vector<int32_t> erange(2 * R), brange(2 * R);
+    E->iid_h_.resize(R);
+    E->lj_on_.resize(R);
+    for (int r = 0; r < R; ++r) E->lj_on_[r] = reps[r]->lj_on ? 1 : 0;
+    int p0 = 0, cell0 = 0;
+    for (int r = 0; r < R; ++r) {
+        const HostSystem &S = *reps[r];
+        RepDesc &D = rd[r];
+        const int n = (int)S.x.size();
+        D.p0 = p0;
+        D.p1 = p0 + n;
+        D.cell0 = cell0;
+        D.pt = (int32_t)hpt.size();
+        for (int k = 0; k < S.ntypes * S.ntypes; ++k) {
+            PairEntry pe;
+            pe.e48 = 48.0f * S.eps[k];
+            pe.e4 = 4.0f * S.eps[k];
+            pe.shift = S.ush[k];
+            pe.rc = S.rc[k];
+            hpt.push_back(pe);
+        }
+        // wrap, key, stable order by cell (internal order of this replica)
+        std::vector<float> xw(n), yw(n);
+        std::vector<int> key(n), order(n);
+        for (int u = 0; u < n; ++u) {
+            xw[u] = wrap_host(S.x[u], D.Lx);
+            yw[u] = wrap_host(S.y[u], D.Ly);
+            key[u] = cell_key_host(xw[u], yw[u], D);
+        }
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+        std::vector<int32_t> &iid = E->iid_h_[r];
+        iid.assign(n, 0);
+        for (int k = 0; k < n; ++k) iid[order[k]] = p0 + k;
+        for (int k = 0; k < n; ++k) {
+            const int u = order[k];
+            hx.push_back(xw[u]);
+            hy.push_back(yw[u]);
+            hvx.push_back(S.vx[u]);
+            hvy.push_back(S.vy[u]);
+            const float m = S.mass[u];
+            const bool pinned = !(m > 0.0f) || !std::isfinite(m);
+            hm.push_back(std::isfinite(m) ? m : 0.0f);
+            him.push_back(pinned ? 0.0f : 1.0f / m);
+            hs.push_back(S.sigma[u]);
+            hg.push_back(S.excl[u]);
+            ht.push_back(S.type[u]);
+            huid.push_back(u);
+        }
+        for (int b = p0; b < p0 + n; b += kKickThreads)
+            ks.push_back(Span{r, b, std::min(b + kKickThreads, p0 + n), 0});
+        // elements, ordered by their lowest internal node for locality
+        const int nt = (int)S.tri.size() / 3;
+        std::vector<int> eorder(nt);
+        std::iota(eorder.begin(), eorder.end(), 0);
+        auto lowest = [&](int e) {
+            return std::min(iid[S.tri[3 * e]], std::min(iid[S.tri[3 * e + 1]], iid[S.tri[3 * e + 2]]));
+        };
+        std::stable_sort(eorder.begin(), eorder.end(), [&](int a, int b) { return lowest(a) < lowest(b); });
+        const int e_begin = (int)ht0.size();
+        for (int q = 0; q < nt; ++q) {
+            const int e = eorder[q];
+            const int u0 = S.tri[3 * e], u1 = S.tri[3 * e + 1], u2 = S.tri[3 * e + 2];
+            ht0.push_back(iid[u0]);
+            ht1.push_back(iid[u1]);
+            ht2.push_back(iid[u2]);
+            // femmd2D/src/interactions.cpp:589-602: Dm = [X0-X2, X1-X2],
+            // vol0 = det(Dm)/2 (signed), Bm = Dm^-1; evaluated in FP64
+            const double D00 = (double)S.rx[u0] - S.rx[u2], D10 = (double)S.ry[u0] - S.ry[u2];
+            const double D01 = (double)S.rx[u1] - S.rx[u2], D11 = (double)S.ry[u1] - S.ry[u2];
+            const double det = D00 * D11 - D01 * D10;
+            if (!(std::fabs(det) > 0)) { err = "degenerate element (zero reference area)"; return FM_ERR_ARG; }
+            const double A0 = det / 2;
+            const double B00 = D11 / det, B10 = -D10 / det, B01 = -D01 / det, B11 = D00 / det;
+            hdmbm.push_back(make_float4((float)D00, (float)D01, (float)D10, (float)D11));
+            hdmbm.push_back(make_float4((float)B00, (float)B01, (float)B10, (float)B11));
+            const double Ee = S.tE[e] >= 0 ? S.tE[e] : S.E;
+            const double nue = S.tnu[e] >= 0 ? S.tnu[e] : S.nu;
+            // femmd2D/src/interactions.cpp:44-45
+            double lam = Ee * nue / ((1 - 2 * nue) * (1 + nue));
+            double mu = Ee / (2 * (1 + nue));
+            if (S.model == FM_LEGACY_GPU) { lam *= 2; mu *= 2; }
+            halm.push_back(make_float2((float)(A0 * lam), (float)(A0 * mu)));
+        }
+        erange[2 * r] = e_begin;
+        erange[2 * r + 1] = (int)ht0.size();
+        for (int b = e_begin; b < (int)ht0.size(); b += kElemThreads)
+            es.push_back(Span{r, b, std::min(b + kElemThreads, (int)ht0.size()), 0});
+        const int b_begin = (int)hbi.size();
+        for (size_t l = 0; l < S.bi.size(); ++l) {
+            hbi.push_back(iid[S.bi[l]]);
+            hbj.push_back(iid[S.bj[l]]);
+            hbk.push_back(S.bk[l]);
+            hbr0.push_back(S.br0[l]);
+        }
+        brange[2 * r] = b_begin;
+        brange[2 * r + 1] = (int)hbi.size();
+        for (int b = b_begin; b < (int)hbi.size(); b += kElemThreads)
+            bs.push_back(Span{r, b, std::min(b + kElemThreads, (int)hbi.size()), 0});
+        p0 += n;
+        cell0 += D.ncx * D.ncy;
+    }
+    E->N_ = p0;
+    E->ncells_ = cell0;
+    E->ntri_ = (int)ht0.size();
+    E->nbond_ = (int)hbi.size();
+    E->gather_ = (E->ntri_ + E->nbond_) > 0;
+    E->reps_h_ = rd;
+    E->uid_h_ = huid;
+    E->kspans_h_ = ks;
+    E->espans_h_ = es;
+    E->bspans_h_ = bs;
+
+    // gather CSR: per particle, its corner slots in ascending slot order
+    std::vector<int32_t> goff, gent;
+    if (E->gather_) {
+        const int N = E->N_;
+        std::vector<int32_t> cnt(N + 1, 0);
+        for (int e = 0; e < E->ntri_; ++e) { cnt[ht0[e]]++; cnt[ht1[e]]++; cnt[ht2[e]]++; }
+        for (int l = 0; l < E->nbond_; ++l) { cnt[hbi[l]]++; cnt[hbj[l]]++; }
+        goff.assign(N + 1, 0);
+        for (int i = 0; i < N; ++i) goff[i + 1] = goff[i] + cnt[i];
+        gent.assign(goff[N], 0);
+        std::vector<int32_t> fill(goff.begin(), goff.end() - 1);
+        for (int e = 0; e < E->ntri_; ++e) {
+            gent[fill[ht0[e]]++] = 3 * e;
+            gent[fill[ht1[e]]++] = 3 * e + 1;
+            gent[fill[ht2[e]]++] = 3 * e + 2;
+        }
+        const int b0 = 3 * E->ntri_;
+        for (int l = 0; l < E->nbond_; ++l) {
+            gent[fill[hbi[l]]++] = b0 + 2 * l;
+            gent[fill[hbj[l]]++] = b0 + 2 * l + 1;
+        }
+    }
+
+    // ---- device allocations and uploads ----
+    const int N = E->N_;
+    hipError_t he = hipSuccess;
+#define FM_UP(buf, vec) \
+    if ((he = upload(E->buf, vec, st)) != hipSuccess) return E->check(he, "upload " #buf, err)
+    FM_UP(reps_, rd);
+    FM_UP(ptab_, hpt);
+    FM_UP(x_, hx);
+    FM_UP(y_, hy);
+    FM_UP(vx_, hvx);
+    FM_UP(vy_, hvy);
+    FM_UP(mass_, hm);
+    FM_UP(inv_m_, him);
+    FM_UP(sig_, hs);
+    FM_UP(grp_, hg);
+    FM_UP(typ_, ht);
+    FM_UP(uid_, huid);
+    FM_UP(kspans_, ks);
+    FM_UP(espans_, es);
+    FM_UP(bspans_, bs);
+    FM_UP(t0_, ht0);
+    FM_UP(t1_, ht1);
+    FM_UP(t2_, ht2);
+    FM_UP(dmbm_, hdmbm);
+    FM_UP(alm_, halm);
+    FM_UP(bi_, hbi);
+    FM_UP(bj_, hbj);
+    FM_UP(bk_, hbk);
+    FM_UP(br0_, hbr0);
+    FM_UP(goff_, goff);
+    FM_UP(gent_, gent);
+#undef FM_UP
+#define FM_AL(buf, cnt) \
+    if ((he = E->buf.alloc(cnt)) != hipSuccess) return E->check(he, "alloc " #buf, err)
+    FM_AL(fx_, N);
+    FM_AL(fy_, N);
+    FM_AL(outx_, N);
+    FM_AL(outy_, N);
+    FM_AL(count_, E->ncells_);
+    FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
+    const int nscan = (E->ncells_ + kScanCells - 1) / kScanCells;
+    FM_AL(bsum_, std::max(nscan, 1));
+    FM_AL(start_, E->ncells_ + 1);
+    FM_AL(perm_, N);
+    FM_AL(sorted_, N);
+    FM_AL(sorted_typ_, N);
+    FM_AL(cf_, std::max(1, 3 * E->ntri_ + 2 * E->nbond_));
+    FM_AL(ke_part_, 2 * std::max<size_t>(ks.size(), 1));
+    FM_AL(fea_part_, 2 * std::max<size_t>(es.size(), 1));
+    FM_AL(bond_part_, 2 * std::max<size_t>(bs.size(), 1));
+    FM_AL(ene_, 8 * R);
+    FM_AL(err_, 1);
+#undef FM_AL
+    if ((he = hipMemsetAsync(E->count_.p, 0, sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->err_.p, 0, sizeof(uint32_t), st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->fx_.p, 0, sizeof(float) * N, st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->fy_.p, 0, sizeof(float) * N, st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->ke_part_.p, 0, E->ke_part_.n * sizeof(double), st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->fea_part_.p, 0, E->fea_part_.n * sizeof(double), st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->bond_part_.p, 0, E->bond_part_.n * sizeof(double), st)) != hipSuccess)
+        return E->check(he, "memset", err);
+
+    // tiles (and LDS capacity) for the requested widths
+    std::vector<int> tw(R);
+    for (int r = 0; r < R; ++r) tw[r] = reps[r]->tile_w;
+    rc = E->build_tiles(tw, err);
+    if (rc) return rc;
+    rc = E->check(hipStreamSynchronize(st), "commit", err);
+    if (rc) return rc;
+    out = std::move(E);
+    return FM_OK;
+}
+
+// Tiles of one cell row each: (replica, row, first column, width).  Width 0
+// picks one from the mean cell occupancy so a tile's centre cells hold about
+// 0.9 x block particles.
+int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
+    drop_graph();
+    tiles_h_.clear();
+    tile_w_.assign(R_, 0);
+    rng_h_.assign(8 * R_, 0);
+    int kb = 0, eb = 0, bb = 0;
+    double worst = 0.0;
+    for (int r = 0; r < R_; ++r) {
+        const RepDesc &D = reps_h_[r];
+        const double ppc = (double)(D.p1 - D.p0) / (D.ncx * D.ncy);
+        int W = (r < (int)tile_w.size()) ? tile_w[r] : 0;
+        if (W <= 0) W = (int)std::lround(0.9 * block_ / std::max(ppc, 0.5));
+        W = std::max(1, std::min(W, std::min(kMaxTileW, (int)D.ncx)));
+        tile_w_[r] = W;
+        const int t_begin = (int)tiles_h_.size();
+        for (int cy = 0; cy < (lj_on_[r] ? D.ncy : 0); ++cy)
+            for (int cx0 = 0; cx0 < D.ncx; cx0 += W) {
+                const int w = std::min(W, D.ncx - cx0);
+                // interior tiles (no staged cell wraps, >= 5 cells per axis)
+                // never need the minimum image: |dx| < 2 cells <= 0.4 L
+                const bool interior = D.ncx >= 5 && D.ncy >= 5 && cy >= 1 && cy <= D.ncy - 2 &&
+                                      cx0 >= 1 && cx0 + w <= D.ncx - 1;
+                tiles_h_.push_back(Tile{r, cy, cx0, w | (interior ? 0 : (1 << 16))});
+            }
+        // staged particles of a tile at close packing (1.16 / sigma_min^2)
+        const double acell = (double)(D.Lx / D.ncx) * (D.Ly / D.ncy);
+        worst = std::max(worst, 3.0 * (W + 2) * acell * rho_cp_);
+        int32_t *g = &rng_h_[8 * r];
+        // kick spans
+        g[0] = kb;
+        while (kb < (int)kspans_h_.size() && kspans_h_[kb].rep == r) ++kb;
+        g[1] = kb;
+        g[2] = t_begin;
+        g[3] = (int)tiles_h_.size();
+        g[4] = eb;
+        while (eb < (int)espans_h_.size() && espans_h_[eb].rep == r) ++eb;
+        g[5] = eb;
+        g[6] = bb;
+        while (bb < (int)bspans_h_.size() && bspans_h_[bb].rep == r) ++bb;
+        g[7] = bb;
+    }
+    // LDS capacity: the close-packed count of the staged cells, rounded up
+    int cap = next_pow2((int)std::ceil(worst));
+    const int hard = (types_ ? 2560 : 3072);  // dynamic LDS stays <= 64 KB
+    lds_cap_ = std::max(256, std::min(cap, hard));
+    hipError_t he;
+    if ((he = upload(tiles_, tiles_h_, stream_)) != hipSuccess) return check(he, "tiles", err);
+    if ((he = upload(rng_, rng_h_, stream_)) != hipSuccess) return check(he, "ranges", err);
+    if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
+        return check(he, "lj_part", err);
+    if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess)
+        return check(he, "memset", err);
+    primed_ = false;
+    return FM_OK;
+}
+
+int Engine::set_dt(int r, double dt, std::string &err) {
+    if (r < 0 || r >= R_ || !(dt > 0)) { err = "bad replica or dt"; return FM_ERR_ARG; }
+    reps_h_[r].dt = (float)dt;
+    reps_h_[r].hdt = 0.5f * reps_h_[r].dt;
+    int rc = check(hipMemcpyAsync(reps_.p + r, &reps_h_[r], sizeof(RepDesc), hipMemcpyHostToDevice,
+                                  stream_), "set_dt", err);
+    if (rc) return rc;
+    return check(hipStreamSynchronize(stream_), "set_dt", err);
+}
+
+int Engine::retile(const std::vector<int> &tile_w, int block, std::string &err) {
+    if (block > 0) {
+        if (block != 64 && block != 128 && block != 192 && block != 256) {
+            err = "block must be 64, 128, 192 or 256";
+            return FM_ERR_ARG;
+        }
+        block_ = block;
+    }
+    int rc = check(hipStreamSynchronize(stream_), "sync", err);
+    if (rc) return rc;
+    return build_tiles(tile_w, err);
+}
+
+int Engine::set_tuning(int tile_w, int block, int use_graph, std::string &err) {
+    if (use_graph >= 0) {
+        use_graph_ = use_graph;
+        drop_graph();
+    }
+    if (tile_w > 0 || block > 0) {
+        std::vector<int> tw(R_, tile_w > 0 ? tile_w : 0);
+        if (tile_w <= 0) tw = tile_w_;
+        if (block > 0 && tile_w <= 0) tw.assign(R_, 0);  // re-derive from the new block
+        return retile(tw, block, err);
+    }
+    return FM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+int Engine::launch_bin() {
+    KickArgs a{};
+    a.reps = reps_.p;
+    a.spans = kspans_.p;
+    a.x = x_.p;
+    a.y = y_.p;
+    a.count = count_.p;
+    a.tmp = tmp_.p;
+    a.cap = cap_;
+    a.err = err_.p;
+    return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
+}
+
+static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *tmp, int32_t *bsum,
+                          int32_t *start, int32_t *perm, float *x, float *y, int32_t *grp,
+                          float *sig, int32_t *typ, float4 *sorted, int32_t *sorted_typ) {
+    ScanArgs s{};
+    s.ncells = ncells;
+    s.n = n;
+    s.cap = cap;
+    s.count = count;
+    s.tmp = tmp;
+    s.bsum = bsum;
+    s.start = start;
+    s.perm = perm;
+    s.x = x;
+    s.y = y;
+    s.grp = grp;
+    s.sig = sig;
+    s.typ = typ;
+    s.sorted = sorted;
+    s.sorted_typ = sorted_typ;
+    return s;
+}
+
+// kick2 / drift mode of one step body followed by binning, sort and forces
+int Engine::launch_step_body(bool kick2, bool drift) {
+    hipError_t e;
+    KickArgs k{};
+    k.reps = reps_.p;
+    k.spans = kspans_.p;
+    k.x = x_.p;
+    k.y = y_.p;
+    k.vx = vx_.p;
+    k.vy = vy_.p;
+    k.fx = fx_.p;
+    k.fy = fy_.p;
+    k.cf = cf_.p;
+    k.goff = gather_ ? goff_.p : nullptr;
+    k.gent = gather_ ? gent_.p : nullptr;
+    k.mass = mass_.p;
+    k.inv_m = inv_m_.p;
+    k.count = count_.p;
+    k.tmp = tmp_.p;
+    k.cap = cap_;
+    k.ke_part = ke_part_.p;
+    k.err = err_.p;
+    const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[0], stream_);
+    e = launch_kick(mode, gather_, (int)kspans_h_.size(), k, stream_);
+    if (e != hipSuccess) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[1], stream_);
+    if (!drift) return 0;
+    return launch_forces_only(false);
+}
+
+int Engine::launch_forces_only(bool debug) {
+    hipError_t e;
+    const int nscan = (ncells_ + kScanCells - 1) / kScanCells;
+    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p, tmp_.p, bsum_.p, start_.p, perm_.p, x_.p,
+                           y_.p, grp_.p, sigma_ ? sig_.p : nullptr, types_ ? typ_.p : nullptr,
+                           sorted_.p, types_ ? sorted_typ_.p : nullptr);
+    e = launch_scan_part(0, nscan, s, stream_);
+    if (e != hipSuccess) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
+    e = launch_scan_part(1, nscan, s, stream_);
+    if (e != hipSuccess) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[3], stream_);
+    ForceArgs f{};
+    f.reps = reps_.p;
+    f.ptab = ptab_.p;
+    f.tiles = tiles_.p;
+    f.ntiles = (int)tiles_h_.size();
+    f.lds_cap = lds_cap_;
+    f.start = start_.p;
+    f.perm = perm_.p;
+    f.sorted = sorted_.p;
+    f.sorted_typ = sorted_typ_.p;
+    f.fx = fx_.p;
+    f.fy = fy_.p;
+    f.lj_part = lj_part_.p;
+    f.espans = espans_.p;
+    f.nespans = (int)espans_h_.size();
+    f.t0 = t0_.p;
+    f.t1 = t1_.p;
+    f.t2 = t2_.p;
+    f.dmbm = dmbm_.p;
+    f.alm = alm_.p;
+    f.model = (model_ == FM_NEO_HOOKEAN) ? 0 : 1;
+    f.x = x_.p;
+    f.y = y_.p;
+    f.cf = cf_.p;
+    f.fea_part = fea_part_.p;
+    f.bspans = bspans_.p;
+    f.nbspans = (int)bspans_h_.size();
+    f.bi = bi_.p;
+    f.bj = bj_.p;
+    f.bk = bk_.p;
+    f.br0 = br0_.p;
+    f.cf_bond0 = 3 * ntri_;
+    f.bond_part = bond_part_.p;
+    f.uid = uid_.p;
+    f.dbg_cnt = dbg_cnt_.p;
+    f.dbg_s1 = dbg_s1_.p;
+    f.dbg_s2 = dbg_s2_.p;
+    f.err = err_.p;
+    e = launch_forces(sigma_, types_, debug, block_, f, stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);
+    return e == hipSuccess ? 0 : -1;
+}
+
+// Per-kernel device time of nsteps step bodies (no graph): HIP events on
+// the engine's stream around each launch.  out[0..4] = kick+bin,
+// scan_reduce, scan_scatter, forces, whole body, in ms per step.
+int Engine::profile(int nsteps, double *out, std::string &err) {
+    if (nsteps <= 0) { err = "nsteps must be > 0"; return FM_ERR_ARG; }
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(5 * nsteps);
+    for (auto &e : ev) {
+        rc = check(hipEventCreate(&e), "event", err);
+        if (rc) return rc;
+    }
+    // the first half-kick of this batch of steps (velocities stay a
+    // consistent KDK state: a final half-kick closes the loop below)
+    if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
+    for (int s = 0; s < nsteps; ++s) {
+        prof_ev_ = &ev[5 * s];
+        int bad = launch_step_body(true, true);
+        prof_ev_ = nullptr;
+        if (bad) return check(hipGetLastError(), "step", err);
+    }
+    if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
+    steps_ += nsteps + 1;
+    rc = sync(err);
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        float t;
+        hipEvent_t *e = &ev[5 * s];
+        (void)hipEventElapsedTime(&t, e[0], e[1]); acc[0] += t;
+        (void)hipEventElapsedTime(&t, e[1], e[2]); acc[1] += t;
+        (void)hipEventElapsedTime(&t, e[2], e[3]); acc[2] += t;
+        (void)hipEventElapsedTime(&t, e[3], e[4]); acc[3] += t;
+        (void)hipEventElapsedTime(&t, e[0], e[4]); acc[4] += t;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 5; ++k) out[k] = acc[k] / nsteps;
+    return rc;
+}
+
+int Engine::ensure_primed(std::string &err) {
+    if (primed_) return FM_OK;
+    if (launch_bin()) return check(hipGetLastError(), "bin", err);
+    if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
+    primed_ = true;
+    return FM_OK;
+}
+
+int Engine::step(int nsteps, std::string &err) {
+    if (nsteps < 0) { err = "nsteps must be >= 0"; return FM_ERR_ARG; }
+    if (nsteps == 0) return FM_OK;
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_primed(err);
+    if (rc) return rc;
+    // femmd2D/src/integrator.cpp:8-29 per step; the second half-kick of
+    // step s is fused with the first half-kick of step s+1
+    if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
+    int rem = nsteps - 1;
+    if (use_graph_ && rem >= kGraphSteps) {
+        if (!graph_exec_) {
+            rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture", err);
+            if (rc) return rc;
+            int bad = 0;
+            for (int s = 0; s < kGraphSteps; ++s) bad |= launch_step_body(true, true);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(stream_, &g);
+            if (bad || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return check(ce != hipSuccess ? ce : hipErrorLaunchFailure, "capture step", err);
+            }
+            graph_ = g;
+            rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate", err);
+            if (rc) { drop_graph(); return rc; }
+            graph_steps_ = kGraphSteps;
+        }
+        while (rem >= graph_steps_) {
+            rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);
+            if (rc) return rc;
+            rem -= graph_steps_;
+        }
+    }
+    while (rem-- > 0)
+        if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+    if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
+    steps_ += nsteps;
+    return sync(err);
+}
+
+int Engine::sync(std::string &err) {
+    int rc = check(hipStreamSynchronize(stream_), "step", err);
+    if (rc) return rc;
+    uint32_t e = errors();
+    if (e) {
+        err = dev_error_text(e);
+        return FM_ERR_DEVICE;
+    }
+    return FM_OK;
+}
+
+uint32_t Engine::errors() {
+    uint32_t e = 0;
+    (void)hipMemcpyAsync(&e, err_.p, sizeof(e), hipMemcpyDeviceToHost, stream_);
+    (void)hipStreamSynchronize(stream_);
+    return e;
+}
+
+int Engine::clear_errors() {
+    (void)hipMemsetAsync(err_.p, 0, sizeof(uint32_t), stream_);
+    (void)hipStreamSynchronize(stream_);
+    return FM_OK;
+}
+
+int Engine::compute_forces(std::string &err) {
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    primed_ = false;
+    rc = ensure_primed(err);
+    if (rc) return rc;
+    KickArgs k{};
+    k.spans = kspans_.p;
+    k.vx = vx_.p;
+    k.vy = vy_.p;
+    k.mass = mass_.p;
+    k.ke_part = ke_part_.p;
+    if (launch_kick(4, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
+        return check(hipGetLastError(), "ke", err);
+    return sync(err);
+}
+
+int Engine::energies(int r, fm_energies *out, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    if (!primed_) {
+        rc = compute_forces(err);
+        if (rc) return rc;
+    }
+    if (launch_energy(R_, rng_.p, ke_part_.p, lj_part_.p, fea_part_.p, bond_part_.p, ene_.p,
+                      stream_) != hipSuccess)
+        return check(hipGetLastError(), "energy", err);
+    ene_h_.resize(8 * R_);
+    rc = check(hipMemcpyAsync(ene_h_.data(), ene_.p, 8 * R_ * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_), "energy copy", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    const double *v = &ene_h_[8 * r];
+    const RepDesc &D = reps_h_[r];
+    const double n = D.p1 - D.p0, V = (double)D.Lx * D.Ly;
+    std::memset(out, 0, sizeof(*out));
+    out->ke = 0.5 * v[0];
+    out->vv_max = v[1];
+    out->u_lj = v[2];
+    out->w_lj = v[3];
+    out->u_fea = v[4];
+    out->w_fea = v[5];
+    out->u_bond = v[6];
+    out->w_bond = v[7];
+    // femmd2D/src/properties.cpp:18-34
+    out->temperature = out->ke / n;
+    out->pressure = 0.5 * v[0] / V + 0.5 * (out->w_lj + out->w_fea) / V;
+    out->steps = steps_;
+    out->n = (int64_t)n;
+    return FM_OK;
+}
+
+int Engine::get_state(int r, float *x, float *y, float *vx, float *vy, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    const RepDesc &D = reps_h_[r];
+    const int n = D.p1 - D.p0;
+    std::vector<float> buf(n);
+    float *outs[4] = {x, y, vx, vy};
+    float *srcs[4] = {x_.p, y_.p, vx_.p, vy_.p};
+    const std::vector<int32_t> &iid = iid_h_[r];
+    for (int q = 0; q < 4; ++q) {
+        if (!outs[q]) continue;
+        int rc = check(hipMemcpyAsync(buf.data(), srcs[q] + D.p0, n * sizeof(float),
+                                      hipMemcpyDeviceToHost, stream_), "get_state", err);
+        if (rc) return rc;
+        rc = check(hipStreamSynchronize(stream_), "get_state", err);
+        if (rc) return rc;
+        for (int u = 0; u < n; ++u) outs[q][u] = buf[iid[u] - D.p0];
+    }
+    return FM_OK;
+}
+
+int Engine::set_state(int r, const float *x, const float *y, const float *vx, const float *vy,
+                      std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    const RepDesc &D = reps_h_[r];
+    const int n = D.p1 - D.p0;
+    std::vector<float> buf(n);
+    const float *ins[4] = {x, y, vx, vy};
+    float *dsts[4] = {x_.p, y_.p, vx_.p, vy_.p};
+    const std::vector<int32_t> &iid = iid_h_[r];
+    for (int q = 0; q < 4; ++q) {
+        if (!ins[q]) continue;
+        for (int u = 0; u < n; ++u) {
+            float v = ins[q][u];
+            if (q == 0) v = wrap_host(v, D.Lx);
+            if (q == 1) v = wrap_host(v, D.Ly);
+            buf[iid[u] - D.p0] = v;
+        }
+        int rc = check(hipMemcpyAsync(dsts[q] + D.p0, buf.data(), n * sizeof(float),
+                                      hipMemcpyHostToDevice, stream_), "set_state", err);
+        if (rc) return rc;
+        rc = check(hipStreamSynchronize(stream_), "set_state", err);
+        if (rc) return rc;
+    }
+    if (x || y) primed_ = false;
+    return FM_OK;
+}
+
+int Engine::get_forces(int r, float *fx, float *fy, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    if (launch_total_force(N_, fx_.p, fy_.p, cf_.p, gather_ ? goff_.p : nullptr, gent_.p,
+                           outx_.p, outy_.p, stream_) != hipSuccess)
+        return check(hipGetLastError(), "total force", err);
+    const RepDesc &D = reps_h_[r];
+    const int n = D.p1 - D.p0;
+    std::vector<float> bx(n), by(n);
+    rc = check(hipMemcpyAsync(bx.data(), outx_.p + D.p0, n * sizeof(float), hipMemcpyDeviceToHost,
+                              stream_), "forces", err);
+    if (rc) return rc;
+    rc = check(hipMemcpyAsync(by.data(), outy_.p + D.p0, n * sizeof(float), hipMemcpyDeviceToHost,
+                              stream_), "forces", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    const std::vector<int32_t> &iid = iid_h_[r];
+    for (int u = 0; u < n; ++u) {
+        if (fx) fx[u] = bx[iid[u] - D.p0];
+        if (fy) fy[u] = by[iid[u] - D.p0];
+    }
+    return FM_OK;
+}
+
+int Engine::get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *start,
+                      int32_t *perm, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    const RepDesc &D = reps_h_[r];
+    if (ncx) *ncx = D.ncx;
+    if (ncy) *ncy = D.ncy;
+    const int nc = D.ncx * D.ncy, n = D.p1 - D.p0;
+    std::vector<int32_t> st(nc + 1), pm(n);
+    rc = check(hipMemcpyAsync(st.data(), start_.p + D.cell0, (nc + 1) * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream_), "cells", err);
+    if (rc) return rc;
+    rc = check(hipMemcpyAsync(pm.data(), perm_.p + D.p0, n * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream_), "cells", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    for (int c = 0; c <= nc; ++c) {
+        if (start) start[c] = st[c] - D.p0;
+    }
+    for (int c = 0; c < nc; ++c) {
+        for (int p = st[c]; p < st[c + 1]; ++p) {
+            const int internal = pm[p - D.p0];
+            const int u = uid_h_[internal];
+            if (key) key[u] = c;
+            if (perm) perm[p - D.p0] = u;
+        }
+    }
+    return FM_OK;
+}
+
+int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err) {
+    if (R_ != 1 || r != 0) { err = "pair_sets is available for single systems"; return FM_ERR_STATE; }
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    hipError_t he;
+    if ((he = dbg_cnt_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
+    if ((he = dbg_s1_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
+    if ((he = dbg_s2_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
+    // rebin (the debug pass must see the binning of the current positions)
+    if (launch_bin()) return check(hipGetLastError(), "bin", err);
+    if (launch_forces_only(true)) return check(hipGetLastError(), "pair sets", err);
+    std::vector<long long> a(N_), b(N_);
+    rc = check(hipMemcpyAsync(cnt, dbg_cnt_.p, N_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
+               "copy", err);
+    if (rc) return rc;
+    rc = check(hipMemcpyAsync(a.data(), dbg_s1_.p, N_ * sizeof(long long), hipMemcpyDeviceToHost,
+                              stream_), "copy", err);
+    if (rc) return rc;
+    rc = check(hipMemcpyAsync(b.data(), dbg_s2_.p, N_ * sizeof(long long), hipMemcpyDeviceToHost,
+                              stream_), "copy", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    for (int i = 0; i < N_; ++i) {
+        s1[i] = a[i];
+        s2[i] = b[i];
+    }
+    dbg_cnt_.release();
+    dbg_s1_.release();
+    dbg_s2_.release();
+    return FM_OK;
+}
+
+int Engine::time_forces(int reps, double *ms, std::string &err) {
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, stream_);
+    for (int k = 0; k < reps; ++k) {
+        if (launch_bin()) return check(hipGetLastError(), "bin", err);
+        if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
+    }
+    (void)hipEventRecord(b, stream_);
+    rc = check(hipEventSynchronize(b), "time", err);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t / std::max(reps, 1);
+    if (rc) return rc;
+    return sync(err);
+}
+
+}  // namespace fm

@@ -1,0 +1,172 @@
+// engine.hpp -- host side of the step engine (C++17, HIP runtime).
+//
+// HostSystem is the caller-ordered description of one system, filled by the
+// fm_add_* calls (the reference's Particles / Bonds / FeaElements host
+// vectors, femmd2D/src/particles.h:41-126).  Engine owns the device image of
+// one or more HostSystems (a batch of replicas; a plain system is a batch of
+// one) and runs the step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "../../include/femmd.h"
+
+namespace fm {
+
+struct HostSystem {
+    double Lx = 0, Ly = 0, dt = 0.005;
+    int device = 0;
+    // particles, caller order
+    std::vector<float> x, y, vx, vy, mass, sigma;
+    std::vector<int32_t> type, excl;
+    std::vector<float> rx, ry;  // reference (unwrapped) positions
+    // pair table (reference default: WCA, femmd2D/src/interactions.cpp:25-28)
+    int32_t ntypes = 1;
+    std::vector<float> eps{1.0f}, rc{1.12246204830937f}, ush{1.0f};
+    bool lj_on = true;
+    // elements
+    std::vector<int32_t> tri;            // 3 per element, caller ids
+    std::vector<float> tE, tnu;          // per element, < 0 = global
+    int32_t model = FM_NEO_HOOKEAN;
+    float E = 0.0f, nu = 0.3f;
+    // bonds
+    std::vector<int32_t> bi, bj;
+    std::vector<float> bk, br0;
+    int32_t next_group = 0;
+    // launch tuning
+    int32_t tile_w = 0, block = 256, use_graph = 1;
+};
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, count * sizeof(T));
+    }
+};
+
+class Engine {
+public:
+    ~Engine();
+    // Builds the device image of the given systems.  On failure returns a
+    // negative FM_ERR_* and fills err.
+    static int build(const std::vector<const HostSystem *> &reps, std::unique_ptr<Engine> &out,
+                     std::string &err);
+
+    int step(int nsteps, std::string &err);
+    int compute_forces(std::string &err);
+    int energies(int r, fm_energies *e, std::string &err);
+    int get_state(int r, float *x, float *y, float *vx, float *vy, std::string &err);
+    int set_state(int r, const float *x, const float *y, const float *vx, const float *vy,
+                  std::string &err);
+    int get_forces(int r, float *fx, float *fy, std::string &err);
+    int get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *start,
+                  int32_t *perm, std::string &err);
+    int pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err);
+    int time_forces(int reps, double *ms, std::string &err);
+    int profile(int nsteps, double *out, std::string &err);
+    int set_tuning(int tile_w, int block, int use_graph, std::string &err);
+    uint32_t errors();
+    int clear_errors();
+    int sync(std::string &err);
+    int nrep() const { return R_; }
+    int nparticles(int r) const { return reps_h_[r].p1 - reps_h_[r].p0; }
+    int64_t steps() const { return steps_; }
+    // bucket tuning for batches: per replica tile width
+    int retile(const std::vector<int> &tile_w, int block, std::string &err);
+    int ncells_of(int r) const { return reps_h_[r].ncx * reps_h_[r].ncy; }
+    int set_dt(int r, double dt, std::string &err);
+    const RepDesc &rep(int r) const { return reps_h_[r]; }
+    int tile_width(int r) const { return tile_w_[r]; }
+    int block() const { return block_; }
+    double density_of(int r) const;
+
+private:
+    Engine() = default;
+    int launch_bin();
+    int launch_step_body(bool kick2, bool drift);
+    int launch_forces_only(bool debug);
+    int build_tiles(const std::vector<int> &tile_w, std::string &err);
+    int ensure_primed(std::string &err);
+    int check(hipError_t e, const char *what, std::string &err);
+    int drop_graph();
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    int R_ = 0, N_ = 0, ncells_ = 0, ntri_ = 0, nbond_ = 0, cap_ = 32, lds_cap_ = 2048;
+    int block_ = 256, use_graph_ = 1;
+    bool sigma_ = false, types_ = false, gather_ = false;
+    bool primed_ = false;
+    std::vector<char> lj_on_;
+    int64_t steps_ = 0;
+    int model_ = 0;
+    double rho_cp_ = 1.16;
+
+    std::vector<RepDesc> reps_h_;
+    std::vector<int32_t> uid_h_;               // internal -> caller id (per replica)
+    std::vector<std::vector<int32_t>> iid_h_;  // per replica: caller -> internal
+    std::vector<int> tile_w_;                  // per replica tile width in use
+    std::vector<Span> kspans_h_, espans_h_, bspans_h_;
+    std::vector<Tile> tiles_h_;
+    std::vector<int32_t> rng_h_;  // per replica 8 ints (energy ranges)
+    std::vector<double> ene_h_;
+
+    DBuf<RepDesc> reps_;
+    DBuf<PairEntry> ptab_;
+    DBuf<float> x_, y_, vx_, vy_, fx_, fy_, mass_, inv_m_, sig_;
+    DBuf<int32_t> grp_, typ_, uid_;
+    DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_;
+    DBuf<float4> sorted_;
+    DBuf<Span> kspans_, espans_, bspans_;
+    DBuf<Tile> tiles_;
+    DBuf<int32_t> t0_, t1_, t2_, bi_, bj_, goff_, gent_;
+    DBuf<float4> dmbm_;
+    DBuf<float2> alm_, cf_;
+    DBuf<float> bk_, br0_;
+    DBuf<double> ke_part_, lj_part_, fea_part_, bond_part_, ene_;
+    DBuf<int32_t> rng_;
+    DBuf<uint32_t> err_;
+    DBuf<float> outx_, outy_;
+    DBuf<int32_t> dbg_cnt_;
+    DBuf<long long> dbg_s1_, dbg_s2_;
+
+    hipEvent_t *prof_ev_ = nullptr;  // set only while profiling
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_steps_ = 0;
+};
+
+// launchers (kernels.hip)
+hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a, hipStream_t s);
+hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s);
+hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s);
+hipError_t launch_forces(bool sigma, bool types, bool debug, int block, const ForceArgs &a,
+                         hipStream_t s);
+size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
+hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
+                              const int32_t *goff, const int32_t *gent, float *ox, float *oy,
+                              hipStream_t s);
+hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
+                         const double *fea, const double *bond, double *out, hipStream_t s);
+hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
+
+}  // namespace fm

@@ -1,0 +1,874 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the femmd2D step.
+//
+// Per step (one HIP graph node each):
+//   k_kick      second half-kick of the previous step (+ KE partials),
+//               first half-kick, drift, periodic wrap, cell key, bin
+//               (integer atomic slot per cell into a capacity-padded list)
+//   k_scan_reduce / k_scan_scatter
+//               counting sort: exclusive scan of the cell counts into
+//               cell_start, each cell's list sorted by particle index into
+//               perm, and positions gathered into sorted order
+//   k_forces    one launch with three workgroup roles:
+//               LJ cell tiles (LDS-staged 3x3 cells), CST elements and
+//               harmonic bonds (corner forces into a buffer, no atomics)
+// and the gather of element / bond corner forces happens inside the next
+// k_kick through a per-particle CSR, in a fixed order.
+//
+// The file is compiled with -ffp-contract=off: every FP32 expression that
+// must agree bit for bit with the reference's real=float arithmetic (wrap,
+// cell key, minimum image, cutoff predicate) is evaluated in its source
+// order; fused multiply-adds are written explicitly where wanted.
+#include "device.hpp"
+
+namespace fm {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// femmd2D/src/def.h:112 pbcCalc: r - floor(r / L) * L, FP32, IEEE division.
+__device__ __forceinline__ float wrap1(float r, float L) {
+    float q = r / L;
+    float f = floorf(q);
+    float p = f * L;
+    return r - p;
+}
+
+// femmd2D/src/nnCuda.cu:9-17 cellId: ix = (int)(x * ncx / Lx), ix == ncx
+// clamped to ncx - 1.  The extra clamp into [0, ncx) only matters for
+// non-finite input, which is flagged separately.
+__device__ __forceinline__ int cell_key(float x, float y, const RepDesc &R) {
+    float tx = x * R.fncx;
+    tx = tx / R.Lx;
+    int ix = (int)tx;
+    if (ix == R.ncx) ix -= 1;
+    float ty = y * R.fncy;
+    ty = ty / R.Ly;
+    int iy = (int)ty;
+    if (iy == R.ncy) iy -= 1;
+    ix = min(max(ix, 0), R.ncx - 1);
+    iy = min(max(iy, 0), R.ncy - 1);
+    return ix + iy * R.ncx;
+}
+
+// p - q + k L for k in {-1, 0, 1}, with the coordinate near L moved by L
+// first: that move is exact (Sterbenz), so only one rounding remains at the
+// scale of the small difference.  Used for force values across the
+// periodic boundary; the image decision itself is taken by the caller.
+__device__ __forceinline__ float shifted_diff(float p, float q, int k, float L) {
+    if (k > 0) return p - (q - L);
+    if (k < 0) return (p - L) - q;
+    return p - q;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+// Inclusive prefix sum over the 64 lanes of a wavefront.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Deterministic block reduction of two doubles (fixed wave order); the
+// result is valid in thread 0.  red must hold 2 * (blockDim / 64) doubles.
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int k = 0; k < nw; ++k) { sa += red[2 * k]; sb += red[2 * k + 1]; }
+        a = sa;
+        b = sb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_kick: integrator + binning (femmd2D/src/integratorCuda.cu:110-206,
+// femmd2D/src/nnCuda.cu:69-82 re-designed).  One workgroup covers one span
+// of one replica.
+//   kKick2: v += dt/2 a (second half of the previous step), KE partials
+//   kDrift: v += dt/2 a, x += dt v, wrap, key, bin
+// a = (f_pair + sum of gathered corner forces) * inv_m.
+// ---------------------------------------------------------------------------
+template <bool kKick2, bool kDrift, bool kGather>
+__global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
+    __shared__ double red[2 * (kKickThreads / 64)];
+    __shared__ float redm[kKickThreads / 64];
+    const Span sp = a.spans[blockIdx.x];
+    const RepDesc R = a.reps[sp.rep];
+    const int p = sp.begin + (int)threadIdx.x;
+    double ke = 0.0;
+    float vvmax = 0.0f;
+    if (p < sp.end) {
+        float fx = a.fx[p], fy = a.fy[p];
+        if (kGather) {
+            const int e0 = a.goff[p], e1 = a.goff[p + 1];
+            for (int e = e0; e < e1; ++e) {
+                const float2 c = a.cf[a.gent[e]];
+                fx += c.x;
+                fy += c.y;
+            }
+        }
+        const float im = a.inv_m[p];
+        const float ax = fx * im, ay = fy * im;
+        float vx = a.vx[p], vy = a.vy[p];
+        if (kKick2) {
+            vx = vx + R.hdt * ax;
+            vy = vy + R.hdt * ay;
+            const float vv = vx * vx + vy * vy;
+            ke = (double)(a.mass[p] * vv);
+            vvmax = vv;
+        }
+        if (kDrift) {
+            vx = vx + R.hdt * ax;
+            vy = vy + R.hdt * ay;
+            float x = a.x[p] + R.dt * vx;
+            float y = a.y[p] + R.dt * vy;
+            x = wrap1(x, R.Lx);
+            y = wrap1(y, R.Ly);
+            if (!(isfinite(x) && isfinite(y))) {
+                atomicOr(a.err, 8u);  // FM_DEV_NONFINITE
+                x = 0.0f;
+                y = 0.0f;
+            }
+            a.x[p] = x;
+            a.y[p] = y;
+            const int c = R.cell0 + cell_key(x, y, R);
+            const int slot = atomicAdd(&a.count[c], 1);
+            if (slot < a.cap)
+                a.tmp[(size_t)c * a.cap + slot] = p;
+            else
+                atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW
+        }
+        a.vx[p] = vx;
+        a.vy[p] = vy;
+    }
+    if (kKick2) {
+        double vm = 0.0;
+        // max via a float wave reduction (exact, order independent)
+        float m = wave_max(vvmax);
+        int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) redm[w] = m;
+        block_sum2(ke, vm, red);
+        if (threadIdx.x == 0) {
+            float mm = 0.0f;
+            for (int k = 0; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redm[k]);
+            a.ke_part[2 * blockIdx.x] = ke;
+            a.ke_part[2 * blockIdx.x + 1] = (double)mm;
+        }
+    }
+}
+
+// Kinetic-energy partials of the current velocities (no kick).
+__global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
+    __shared__ double red[2 * (kKickThreads / 64)];
+    __shared__ float redm[kKickThreads / 64];
+    const Span sp = a.spans[blockIdx.x];
+    const int p = sp.begin + (int)threadIdx.x;
+    double ke = 0.0, vm = 0.0;
+    float vvmax = 0.0f;
+    if (p < sp.end) {
+        const float vx = a.vx[p], vy = a.vy[p];
+        const float vv = vx * vx + vy * vy;
+        ke = (double)(a.mass[p] * vv);
+        vvmax = vv;
+    }
+    float m = wave_max(vvmax);
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    block_sum2(ke, vm, red);
+    if (threadIdx.x == 0) {
+        float mm = 0.0f;
+        for (int k = 0; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redm[k]);
+        a.ke_part[2 * blockIdx.x] = ke;
+        a.ke_part[2 * blockIdx.x + 1] = (double)mm;
+    }
+}
+
+// Binning of the current positions without moving them (used after
+// fm_set_state / fm_commit and by fm_compute_forces).
+__global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
+    const Span sp = a.spans[blockIdx.x];
+    const RepDesc R = a.reps[sp.rep];
+    const int p = sp.begin + (int)threadIdx.x;
+    if (p >= sp.end) return;
+    float x = a.x[p], y = a.y[p];
+    if (!(isfinite(x) && isfinite(y))) {
+        atomicOr(a.err, 8u);
+        x = 0.0f;
+        y = 0.0f;
+    }
+    const int c = R.cell0 + cell_key(x, y, R);
+    const int slot = atomicAdd(&a.count[c], 1);
+    if (slot < a.cap)
+        a.tmp[(size_t)c * a.cap + slot] = p;
+    else
+        atomicOr(a.err, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort over all cells of all replicas.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
+    __shared__ int red[kScanThreads / 64];
+    const int base = blockIdx.x * kScanCells + 4 * (int)threadIdx.x;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int c = base + q;
+        if (c < a.ncells) s += min(a.count[c], a.cap);
+    }
+    s = wave_sum_int(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < kScanThreads / 64; ++k) t += red[k];
+        a.bsum[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan -> cell_start; each cell's slot list sorted by particle
+// index (the in-cell order of femmd2D/src/nn.cpp:8-20) -> perm; positions
+// gathered into sorted order; counters zeroed for the next binning.
+__global__ __launch_bounds__(kScanThreads) void k_scan_scatter(ScanArgs a) {
+    extern __shared__ int sbuf[];  // kScanThreads * (cap + 1)
+    __shared__ int red[kScanThreads / 64];
+    __shared__ int wsum[kScanThreads / 64];
+    __shared__ int s_pre;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // prefix of all earlier scan workgroups
+    int pre = 0;
+    for (int k = t; k < (int)blockIdx.x; k += kScanThreads) pre += a.bsum[k];
+    pre = wave_sum_int(pre);
+    if (lane == 0) red[w] = pre;
+    const int base = blockIdx.x * kScanCells + 4 * t;
+    int n[4];
+    int tsum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int c = base + q;
+        n[q] = (c < a.ncells) ? min(a.count[c], a.cap) : 0;
+        tsum += n[q];
+    }
+    // block exclusive scan of the per-thread sums
+    int incl = wave_incl_scan(tsum);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (t == 0) {
+        int p = 0;
+        for (int k = 0; k < kScanThreads / 64; ++k) p += red[k];
+        s_pre = p;
+    }
+    __syncthreads();
+    int wbase = 0;
+    for (int k = 0; k < w; ++k) wbase += wsum[k];
+    int run = s_pre + wbase + incl - tsum;
+    int *mine = sbuf + t * (a.cap + 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = base + q;
+        if (c >= a.ncells) break;
+        a.start[c] = run;
+        const int m = n[q];
+        const int *src = a.tmp + (size_t)c * a.cap;
+        // insertion sort of the cell's particle indices (ascending)
+        for (int k = 0; k < m; ++k) {
+            int v = src[k];
+            int j = k - 1;
+            while (j >= 0 && mine[j] > v) {
+                mine[j + 1] = mine[j];
+                --j;
+            }
+            mine[j + 1] = v;
+        }
+        for (int k = 0; k < m; ++k) {
+            const int i = mine[k];
+            const int pos = run + k;
+            if (pos >= a.n) break;  // only after an aborted binning sequence
+            a.perm[pos] = i;
+            float4 e;
+            e.x = a.x[i];
+            e.y = a.y[i];
+            e.z = __int_as_float(a.grp[i]);
+            e.w = a.sig ? a.sig[i] : 0.0f;
+            a.sorted[pos] = e;
+            if (a.typ) a.sorted_typ[pos] = a.typ[i];
+        }
+        a.count[c] = 0;
+        run += m;
+        if (c == a.ncells - 1) a.start[a.ncells] = run;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Force pass.
+// ---------------------------------------------------------------------------
+
+// Workgroup id -> tile id so that each XCD (workgroups are dealt round-robin
+// over the 8 XCDs) walks a contiguous band of cell rows, keeping the
+// neighbouring rows it re-reads in its own L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int nx = 8;
+    const int q = n / nx, r = n % nx;
+    const int x = b % nx, k = b / nx;
+    return x * q + min(x, r) + k;
+}
+
+struct PairLds {
+    float4 *ent;   // staged particles {x, y, grp bits, sigma}
+    int *typ;      // staged types (typed path)
+    int *sp;       // staged sorted positions (diagnostic path)
+    int *cstart;   // per slot: first sorted position
+    int *off;      // per slot: first LDS entry (S + 1)
+    PairEntry *pt; // pair table of the replica
+    double *red;
+};
+
+// LJ over one tile (femmd2D/src/interactionsCuda.cu:43-100 pair maths,
+// full-list accumulation of femmd2D/src/interactions.cpp:347-389: each
+// ordered pair adds to f_i only, U and W take 1/2).
+template <bool kWrap, bool kSigma, bool kTypes, bool kDebug>
+__device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
+                               int tile, PairLds &L) {
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const int nt = R.ntypes;
+    double Ublk = 0.0, Wblk = 0.0;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int me = c_lo + k;
+        // slot of this centre particle within row 1 (1..W)
+        int lo = row + 1, hi = row + W + 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= me) lo = mid; else hi = mid;
+        }
+        const int col = lo - row;
+        const float4 pi = L.ent[me];
+        const float xi = pi.x, yi = pi.y;
+        const float xim = xi - Lx, yim = yi - Ly;
+        const int gi = __float_as_int(pi.z);
+        const int gix = gi >= 0 ? gi : -2;  // -1 never matches -2
+        const float si = pi.w;
+        const int ti = kTypes ? L.typ[me] : 0;
+        const int a0 = L.off[col - 1], b0 = L.off[col + 2];
+        const int a1 = L.off[row + col - 1], b1 = L.off[row + col + 2];
+        const int a2 = L.off[2 * row + col - 1], b2 = L.off[2 * row + col + 2];
+        const int total = (b0 - a0) + (b1 - a1) + (b2 - a2);
+        int j = a0;
+        if (j == b0) j = a1;
+        if (j == b1) j = a2;
+        float fx = 0.f, fy = 0.f, uacc = 0.f, vir = 0.f, ush = 0.f;
+        int nint = 0;
+        long long s1 = 0, s2 = 0;
+        for (int t = 0; t < total; ++t) {
+            const float4 pj = L.ent[j];
+            const int gj = __float_as_int(pj.z);
+            // femmd2D/src/interactionsCuda.cu:84-90, FP32, reference order
+            float dx = xi - pj.x;
+            float dy = yi - pj.y;
+            int sx = 0, sy = 0;
+            if (kWrap) {
+                if (dx >= hLx) { dx -= Lx; sx = 1; } else if (dx < -hLx) { dx += Lx; sx = -1; }
+                if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
+            }
+            const float dr2 = dx * dx + dy * dy;
+            float rc2, s2ij, e48 = 0.f, e4 = 0.f, shift = 0.f;
+            if (kTypes) {
+                const PairEntry pe = L.pt[ti * nt + (kTypes ? L.typ[j] : 0)];
+                e48 = pe.e48; e4 = pe.e4; shift = pe.shift;
+                const float sij = kSigma ? (si + pj.w) * 0.5f : si;
+                rc2 = pe.rc * pe.rc * sij * sij;
+                s2ij = sij * sij;
+            } else if (kSigma) {
+                const float sij = (si + pj.w) * 0.5f;
+                const float rc = L.pt[0].rc;
+                rc2 = rc * rc * sij * sij;
+                s2ij = sij * sij;
+            } else {
+                rc2 = R.rc2_uniform;
+                s2ij = R.s2_uniform;
+            }
+            const bool hit = (dr2 < rc2) & (j != me) & (gj != gix);
+            if (hit) {
+                // force from the image-exact difference: the far coordinate
+                // is shifted by L before subtracting (exact by Sterbenz)
+                float dxa = dx, dya = dy;
+                if (kWrap) {
+                    if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
+                    if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);
+                }
+                const float r2a = __builtin_fmaf(dxa, dxa, dya * dya);
+                const float inv = __builtin_amdgcn_rcpf(r2a);
+                const float sr2 = s2ij * inv;
+                const float sr6 = sr2 * sr2 * sr2;
+                const float fcv = sr6 * (sr6 - 0.5f) * inv;
+                const float fc = kTypes ? e48 * fcv : fcv;
+                fx = __builtin_fmaf(fc, dxa, fx);
+                fy = __builtin_fmaf(fc, dya, fy);
+                const float ue = sr6 * (sr6 - 1.0f);
+                if (kTypes) {
+                    uacc = __builtin_fmaf(e4, ue, uacc);
+                    ush += shift;
+                } else {
+                    uacc += ue;
+                }
+                vir = __builtin_fmaf(fc, r2a, vir);
+                ++nint;
+                if (kDebug) {
+                    const long long v = (long long)a.uid[a.perm[L.sp[j]]] + 1;
+                    s1 += v;
+                    s2 += v * v;
+                }
+            }
+            ++j;
+            if (j == b0) j = a1;
+            if (j == b1) j = a2;
+        }
+        if (!kTypes) {
+            const PairEntry pe = L.pt[0];
+            fx *= pe.e48;
+            fy *= pe.e48;
+            vir *= pe.e48;
+            uacc = pe.e4 * uacc;
+            ush = (float)nint * pe.shift;
+        }
+        const int id = a.perm[sbase + k];
+        a.fx[id] = fx;
+        a.fy[id] = fy;
+        Ublk += 0.5 * ((double)uacc + (double)ush);
+        Wblk += 0.5 * (double)vir;
+        if (kDebug) {
+            const int u = a.uid[id];
+            a.dbg_cnt[u] = nint;
+            a.dbg_s1[u] = s1;
+            a.dbg_s2[u] = s2;
+        }
+    }
+    block_sum2(Ublk, Wblk, L.red);
+    if (threadIdx.x == 0) {
+        a.lj_part[2 * tile] = Ublk;
+        a.lj_part[2 * tile + 1] = Wblk;
+    }
+}
+
+template <bool kSigma, bool kTypes, bool kDebug>
+__device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    const bool wrapT = (T.w_flags >> 16) & 1;
+    const int row = W + 2, S = 3 * row;
+    const int tid = threadIdx.x;
+    // slot table: rows cy-1, cy, cy+1; columns cx0-1 .. cx0+W (periodic,
+    // femmd2D/src/nnCuda.cu:26-37)
+    for (int s = tid; s < S; s += blockDim.x) {
+        const int r = s / row, col = s - r * row;
+        int cy = T.cy + r - 1;
+        if (cy < 0) cy += R.ncy; else if (cy >= R.ncy) cy -= R.ncy;
+        int cx = T.cx0 + col - 1;
+        if (cx < 0) cx += R.ncx; else if (cx >= R.ncx) cx -= R.ncx;
+        const int c = R.cell0 + cy * R.ncx + cx;
+        const int st = a.start[c];
+        L.cstart[s] = st;
+        L.off[s + 1] = a.start[c + 1] - st;  // count, scanned below
+    }
+    for (int k = tid; k < R.ntypes * R.ntypes; k += blockDim.x) L.pt[k] = a.ptab[R.pt + k];
+    __syncthreads();
+    // exclusive scan of S <= 198 counts by the first wavefront (4 per lane)
+    if (tid < 64) {
+        int v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = 4 * tid + q;
+            v[q] = (idx < S) ? L.off[idx + 1] : 0;
+            s += v[q];
+        }
+        const int incl = wave_incl_scan(s);
+        int run = incl - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = 4 * tid + q;
+            if (idx < S) {
+                L.off[idx] = run;
+                run += v[q];
+            }
+        }
+        if (tid == 63) L.off[S] = incl;
+    }
+    __syncthreads();
+    const int M = L.off[S];
+    if (M > a.lds_cap) {
+        if (tid == 0) {
+            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+            a.lj_part[2 * tile] = 0.0;
+            a.lj_part[2 * tile + 1] = 0.0;
+        }
+        return;
+    }
+    // stage the 3 x (W+2) cells: sorted positions are contiguous per cell
+    for (int e = tid; e < M; e += blockDim.x) {
+        int lo = 0, hi = S;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= e) lo = mid; else hi = mid;
+        }
+        const int p = L.cstart[lo] + (e - L.off[lo]);
+        L.ent[e] = a.sorted[p];
+        if (kTypes) L.typ[e] = a.sorted_typ[p];
+        if (kDebug) L.sp[e] = p;
+    }
+    __syncthreads();
+    if (wrapT)
+        pair_tile_body<true, kSigma, kTypes, kDebug>(a, R, W, tile, L);
+    else
+        pair_tile_body<false, kSigma, kTypes, kDebug>(a, R, W, tile, L);
+}
+
+// CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
+// displacement form: with U = Ds - Dm and G = U Bm = F - I, every term of P
+// is O(|G|), so FP32 keeps relative accuracy at small strain.  Corner forces
+// go to cf[3e + k]; no atomics.
+__device__ void elem_block(const ForceArgs &a, int b, double *red) {
+    const Span sp = a.espans[b];
+    const RepDesc R = a.reps[sp.rep];
+    double U = 0.0, Wv = 0.0;
+    for (int e = sp.begin + (int)threadIdx.x; e < sp.end; e += blockDim.x) {
+        const int n0 = a.t0[e], n1 = a.t1[e], n2 = a.t2[e];
+        const float x0 = a.x[n0], y0 = a.y[n0];
+        const float x1 = a.x[n1], y1 = a.y[n1];
+        const float x2 = a.x[n2], y2 = a.y[n2];
+        // half-box unwrap of nodes 1, 2 against node 0
+        // (femmd2D/src/interactions.cpp:554-568): shifts s1, s2 in {-1,0,1}
+        const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+        const float dx01 = x0 - x1, dx02 = x0 - x2, dy01 = y0 - y1, dy02 = y0 - y2;
+        const int s1x = (dx01 > hLx) - (dx01 < -hLx);
+        const int s2x = (dx02 > hLx) - (dx02 < -hLx);
+        const int s1y = (dy01 > hLy) - (dy01 < -hLy);
+        const int s2y = (dy02 > hLy) - (dy02 < -hLy);
+        const float Ds00 = shifted_diff(x0, x2, -s2x, Lx);
+        const float Ds10 = shifted_diff(y0, y2, -s2y, Ly);
+        const float Ds01 = shifted_diff(x1, x2, s1x - s2x, Lx);
+        const float Ds11 = shifted_diff(y1, y2, s1y - s2y, Ly);
+        const float4 Dm = a.dmbm[2 * e];
+        const float4 B = a.dmbm[2 * e + 1];
+        const float2 alm = a.alm[e];
+        const float al = alm.x, am = alm.y;
+        // Dm = {Dm00, Dm01, Dm10, Dm11}; B = {B00, B01, B10, B11}
+        const float U00 = Ds00 - Dm.x, U01 = Ds01 - Dm.y;
+        const float U10 = Ds10 - Dm.z, U11 = Ds11 - Dm.w;
+        const float G00 = U00 * B.x + U01 * B.z;
+        const float G10 = U10 * B.x + U11 * B.z;
+        const float G01 = U00 * B.y + U01 * B.w;
+        const float G11 = U10 * B.y + U11 * B.w;
+        const float F00 = 1.0f + G00, F11 = 1.0f + G11;
+        float P00, P01, P10, P11, ue;
+        const float trG = G00 + G11;
+        const float detG = G00 * G11 - G01 * G10;
+        const float tJ = trG + detG;  // J - 1
+        if (!(tJ > -1.0f)) atomicOr(a.err, 4u);  // FM_DEV_INVERTED
+        if (a.model == 0) {
+            // Neo-Hookean: P = [mu (J F - cof F) + lambda lnJ cof F] / J
+            const float lJ = log1pf(tJ);
+            const float iJ = 1.0f / (1.0f + tJ);
+            const float A00 = (G00 - G11) + tJ * F00;
+            const float A01 = G01 + tJ * G01 + G10;
+            const float A10 = G10 + tJ * G10 + G01;
+            const float A11 = (G11 - G00) + tJ * F11;
+            const float ll = al * lJ;
+            P00 = (am * A00 + ll * F11) * iJ;
+            P01 = (am * A01 - ll * G10) * iJ;
+            P10 = (am * A10 - ll * G01) * iJ;
+            P11 = (am * A11 + ll * F00) * iJ;
+            const float g2 = G00 * G00 + G01 * G01 + G10 * G10 + G11 * G11;
+            ue = am * (trG - lJ) + 0.5f * am * g2 + 0.5f * al * lJ * lJ;
+        } else {
+            // SVK: E = (G + G^T + G^T G)/2, S = 2 mu E + lambda tr(E) I, P = F S
+            const float E00 = G00 + 0.5f * (G00 * G00 + G10 * G10);
+            const float E11 = G11 + 0.5f * (G01 * G01 + G11 * G11);
+            const float E01 = 0.5f * (G01 + G10) + 0.5f * (G00 * G01 + G10 * G11);
+            const float tr = E00 + E11;
+            const float S00 = 2.0f * am * E00 + al * tr;
+            const float S11 = 2.0f * am * E11 + al * tr;
+            const float S01 = 2.0f * am * E01;
+            P00 = F00 * S00 + G01 * S01;
+            P01 = F00 * S01 + G01 * S11;
+            P10 = G10 * S00 + F11 * S01;
+            P11 = G10 * S01 + F11 * S11;
+            ue = am * (E00 * E00 + E11 * E11 + 2.0f * E01 * E01) + 0.5f * al * tr * tr;
+        }
+        // H = -P' Bm^T (P' = A0 P, A0 folded into al, am)
+        const float f0x = -(P00 * B.x + P01 * B.y);
+        const float f0y = -(P10 * B.x + P11 * B.y);
+        const float f1x = -(P00 * B.z + P01 * B.w);
+        const float f1y = -(P10 * B.z + P11 * B.w);
+        a.cf[3 * e] = make_float2(f0x, f0y);
+        a.cf[3 * e + 1] = make_float2(f1x, f1y);
+        a.cf[3 * e + 2] = make_float2(-f0x - f1x, -f0y - f1y);
+        U += (double)ue;
+        Wv += (double)(f0x * Ds00 + f0y * Ds10) + (double)(f1x * Ds01 + f1y * Ds11);
+    }
+    block_sum2(U, Wv, red);
+    if (threadIdx.x == 0) {
+        a.fea_part[2 * b] = U;
+        a.fea_part[2 * b + 1] = Wv;
+    }
+}
+
+// Harmonic bonds (femmd2D/src/interactions.cpp:143-173, per-bond k, r0).
+__device__ void bond_block(const ForceArgs &a, int b, double *red) {
+    const Span sp = a.bspans[b];
+    const RepDesc R = a.reps[sp.rep];
+    double U = 0.0, Wv = 0.0;
+    for (int l = sp.begin + (int)threadIdx.x; l < sp.end; l += blockDim.x) {
+        const int i = a.bi[l], j = a.bj[l];
+        const float xi = a.x[i], yi = a.y[i], xj = a.x[j], yj = a.y[j];
+        float dx = xi - xj, dy = yi - yj;
+        const int sx = (dx >= R.hLx) - (dx < -R.hLx);
+        const int sy = (dy >= R.hLy) - (dy < -R.hLy);
+        dx = shifted_diff(xi, xj, -sx, R.Lx);
+        dy = shifted_diff(yi, yj, -sy, R.Ly);
+        const float r2 = dx * dx + dy * dy;
+        const float r = sqrtf(r2);
+        const float k = a.bk[l];
+        const float d = r - a.br0[l];
+        const float kdr = -k * d / r;
+        const float fx = kdr * dx, fy = kdr * dy;
+        a.cf[a.cf_bond0 + 2 * l] = make_float2(fx, fy);
+        a.cf[a.cf_bond0 + 2 * l + 1] = make_float2(-fx, -fy);
+        U += 0.5 * (double)(k * d * d);
+        Wv += (double)(kdr * r2);
+    }
+    block_sum2(U, Wv, red);
+    if (threadIdx.x == 0) {
+        a.bond_part[2 * b] = U;
+        a.bond_part[2 * b + 1] = Wv;
+    }
+}
+
+template <bool kSigma, bool kTypes, bool kDebug>
+__global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
+    extern __shared__ float4 dyn[];
+    __shared__ int s_cstart[kMaxSlots];
+    __shared__ int s_off[kMaxSlots + 1];
+    __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
+    __shared__ double s_red[2 * 4];
+    int b = blockIdx.x;
+    if (b < a.ntiles) {
+        PairLds L;
+        L.ent = dyn;
+        int *tail = reinterpret_cast<int *>(dyn + a.lds_cap);
+        L.typ = tail;
+        L.sp = tail + (kTypes ? a.lds_cap : 0);
+        L.cstart = s_cstart;
+        L.off = s_off;
+        L.pt = s_pt;
+        L.red = s_red;
+        pair_tile<kSigma, kTypes, kDebug>(a, xcd_remap(b, a.ntiles), L);
+        return;
+    }
+    b -= a.ntiles;
+    if (b < a.nespans) {
+        elem_block(a, b, s_red);
+        return;
+    }
+    b -= a.nespans;
+    if (b < a.nbspans) bond_block(a, b, s_red);
+}
+
+// Total force per particle (pair + gathered corners), internal order.
+__global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
+                                                     const float *fy,
+                                                     const float2 *cf,
+                                                     const int32_t *goff,
+                                                     const int32_t *gent,
+                                                     float *ox, float *oy) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float x = fx[p], y = fy[p];
+    if (goff) {
+        for (int e = goff[p]; e < goff[p + 1]; ++e) {
+            const float2 c = cf[gent[e]];
+            x += c.x;
+            y += c.y;
+        }
+    }
+    ox[p] = x;
+    oy[p] = y;
+}
+
+// Per-replica energy totals in a fixed order: one workgroup per replica.
+// out[r * 8 + ...] = {KE2 (sum m v^2), vvmax, U_lj, W_lj, U_fea, W_fea,
+//                     U_bond, W_bond}
+struct EnergyArgs {
+    const int32_t *rng;  // per replica 8 ints: kick, tile, elem, bond ranges
+    const double *ke_part, *lj_part, *fea_part, *bond_part;
+    double *out;
+};
+__global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
+    __shared__ double red[8];
+    const int r = blockIdx.x;
+    const int32_t *g = a.rng + 8 * r;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double vmax = 0.0;
+    for (int k = g[0] + (int)threadIdx.x; k < g[1]; k += blockDim.x) {
+        v[0] += a.ke_part[2 * k];
+        vmax = fmax(vmax, a.ke_part[2 * k + 1]);
+    }
+    for (int k = g[2] + (int)threadIdx.x; k < g[3]; k += blockDim.x) {
+        v[2] += a.lj_part[2 * k];
+        v[3] += a.lj_part[2 * k + 1];
+    }
+    for (int k = g[4] + (int)threadIdx.x; k < g[5]; k += blockDim.x) {
+        v[4] += a.fea_part[2 * k];
+        v[5] += a.fea_part[2 * k + 1];
+    }
+    for (int k = g[6] + (int)threadIdx.x; k < g[7]; k += blockDim.x) {
+        v[6] += a.bond_part[2 * k];
+        v[7] += a.bond_part[2 * k + 1];
+    }
+    double z = 0.0;
+    block_sum2(v[0], z, red);
+    if (threadIdx.x == 0) a.out[8 * r] = v[0];
+    block_sum2(v[2], v[3], red);
+    if (threadIdx.x == 0) { a.out[8 * r + 2] = v[2]; a.out[8 * r + 3] = v[3]; }
+    block_sum2(v[4], v[5], red);
+    if (threadIdx.x == 0) { a.out[8 * r + 4] = v[4]; a.out[8 * r + 5] = v[5]; }
+    block_sum2(v[6], v[7], red);
+    if (threadIdx.x == 0) { a.out[8 * r + 6] = v[6]; a.out[8 * r + 7] = v[7]; }
+    // max over the block
+    for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
+        a.out[8 * r + 1] = m;
+    }
+}
+
+// Zero an int array (graph-safe replacement of a memset node).
+__global__ void k_fill_int(int32_t *p, int32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host side)
+// ---------------------------------------------------------------------------
+hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
+                       hipStream_t s) {
+    // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
+    // 4 = kinetic-energy partials only
+    if (nblk <= 0) return hipSuccess;
+    dim3 g(nblk), b(kKickThreads);
+    switch (mode) {
+        case 0: hipLaunchKernelGGL(k_bin, g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
+        case 1:
+            if (gather) hipLaunchKernelGGL((k_kick<false, true, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_kick<false, true, false>), g, b, 0, s, a);
+            break;
+        case 2:
+            if (gather) hipLaunchKernelGGL((k_kick<true, false, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_kick<true, false, false>), g, b, 0, s, a);
+            break;
+        default:
+            if (gather) hipLaunchKernelGGL((k_kick<true, true, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_kick<true, true, false>), g, b, 0, s, a);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    if (part == 0) {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(kScanThreads), 0, s, a);
+    } else {
+        size_t lds = (size_t)kScanThreads * (a.cap + 1) * sizeof(int);
+        hipLaunchKernelGGL(k_scan_scatter, dim3(nblk), dim3(kScanThreads), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s) {
+    hipError_t e = launch_scan_part(0, nblk, a, s);
+    if (e != hipSuccess) return e;
+    return launch_scan_part(1, nblk, a, s);
+}
+
+size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
+    return (size_t)lds_cap * sizeof(float4) + (types ? lds_cap * sizeof(int) : 0) +
+           (debug ? lds_cap * sizeof(int) : 0);
+}
+
+hipError_t launch_forces(bool sigma, bool types, bool debug, int block,
+                         const ForceArgs &a, hipStream_t s) {
+    const int nblk = a.ntiles + a.nespans + a.nbspans;
+    if (nblk <= 0) return hipSuccess;
+    const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
+    dim3 g(nblk), b(block);
+#define FM_LAUNCH(S, T, D) hipLaunchKernelGGL((k_forces<S, T, D>), g, b, lds, s, a)
+    if (debug) {
+        if (sigma) { if (types) FM_LAUNCH(true, true, true); else FM_LAUNCH(true, false, true); }
+        else { if (types) FM_LAUNCH(false, true, true); else FM_LAUNCH(false, false, true); }
+    } else {
+        if (sigma) { if (types) FM_LAUNCH(true, true, false); else FM_LAUNCH(true, false, false); }
+        else { if (types) FM_LAUNCH(false, true, false); else FM_LAUNCH(false, false, false); }
+    }
+#undef FM_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_total_force(int n, const float *fx, const float *fy,
+                              const float2 *cf, const int32_t *goff,
+                              const int32_t *gent, float *ox, float *oy,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_total_force, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                       fx, fy, cf, goff, gent, ox, oy);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke,
+                         const double *lj, const double *fea,
+                         const double *bond, double *out, hipStream_t s) {
+    EnergyArgs a{rng, ke, lj, fea, bond, out};
+    hipLaunchKernelGGL(k_energy, dim3(nrep), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace fm

@@ -1,0 +1,186 @@
+/*
+ * femmd.h -- C ABI of the MI355X-native femmd2D step engine.
+ *
+ * The engine owns device memory; the caller owns every host buffer passed
+ * in or out.  Every call returns an int status (FM_OK = 0, negative on
+ * error) and never exits the process; fm_last_error() gives the text.
+ * One handle per host thread; each handle owns its own HIP stream.
+ *
+ * Particle indices in this API are the caller's: the order in which
+ * particles were added (fm_add_particles, then fm_add_elastomer nodes),
+ * as in the reference (femmd2D/src/particles.cpp:201-231, :295-385).
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   engine -> compute ("operator API"):
+ *     computeForcesNNGPUWrapper      femmd2D/src/interactions.h:77-86
+ *     computeBondForcesGPUWrapper    femmd2D/src/interactions.h:88-94
+ *     computeFeaForcesGPUWrapper     femmd2D/src/interactions.h:96-104
+ *     integratorStep1Wrapper         femmd2D/src/integrator.h:39
+ *     integratorStep2Wrapper         femmd2D/src/integrator.h:41
+ *     NeighborList::buildNNGPU       femmd2D/src/nn.h:74
+ *   host API (users):
+ *     MDSystem / c_* / System.*      femmd2D/src/md3dsystem.h:14-116,
+ *                                    femmd2D/src/interface.cpp:24-81,656-696
+ */
+#ifndef FEMMD_H
+#define FEMMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fm_system fm_system; /* one periodic 2D system */
+typedef struct fm_batch fm_batch;   /* many independent replicas */
+
+/* ---- status codes ---- */
+#define FM_OK 0
+#define FM_ERR_ARG (-1)      /* invalid argument                         */
+#define FM_ERR_STATE (-2)    /* call not valid in the current state      */
+#define FM_ERR_HIP (-3)      /* HIP runtime failure                      */
+#define FM_ERR_DEVICE (-4)   /* device-side sticky error, see below      */
+#define FM_ERR_GEOMETRY (-5) /* box too small for the cell grid, etc.    */
+#define FM_ERR_NOMEM (-6)
+
+/* ---- device error bits (sticky until fm_clear_errors) ---- */
+#define FM_DEV_CELL_OVERFLOW 1u  /* more particles in a cell than capacity */
+#define FM_DEV_TILE_OVERFLOW 2u  /* a cell tile did not fit in LDS         */
+#define FM_DEV_INVERTED 4u       /* an element with det F <= 0             */
+#define FM_DEV_NONFINITE 8u      /* a non-finite coordinate after a drift  */
+
+/* ---- material models (femmd2D/src/interactions.cpp:411-689) ---- */
+#define FM_NEO_HOOKEAN 0 /* calcFeaForcesCPU2, svk = 0 (reference CPU) */
+#define FM_SVK 1         /* calcFeaForcesCPU2, svk = 1                 */
+#define FM_LEGACY_GPU 2  /* computeFeaForcesGPU energy (= 2 x SVK)     */
+
+typedef struct fm_params {
+    double Lx, Ly;  /* periodic box (MDSystem::init, md3dsystem.cpp:49-80) */
+    double dt;      /* time step; reference default 0.005                  */
+    int32_t device; /* HIP device ordinal                                  */
+    int32_t reserved[7];
+} fm_params;
+
+/* Energies and observables, FP64 sums (femmd2D/src/properties.cpp:15-36,
+ * femmd2D/src/md3dsystem.cpp:151-179). */
+typedef struct fm_energies {
+    double ke;          /* 1/2 sum m v^2                                   */
+    double u_lj, w_lj;  /* pair energy and virial sum f_ij . r_ij          */
+    double u_bond, w_bond;
+    double u_fea, w_fea;
+    double vv_max;      /* max |v|^2                                       */
+    double temperature; /* KE / N (2D, kB = 1)                             */
+    double pressure;    /* 0.5 sum m v^2 / V + 0.5 (w_lj + w_fea) / V      */
+    int64_t steps;
+    int64_t n;
+} fm_energies;
+
+/* Number of HIP devices visible (md_device_init, femmd2D/src/CudaUtil.cu:45-102). */
+int fm_device_count(int32_t *n);
+
+/* ---- system construction (replaces MDSystem::init and friends) ---- */
+int fm_create(const fm_params *p, fm_system **out);
+void fm_destroy(fm_system *h);
+const char *fm_last_error(const fm_system *h); /* never NULL */
+
+/* Particles::addParticle (femmd2D/src/particles.cpp:201-231), vectorised.
+ * vx, vy may be NULL (zero).  type may be NULL (type 0). */
+int fm_add_particles(fm_system *h, int32_t n, const float *x, const float *y,
+                     const float *vx, const float *vy, const float *mass,
+                     const float *sigma, const int32_t *type,
+                     int32_t *first_id);
+
+/* Particles::addElastomer (femmd2D/src/particles.cpp:295-385).
+ * verts: 2*nv coordinates relative to (cx, cy); tris: 3*nt indices with
+ * index_base (1 for the reference mesh files).  Node masses: node_mass if
+ * > 0, else total_mass spread as rho*A/3 per corner.  exclude != 0 puts
+ * the nodes in a fresh exclusion group (no LJ among themselves).
+ * E_per_tri / nu_per_tri may be NULL (global values of fm_set_material).
+ * Unlike the reference, no edge bonds are created implicitly. */
+int fm_add_elastomer(fm_system *h, int32_t nv, const float *verts, int32_t nt,
+                     const int32_t *tris, int32_t index_base, float cx,
+                     float cy, int32_t type, float sigma, float node_mass,
+                     float total_mass, int32_t exclude, const float *E_per_tri,
+                     const float *nu_per_tri, int32_t *first_node);
+
+/* Bonds::createBond (femmd2D/src/bonds.cpp:14-29), per-bond k and r0;
+ * r0 < 0 means the current (minimum-image) distance. */
+int fm_add_bonds(fm_system *h, int32_t nb, const int32_t *i, const int32_t *j,
+                 const float *k, const float *r0);
+
+/* Interactions::setPairForce (femmd2D/src/interactions.cpp:65-79): row-major
+ * ntypes x ntypes tables; rcut in sigma units; ushift additive. */
+int fm_set_pair_table(fm_system *h, int32_t ntypes, const float *eps,
+                      const float *rcut, const float *ushift);
+
+/* setElasticConstants / set_youngs_modulus
+ * (femmd2D/src/interactions.cpp:43-50, femmd2D/src/interface.cpp:479-490). */
+int fm_set_material(fm_system *h, int32_t model, float E, float nu);
+int fm_set_dt(fm_system *h, double dt);
+int fm_set_lj(fm_system *h, int32_t on);
+/* Exclusion group of one particle (Particles::insertPartInExclusionGroup,
+ * femmd2D/src/particles.cpp:289-293); -1 = none.  Before fm_commit. */
+int fm_set_exclusion(fm_system *h, int32_t i, int32_t group);
+
+/* Masses of the given particles (Particles::mass, femmd2D/src/particles.h:56).
+ * A zero, negative or infinite mass pins the particle (it never moves).
+ * Before fm_commit. */
+int fm_set_mass(fm_system *h, int32_t n, const int32_t *ids, const float *mass);
+
+/* Upload, topology build, validation (allocate_cuda + init_neighbor_list,
+ * femmd2D/src/md3dsystem.h:86-94, femmd2D/src/nn.cpp:192-250). */
+int fm_commit(fm_system *h);
+
+/* ---- stepping (Integrator::run / doStep / calcForces) ---- */
+int fm_step(fm_system *h, int32_t nsteps);     /* c_run, System.run       */
+int fm_compute_forces(fm_system *h);           /* c_calcForces            */
+int fm_get_energies(fm_system *h, fm_energies *e);
+int fm_get_state(fm_system *h, float *x, float *y, float *vx, float *vy);
+int fm_set_state(fm_system *h, const float *x, const float *y, const float *vx,
+                 const float *vy);
+int fm_get_forces(fm_system *h, float *fx, float *fy);
+int fm_num_particles(const fm_system *h, int32_t *n);
+int fm_sync(fm_system *h);
+
+/* Launch tuning: cells per tile row and threads per workgroup of the force
+ * pass (0 = keep), graph capture of the step loop on/off. */
+int fm_set_tuning(fm_system *h, int32_t tile_cells, int32_t block,
+                  int32_t use_graph);
+
+/* ---- diagnostics used by the parity tests ---- */
+uint32_t fm_device_errors(fm_system *h);
+int fm_clear_errors(fm_system *h);
+/* Cell grid of the last binning.  key: n, caller order.  start: ncells+1
+ * and perm: n (caller ids in sorted order), either may be NULL. */
+int fm_get_cells(fm_system *h, int32_t *ncx, int32_t *ncy, int32_t *key,
+                 int32_t *start, int32_t *perm);
+/* Interacting sets at the current positions: per particle the count, the
+ * sum of (j+1) and the sum of (j+1)^2 over its interacting partners j. */
+int fm_pair_sets(fm_system *h, int32_t *cnt, int64_t *s1, int64_t *s2);
+/* Time one force pass on the device (ms per pass, averaged over reps). */
+int fm_time_forces(fm_system *h, int32_t reps, double *ms);
+/* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel with HIP
+ * events on the engine's stream (no graph).  ms[5] = per step: kick + bin,
+ * scan reduce, scan scatter, force pass, whole step. */
+int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
+
+/* ---- batched replicas (no reference counterpart) ---- */
+int fm_batch_create(int32_t n, fm_system *const *systems, fm_batch **out);
+void fm_batch_destroy(fm_batch *b);
+const char *fm_batch_last_error(const fm_batch *b);
+int fm_batch_step(fm_batch *b, int32_t nsteps);
+int fm_batch_get_state(fm_batch *b, int32_t r, float *x, float *y, float *vx,
+                       float *vy);
+int fm_batch_get_energies(fm_batch *b, int32_t r, fm_energies *e);
+int fm_batch_set_tuning(fm_batch *b, int32_t tile_cells, int32_t block,
+                        int32_t use_graph);
+/* Autotune tile width and workgroup size per replica-size bucket; results
+ * are cached in cache_path (a text file, one line per batch signature) when
+ * it is not NULL. */
+int fm_batch_tune(fm_batch *b, const char *cache_path);
+uint32_t fm_batch_device_errors(fm_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

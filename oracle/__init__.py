@@ -1,0 +1,248 @@
+"""Python access to the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by femmd_old_amd/.  See oracle.h for the
+reference lines each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+OR_U_LJ, OR_W_LJ, OR_U_BOND, OR_W_BOND, OR_U_FEA, OR_W_FEA, OR_NPAIRS, OR_NINV = range(8)
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int32)
+_l = C.POINTER(C.c_int64)
+_f = C.POINTER(C.c_float)
+
+
+class OrSystem(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("Lx", C.c_double), ("Ly", C.c_double),
+        ("x", _d), ("y", _d), ("sigma", _d),
+        ("type", _i), ("excl", _i),
+        ("ntypes", C.c_int32),
+        ("eps", _d), ("rc", _d), ("ushift", _d),
+        ("lj_on", C.c_int32),
+        ("nbond", C.c_int32),
+        ("bi", _i), ("bj", _i), ("bk", _d), ("br0", _d),
+        ("ntri", C.c_int32),
+        ("tri", _i),
+        ("rx", _d), ("ry", _d),
+        ("tE", _d), ("tnu", _d),
+        ("model", C.c_int32),
+    ]
+
+
+def build():
+    """Compile liboracle.so with its Makefile (g++)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.or_wrap_f32.argtypes = [C.c_int32, _f, _f, C.c_float, C.c_float]
+        L.or_cell_keys_f32.argtypes = [C.c_int32, _f, _f, C.c_float, C.c_float, C.c_int32,
+                                       C.c_int32, _i]
+        L.or_cell_count.argtypes = [C.c_double, C.c_double]
+        L.or_cell_count.restype = C.c_int32
+        L.or_counting_sort.argtypes = [C.c_int32, _i, C.c_int32, _i, _i]
+        L.or_pair_sets_f32.argtypes = [C.POINTER(OrSystem), C.c_int32, _i, _l, _l]
+        L.or_pair_sets_f32.restype = C.c_int32
+        L.or_forces.argtypes = [C.POINTER(OrSystem), C.c_int32, C.c_int32, _d, _d, _d, _d]
+        L.or_forces.restype = C.c_int32
+        L.or_run.argtypes = [C.POINTER(OrSystem), _d, _d, _d, _d, _d, _d, C.c_int32, C.c_double,
+                             C.c_int32, _d]
+        L.or_run.restype = C.c_int32
+        L.or_bl_create.argtypes = [C.POINTER(OrSystem), _f, _f, _f, C.c_float, C.c_float,
+                                   C.c_int32]
+        L.or_bl_create.restype = C.c_void_p
+        L.or_bl_run.argtypes = [C.c_void_p, C.c_int32]
+        L.or_bl_run.restype = C.c_double
+        L.or_bl_get.argtypes = [C.c_void_p, _f, _f, _f, _f, _d]
+        L.or_bl_destroy.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_i)
+
+
+class OracleSystem:
+    """Holds the numpy arrays behind an OrSystem (keeps them alive)."""
+
+    def __init__(self, Lx, Ly, x, y, sigma=None, type=None, excl=None, eps=(1.0,), rc=(2.5,),
+                 ushift=(0.0,), lj_on=True, bonds=None, tri=None, refx=None, refy=None, E=None,
+                 nu=None, model=0):
+        n = len(x)
+        f64 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+        i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int32))
+        self.keep = {}
+        k = self.keep
+        k["x"], k["y"] = f64(x), f64(y)
+        k["sigma"] = f64(np.ones(n) if sigma is None else sigma)
+        k["type"] = i32(np.zeros(n) if type is None else type)
+        k["excl"] = i32(-np.ones(n) if excl is None else excl)
+        k["eps"], k["rc"], k["ushift"] = f64(np.ravel(eps)), f64(np.ravel(rc)), f64(np.ravel(ushift))
+        nt = int(round(np.sqrt(k["eps"].shape[0])))
+        s = OrSystem()
+        s.n = n
+        s.Lx, s.Ly = float(Lx), float(Ly)
+        s.x, s.y, s.sigma = _dp(k["x"]), _dp(k["y"]), _dp(k["sigma"])
+        s.type, s.excl = _ip(k["type"]), _ip(k["excl"])
+        s.ntypes = nt
+        s.eps, s.rc, s.ushift = _dp(k["eps"]), _dp(k["rc"]), _dp(k["ushift"])
+        s.lj_on = int(bool(lj_on))
+        if bonds is not None and len(bonds[0]):
+            k["bi"], k["bj"] = i32(bonds[0]), i32(bonds[1])
+            k["bk"], k["br0"] = f64(bonds[2]), f64(bonds[3])
+            s.nbond = k["bi"].shape[0]
+            s.bi, s.bj, s.bk, s.br0 = _ip(k["bi"]), _ip(k["bj"]), _dp(k["bk"]), _dp(k["br0"])
+        if tri is not None and len(tri):
+            k["tri"] = i32(np.ravel(tri))
+            k["rx"] = f64(x if refx is None else refx)
+            k["ry"] = f64(y if refy is None else refy)
+            ntri = k["tri"].shape[0] // 3
+            k["tE"] = f64(np.broadcast_to(np.asarray(E, float), (ntri,)))
+            k["tnu"] = f64(np.broadcast_to(np.asarray(nu, float), (ntri,)))
+            s.ntri = ntri
+            s.tri, s.rx, s.ry = _ip(k["tri"]), _dp(k["rx"]), _dp(k["ry"])
+            s.tE, s.tnu = _dp(k["tE"]), _dp(k["tnu"])
+        s.model = int(model)
+        self.s = s
+        self.n = n
+
+    def with_positions(self, x, y):
+        """Same system at other positions (FP64 arrays)."""
+        self.keep["x"] = np.ascontiguousarray(np.asarray(x, np.float64))
+        self.keep["y"] = np.ascontiguousarray(np.asarray(y, np.float64))
+        self.s.x, self.s.y = _dp(self.keep["x"]), _dp(self.keep["y"])
+        return self
+
+
+def from_workload(w, x=None, y=None) -> OracleSystem:
+    """Oracle system of a configs.Workload at caller-order positions (x, y),
+    default the generated ones (wrapped in FP32 as the engine stores them)."""
+    ga = w.global_arrays()
+    if x is None:
+        xf = ga["x"].astype(np.float32)
+        yf = ga["y"].astype(np.float32)
+        wrap_f32(xf, yf, w.Lx, w.Ly)
+        x, y = xf, yf
+    return OracleSystem(w.Lx, w.Ly, x, y, sigma=ga["sigma"], excl=ga["excl"], eps=[w.eps],
+                        rc=[w.rc], ushift=[w.ushift], bonds=w.bonds, tri=ga["tri"],
+                        refx=ga["refx"].astype(np.float32), refy=ga["refy"].astype(np.float32),
+                        E=ga["E"], nu=ga["nu"], model=w.model)
+
+
+# ---------------------------------------------------------------------------
+def wrap_f32(x: np.ndarray, y: np.ndarray, Lx: float, Ly: float):
+    assert x.dtype == np.float32 and y.dtype == np.float32
+    lib().or_wrap_f32(x.shape[0], x.ctypes.data_as(_f), y.ctypes.data_as(_f), Lx, Ly)
+
+
+def cell_keys(x, y, Lx, Ly, ncx, ncy) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    key = np.empty(x.shape[0], np.int32)
+    lib().or_cell_keys_f32(x.shape[0], x.ctypes.data_as(_f), y.ctypes.data_as(_f), Lx, Ly, ncx,
+                           ncy, key.ctypes.data_as(_i))
+    return key
+
+
+def counting_sort(key: np.ndarray, ncells: int):
+    key = np.ascontiguousarray(key, np.int32)
+    start = np.empty(ncells + 1, np.int32)
+    perm = np.empty(key.shape[0], np.int32)
+    lib().or_counting_sort(key.shape[0], _ip(key), ncells, _ip(start), _ip(perm))
+    return start, perm
+
+
+def pair_sets(osys: OracleSystem, mode: int = 1):
+    n = osys.n
+    cnt = np.empty(n, np.int32)
+    s1 = np.empty(n, np.int64)
+    s2 = np.empty(n, np.int64)
+    rc = lib().or_pair_sets_f32(C.byref(osys.s), mode, _ip(cnt), s1.ctypes.data_as(_l),
+                                s2.ctypes.data_as(_l))
+    assert rc == 0
+    return cnt, s1, s2
+
+
+def forces(osys: OracleSystem, pred_f32: bool = True, all_pairs: bool = False,
+           scale: bool = False):
+    """FP64 forces (fx, fy), energies[8] and, with scale=True, the per-particle
+    sum of force-term magnitudes."""
+    n = osys.n
+    fx = np.empty(n)
+    fy = np.empty(n)
+    en = np.empty(8)
+    fs = np.empty(n) if scale else None
+    rc = lib().or_forces(C.byref(osys.s), int(pred_f32), int(all_pairs), _dp(fx), _dp(fy), _dp(en),
+                         _dp(fs))
+    assert rc == 0, rc
+    return (fx, fy, en, fs) if scale else (fx, fy, en)
+
+
+def run(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt, pred_f32=False, history=False):
+    """FP64 KDK rollout; returns (x, y, vx, vy, hist or None)."""
+    x, y, vx, vy = (np.array(a, np.float64) for a in (x, y, vx, vy))
+    mass = np.asarray(mass, np.float64)
+    inv_m = np.where(np.isfinite(mass) & (mass > 0), 1.0 / np.where(mass > 0, mass, 1.0), 0.0)
+    m_ke = np.where(np.isfinite(mass), mass, 0.0)
+    hist = np.empty((nsteps, 10)) if history else None
+    rc = lib().or_run(C.byref(osys.s), _dp(x), _dp(y), _dp(vx), _dp(vy), _dp(m_ke), _dp(inv_m),
+                      int(nsteps), float(dt), int(pred_f32), _dp(hist))
+    assert rc == 0, rc
+    return x, y, vx, vy, hist
+
+
+class Baseline:
+    """The reference DEV=CPU path restated in FP32 (Verlet list, skin 0.4)."""
+
+    def __init__(self, osys: OracleSystem, vx, vy, mass, dt, skin=0.4, threads=1):
+        self._vx = np.ascontiguousarray(vx, np.float32)
+        self._vy = np.ascontiguousarray(vy, np.float32)
+        self._m = np.ascontiguousarray(mass, np.float32)
+        self.osys = osys
+        self.h = lib().or_bl_create(C.byref(osys.s), self._vx.ctypes.data_as(_f),
+                                    self._vy.ctypes.data_as(_f), self._m.ctypes.data_as(_f),
+                                    float(dt), float(skin), int(threads))
+        if not self.h:
+            raise RuntimeError("baseline create failed (box too small?)")
+        self.threads = threads
+
+    def run(self, nsteps: int) -> float:
+        return lib().or_bl_run(self.h, int(nsteps))
+
+    def state(self):
+        n = self.osys.n
+        out = [np.empty(n, np.float32) for _ in range(4)]
+        props = np.empty(5)
+        lib().or_bl_get(self.h, *(a.ctypes.data_as(_f) for a in out), _dp(props))
+        return out, props
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_bl_destroy(self.h)
+            self.h = None

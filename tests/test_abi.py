@@ -1,0 +1,101 @@
+"""The C ABI library loads, exports every entry point include/femmd.h
+declares, and its host-side validation behaves (no GPU work is issued by
+any call before fm_commit)."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from femmd_old_amd import _lib, System, FemmdError
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_entry_points():
+    text = (ROOT / "include" / "femmd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fm_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_entry_points()
+    assert len(names) >= 35
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (fm_[a-z_]+)\b", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+    # the ctypes table covers the whole header
+    assert set(_lib.PROTOTYPES) == set(names)
+
+
+def test_library_is_gfx950_only():
+    # the offload bundle ids embedded in our own build name only gfx950
+    blob = _lib.LIB_PATH.read_bytes()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa-+(gfx[0-9a-z]+)", blob))
+    assert targets == {b"gfx950"}, targets
+
+
+def test_create_rejects_bad_box():
+    lib = _lib.load()
+    h = C.c_void_p()
+    p = _lib.FmParams(Lx=-1.0, Ly=10.0, dt=0.005, device=0)
+    assert lib.fm_create(C.byref(p), C.byref(h)) == _lib.FM_ERR_ARG
+    p = _lib.FmParams(Lx=10.0, Ly=10.0, dt=0.0, device=0)
+    assert lib.fm_create(C.byref(p), C.byref(h)) == _lib.FM_ERR_ARG
+
+
+def test_host_side_construction_and_validation():
+    s = System(20.0, 20.0)
+    first = s.add_particles([1.0, 2.0], [3.0, 4.0], m=1.0, sig=1.0)
+    assert first == 0 and s.N == 2
+    v, t = np.array([0, 0, 1, 0, 0, 1], np.float32), np.array([1, 2, 3], np.int32)
+    node0 = s.add_elastomer(v, t, 10.0, 10.0, mass=3.0)
+    assert node0 == 2 and s.N == 5
+    x, y, vx, vy = s.get_state()  # before commit: host copy, caller order
+    np.testing.assert_allclose(x, [1, 2, 10, 11, 10])
+    with pytest.raises(FemmdError) as e:
+        s.add_elastomer(v, np.array([1, 2, 4], np.int32), 0, 0, mass=1.0)
+    assert e.value.code == _lib.FM_ERR_ARG
+    with pytest.raises(FemmdError) as e:
+        s.add_bonds([0], [7], [1.0])
+    assert e.value.code == _lib.FM_ERR_ARG
+    with pytest.raises(ValueError):
+        s.set_pair_potential([1.0, 1.0], [2.5, 2.5], [0.0, 0.0])  # not square
+    with pytest.raises(FemmdError) as e:
+        s.set_pair_potential([1.0], [-2.5], [0.0])  # negative cutoff
+    assert e.value.code == _lib.FM_ERR_ARG
+    with pytest.raises(ValueError):
+        s.add_particles([1.0], [1.0, 2.0])
+    with pytest.raises(FemmdError) as e:
+        s.run(10)  # before commit
+    assert e.value.code == _lib.FM_ERR_STATE
+    with pytest.raises(FemmdError) as e:
+        s.add_particles([np.nan], [1.0])
+    assert e.value.code == _lib.FM_ERR_ARG
+    with pytest.raises(FemmdError) as e:
+        s.set_youngs_modulus(10.0, 0.5)  # nu = 0.5 is incompressible
+    assert e.value.code == _lib.FM_ERR_ARG
+    s.close()
+
+
+def test_set_mass_and_tuning_validation():
+    s = System(20.0, 20.0)
+    s.add_particles([1.0, 2.0], [3.0, 4.0], vx=[1.0, 1.0], vy=[0.5, 0.5])
+    s.set_mass([1], np.inf)  # pinned: its velocity is cleared
+    _, _, vx, vy = s.get_state()
+    assert vx[1] == 0 and vy[1] == 0 and vx[0] == 1.0
+    with pytest.raises(FemmdError):
+        s.set_mass([5], 1.0)
+    with pytest.raises(FemmdError):
+        s.set_tuning(tile_cells=100)
+    with pytest.raises(FemmdError):
+        s.set_tuning(block=96)
+    s.set_tuning(tile_cells=16, block=128, use_graph=0)
+    s.close()
