@@ -304,9 +304,10 @@ class System:
 
     def profile(self, nsteps: int = 10) -> dict:
         """Per-step device ms of each kernel (HIP events, engine stream)."""
-        ms = (C.c_double * 5)()
+        ms = (C.c_double * 6)()
         self._check(self._lib.fm_profile_step(self._h, int(nsteps), ms))
-        return dict(zip(("kick", "scan_reduce", "scan_scatter", "forces", "step"), list(ms)))
+        return dict(zip(("kick", "scan_reduce", "scan_apply", "scatter", "forces", "step"),
+                        list(ms)))
 
 
 class Batch:

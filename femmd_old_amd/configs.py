@@ -229,6 +229,31 @@ def _rsa(rng, L, radii, gap, max_tries=200000):
     return centres
 
 
+def _hex_sites(rng, L, n, radii, gap):
+    """n disk centres on a periodic hexagonal lattice of sites (rows offset
+    by half a spacing), each jittered inside the slack its radius leaves."""
+    rmax = float(np.max(radii))
+    smin = 2 * rmax + gap
+    ncol = int(math.floor(L / smin))
+    nrow = int(math.floor(L / (smin * math.sqrt(3) / 2))) // 2 * 2
+    if ncol * nrow < n or ncol < 1 or nrow < 2:
+        raise RuntimeError("disks do not fit on a hexagonal lattice of sites")
+    sx, sy = L / ncol, L / nrow
+    nn = min(sx, math.hypot(sx / 2, sy))  # nearest-neighbour site distance
+    j, i = np.meshgrid(np.arange(nrow), np.arange(ncol), indexing="ij")
+    px = (i + 0.5 * (j % 2)) * sx
+    py = j * sy
+    sites = np.stack([px.ravel(), py.ravel()], 1)
+    pick = rng.choice(sites.shape[0], n, replace=False)
+    out = sites[pick].copy()
+    slack = np.maximum(0.0, (nn - gap) / 2 - np.asarray(radii))
+    ang = rng.uniform(0, 2 * math.pi, n)
+    rad = rng.uniform(0, 1, n) * slack
+    out[:, 0] = np.mod(out[:, 0] + rad * np.cos(ang), L)
+    out[:, 1] = np.mod(out[:, 1] + rad * np.sin(ang), L)
+    return out
+
+
 def _lattice_solvent(rng, L, n, centres, radii, margin, jitter):
     """n solvent particles on a jittered square lattice over the box, with the
     lattice sites inside any disk (radius + margin) removed; surplus free
@@ -331,7 +356,15 @@ def c3(seed: int = 3, scale: float = 1.0) -> Workload:
     nd = max(1, int(round(4096 * scale)))
     radii = np.exp(rng.uniform(math.log(4.0), math.log(8.0), nd))
     L = math.sqrt(ns / 0.75 + np.sum(np.pi * radii ** 2))
-    centres = _rsa(rng, L, radii + 1.5, 1.0)
+    # disk + anchor shell (rim + 1) occupies 36-60 % of the box: random
+    # sequential addition jams there, so disks sit on a jittered hexagonal
+    # lattice of sites whose spacing clears the largest disk and its shell
+    while True:  # small test scales may need a slightly larger box
+        try:
+            centres = _hex_sites(rng, L, nd, radii + 1.5, 0.5)
+            break
+        except RuntimeError:
+            L *= 1.02
     els = []
     anchors = []
     for k, (c, r) in enumerate(zip(centres, radii)):
@@ -343,8 +376,9 @@ def c3(seed: int = 3, scale: float = 1.0) -> Workload:
         els.append(el)
         rv = np.sqrt((v * v).sum(1))
         rim = rv.max()
+        rot = rng.uniform(0, 2 * math.pi)
         for q in range(8):
-            th = 2 * math.pi * q / 8 + rng.uniform(0, 2 * math.pi / 8)
+            th = 2 * math.pi * q / 8 + rot
             ax, ay = (rim + 1.0) * math.cos(th), (rim + 1.0) * math.sin(th)
             node = int(np.argmin((v[:, 0] - ax) ** 2 + (v[:, 1] - ay) ** 2))
             # anchor at distance 1 from its rim node, radially outward
@@ -352,7 +386,8 @@ def c3(seed: int = 3, scale: float = 1.0) -> Workload:
             nr = math.hypot(nx, ny) or 1.0
             anchors.append((k, node, c[0] + nx + nx / nr, c[1] + ny + ny / nr))
     na = len(anchors)
-    xb, yb = _lattice_solvent(rng, L, ns - na, centres, radii + 1.5, 0.0, 0.1)
+    # solvent keeps >= 1 sigma from the anchor shell (rim + 1)
+    xb, yb = _lattice_solvent(rng, L, ns - na, centres, radii + 2.0, 0.0, 0.1)
     ax = np.array([a[2] for a in anchors]) % L
     ay = np.array([a[3] for a in anchors]) % L
     x = np.concatenate([xb, ax])

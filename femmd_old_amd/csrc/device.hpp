@@ -64,6 +64,7 @@ struct KickArgs {
     const float *mass, *inv_m;
     int32_t *count;            // per-cell counters (zero on entry)
     int32_t *tmp;              // cell * cap + slot -> particle
+    int32_t *key;              // particle -> global cell (-1 = dropped)
     int32_t cap;
     double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
     uint32_t *err;
@@ -72,8 +73,10 @@ struct KickArgs {
 struct ScanArgs {
     int32_t ncells;
     int32_t cap;
-    int32_t n;                 // particles (bound on every perm / sorted write)
-    int32_t *count;            // zeroed here for the next binning
+    int32_t n;                 // particles
+    const int32_t *count;      // counters of this binning
+    int32_t *count_zero;       // counters of the previous binning, zeroed
+    const int32_t *key;
     const int32_t *tmp;
     int32_t *bsum;             // per scan workgroup totals
     int32_t *start;            // ncells + 1

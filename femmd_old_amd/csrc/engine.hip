@@ -3,8 +3,9 @@
 // Replaces the reference's per-step host loop (femmd2D/src/integrator.cpp:
 // 8-57) and its upload / topology setup (femmd2D/src/particles.cpp:105-199,
 // femmd2D/src/bonds.cpp:31-83, femmd2D/src/nn.cpp:192-250).  One step is
-// four kernels on one stream (kick+bin, scan, scatter, forces) with no host
-// synchronisation; the step loop is replayed from a captured HIP graph.
+// five kernels on one stream (kick+bin, scan reduce, scan apply, scatter,
+// forces) with no host synchronisation; the step loop is replayed from a
+// captured HIP graph.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -17,6 +18,7 @@ namespace fm {
 namespace {
 
 constexpr int kGraphSteps = 16;  // step bodies per captured graph
+static_assert(kGraphSteps % 2 == 0, "graph bodies must return to their counter parity");
 
 float wrap_host(float r, float L) {  // femmd2D/src/def.h:112, FP32
     float q = r / L;
@@ -345,7 +347,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(fy_, N);
     FM_AL(outx_, N);
     FM_AL(outy_, N);
-    FM_AL(count_, E->ncells_);
+    FM_AL(count_, 2 * (size_t)E->ncells_);
+    FM_AL(key_, N);
     FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
     const int nscan = (E->ncells_ + kScanCells - 1) / kScanCells;
     FM_AL(bsum_, std::max(nscan, 1));
@@ -360,7 +363,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(ene_, 8 * R);
     FM_AL(err_, 1);
 #undef FM_AL
-    if ((he = hipMemsetAsync(E->count_.p, 0, sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
+    if ((he = hipMemsetAsync(E->count_.p, 0, 2 * sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
         return E->check(he, "memset", err);
     if ((he = hipMemsetAsync(E->err_.p, 0, sizeof(uint32_t), st)) != hipSuccess)
         return E->check(he, "memset", err);
@@ -491,14 +494,16 @@ int Engine::launch_bin() {
     a.spans = kspans_.p;
     a.x = x_.p;
     a.y = y_.p;
-    a.count = count_.p;
+    a.count = count_.p + (size_t)parity_ * ncells_;
+    a.key = key_.p;
     a.tmp = tmp_.p;
     a.cap = cap_;
     a.err = err_.p;
     return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
-static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *tmp, int32_t *bsum,
+static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *count_zero,
+                          const int32_t *key, int32_t *tmp, int32_t *bsum,
                           int32_t *start, int32_t *perm, float *x, float *y, int32_t *grp,
                           float *sig, int32_t *typ, float4 *sorted, int32_t *sorted_typ) {
     ScanArgs s{};
@@ -506,6 +511,8 @@ static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *t
     s.n = n;
     s.cap = cap;
     s.count = count;
+    s.count_zero = count_zero;
+    s.key = key;
     s.tmp = tmp;
     s.bsum = bsum;
     s.start = start;
@@ -537,7 +544,8 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.gent = gather_ ? gent_.p : nullptr;
     k.mass = mass_.p;
     k.inv_m = inv_m_.p;
-    k.count = count_.p;
+    k.count = count_.p + (size_t)parity_ * ncells_;
+    k.key = key_.p;
     k.tmp = tmp_.p;
     k.cap = cap_;
     k.ke_part = ke_part_.p;
@@ -554,15 +562,16 @@ int Engine::launch_step_body(bool kick2, bool drift) {
 int Engine::launch_forces_only(bool debug) {
     hipError_t e;
     const int nscan = (ncells_ + kScanCells - 1) / kScanCells;
-    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p, tmp_.p, bsum_.p, start_.p, perm_.p, x_.p,
+    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p + (size_t)parity_ * ncells_,
+                           count_.p + (size_t)(1 - parity_) * ncells_, key_.p, tmp_.p, bsum_.p, start_.p, perm_.p, x_.p,
                            y_.p, grp_.p, sigma_ ? sig_.p : nullptr, types_ ? typ_.p : nullptr,
                            sorted_.p, types_ ? sorted_typ_.p : nullptr);
-    e = launch_scan_part(0, nscan, s, stream_);
-    if (e != hipSuccess) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
-    e = launch_scan_part(1, nscan, s, stream_);
-    if (e != hipSuccess) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[3], stream_);
+    for (int part = 0; part < 3; ++part) {
+        e = launch_scan_part(part, nscan, s, stream_);
+        if (e != hipSuccess) return -1;
+        if (prof_ev_) (void)hipEventRecord(prof_ev_[2 + part], stream_);
+    }
+    parity_ ^= 1;  // the next binning uses the other (now zeroed) counters
     ForceArgs f{};
     f.reps = reps_.p;
     f.ptab = ptab_.p;
@@ -602,18 +611,19 @@ int Engine::launch_forces_only(bool debug) {
     f.dbg_s2 = dbg_s2_.p;
     f.err = err_.p;
     e = launch_forces(sigma_, types_, debug, block_, f, stream_);
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
 
 // Per-kernel device time of nsteps step bodies (no graph): HIP events on
-// the engine's stream around each launch.  out[0..4] = kick+bin,
-// scan_reduce, scan_scatter, forces, whole body, in ms per step.
+// the engine's stream around each launch.  out[0..5] = kick+bin,
+// scan_reduce, scan_apply, scatter, forces, whole body, in ms per step.
 int Engine::profile(int nsteps, double *out, std::string &err) {
     if (nsteps <= 0) { err = "nsteps must be > 0"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev(5 * nsteps);
+    const int K = kProfEvents;
+    std::vector<hipEvent_t> ev(K * nsteps);
     for (auto &e : ev) {
         rc = check(hipEventCreate(&e), "event", err);
         if (rc) return rc;
@@ -622,7 +632,7 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     // consistent KDK state: a final half-kick closes the loop below)
     if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
     for (int s = 0; s < nsteps; ++s) {
-        prof_ev_ = &ev[5 * s];
+        prof_ev_ = &ev[K * s];
         int bad = launch_step_body(true, true);
         prof_ev_ = nullptr;
         if (bad) return check(hipGetLastError(), "step", err);
@@ -630,18 +640,19 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps + 1;
     rc = sync(err);
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[kProfEvents] = {0};
     for (int s = 0; s < nsteps && !rc; ++s) {
         float t;
-        hipEvent_t *e = &ev[5 * s];
-        (void)hipEventElapsedTime(&t, e[0], e[1]); acc[0] += t;
-        (void)hipEventElapsedTime(&t, e[1], e[2]); acc[1] += t;
-        (void)hipEventElapsedTime(&t, e[2], e[3]); acc[2] += t;
-        (void)hipEventElapsedTime(&t, e[3], e[4]); acc[3] += t;
-        (void)hipEventElapsedTime(&t, e[0], e[4]); acc[4] += t;
+        hipEvent_t *e = &ev[K * s];
+        for (int k = 0; k + 1 < K; ++k) {
+            (void)hipEventElapsedTime(&t, e[k], e[k + 1]);
+            acc[k] += t;
+        }
+        (void)hipEventElapsedTime(&t, e[0], e[K - 1]);
+        acc[K - 1] += t;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k < 5; ++k) out[k] = acc[k] / nsteps;
+    for (int k = 0; k < K; ++k) out[k] = acc[k] / nsteps;
     return rc;
 }
 
@@ -664,14 +675,18 @@ int Engine::step(int nsteps, std::string &err) {
     // step s is fused with the first half-kick of step s+1
     if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
     int rem = nsteps - 1;
-    if (use_graph_ && rem >= kGraphSteps) {
+    if (use_graph_ && rem >= kGraphSteps + 1) {
         if (!graph_exec_) {
+            // the captured bodies alternate the counter halves; an even
+            // number of them returns to the parity they started from
+            graph_parity_ = parity_;
             rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture", err);
             if (rc) return rc;
             int bad = 0;
             for (int s = 0; s < kGraphSteps; ++s) bad |= launch_step_body(true, true);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(stream_, &g);
+            parity_ = graph_parity_;
             if (bad || ce != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
                 return check(ce != hipSuccess ? ce : hipErrorLaunchFailure, "capture step", err);
@@ -680,6 +695,10 @@ int Engine::step(int nsteps, std::string &err) {
             rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate", err);
             if (rc) { drop_graph(); return rc; }
             graph_steps_ = kGraphSteps;
+        }
+        if (parity_ != graph_parity_) {
+            if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+            --rem;
         }
         while (rem >= graph_steps_) {
             rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);

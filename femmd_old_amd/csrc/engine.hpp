@@ -134,7 +134,7 @@ private:
     DBuf<PairEntry> ptab_;
     DBuf<float> x_, y_, vx_, vy_, fx_, fy_, mass_, inv_m_, sig_;
     DBuf<int32_t> grp_, typ_, uid_;
-    DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_;
+    DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_, key_;
     DBuf<float4> sorted_;
     DBuf<Span> kspans_, espans_, bspans_;
     DBuf<Tile> tiles_;
@@ -149,7 +149,12 @@ private:
     DBuf<int32_t> dbg_cnt_;
     DBuf<long long> dbg_s1_, dbg_s2_;
 
-    hipEvent_t *prof_ev_ = nullptr;  // set only while profiling
+    // profiling: events before the kick, after kick, reduce, apply,
+    // scatter and forces (set only while profiling)
+    static constexpr int kProfEvents = 6;
+    hipEvent_t *prof_ev_ = nullptr;
+    int parity_ = 0;        // which half of the double-buffered cell counters
+    int graph_parity_ = 0;  // parity the captured step graph starts from
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     int graph_steps_ = 0;

@@ -3,11 +3,13 @@
 // Per step (one HIP graph node each):
 //   k_kick      second half-kick of the previous step (+ KE partials),
 //               first half-kick, drift, periodic wrap, cell key, bin
-//               (integer atomic slot per cell into a capacity-padded list)
-//   k_scan_reduce / k_scan_scatter
+//               (wavefront-aggregated integer atomics into a capacity-
+//               padded slot list per cell)
+//   k_scan_reduce / k_scan_apply / k_scatter
 //               counting sort: exclusive scan of the cell counts into
-//               cell_start, each cell's list sorted by particle index into
-//               perm, and positions gathered into sorted order
+//               cell_start, then one thread per particle places it at
+//               cell_start + its rank by index within the cell (perm) and
+//               copies its position into sorted order
 //   k_forces    one launch with three workgroup roles:
 //               LJ cell tiles (LDS-staged 3x3 cells), CST elements and
 //               harmonic bonds (corner forces into a buffer, no atomics)
@@ -104,6 +106,38 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
     }
 }
 
+// Bin one particle per lane into its cell's slot list (the reference's
+// atomic cell fill, femmd2D/src/nnCuda.cu:69-82, re-designed): runs of
+// equal cells among consecutive lanes -- the common case, since particles
+// are stored near their cell -- take one integer atomicAdd for the whole
+// run and hand out slots by lane offset.  Slot order within a cell is not
+// deterministic; the scatter below orders each cell by particle index.
+// All 64 lanes of the wavefront must call this (ballot / shuffles).
+__device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, int c) {
+    const int lane = threadIdx.x & 63;
+    const int ck = active ? c : -1 - lane;  // inactive lanes never join a run
+    const int prev = __shfl_up(ck, 1);
+    const bool head = (lane == 0) || (ck != prev);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? (__ffsll((unsigned long long)after) - 1) : 64;
+    int base = 0;
+    if (head && active) base = atomicAdd(&a.count[c], next - lane);
+    base = __shfl(base, h);
+    if (active) {
+        const int slot = base + (lane - h);
+        if (slot < a.cap) {
+            a.tmp[(size_t)c * a.cap + slot] = p;
+            a.key[p] = c;
+        } else {
+            atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW
+            a.key[p] = -1;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_kick: integrator + binning (femmd2D/src/integratorCuda.cu:110-206,
 // femmd2D/src/nnCuda.cu:69-82 re-designed).  One workgroup covers one span
@@ -121,6 +155,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const int p = sp.begin + (int)threadIdx.x;
     double ke = 0.0;
     float vvmax = 0.0f;
+    int cell = 0;
     if (p < sp.end) {
         float fx = a.fx[p], fy = a.fy[p];
         if (kGather) {
@@ -155,16 +190,12 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             }
             a.x[p] = x;
             a.y[p] = y;
-            const int c = R.cell0 + cell_key(x, y, R);
-            const int slot = atomicAdd(&a.count[c], 1);
-            if (slot < a.cap)
-                a.tmp[(size_t)c * a.cap + slot] = p;
-            else
-                atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW
+            cell = R.cell0 + cell_key(x, y, R);
         }
         a.vx[p] = vx;
         a.vy[p] = vy;
     }
+    if (kDrift) bin_wave(a, p < sp.end, p, cell);
     if (kKick2) {
         double vm = 0.0;
         // max via a float wave reduction (exact, order independent)
@@ -212,23 +243,23 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
     const Span sp = a.spans[blockIdx.x];
     const RepDesc R = a.reps[sp.rep];
     const int p = sp.begin + (int)threadIdx.x;
-    if (p >= sp.end) return;
-    float x = a.x[p], y = a.y[p];
-    if (!(isfinite(x) && isfinite(y))) {
-        atomicOr(a.err, 8u);
-        x = 0.0f;
-        y = 0.0f;
+    const bool active = p < sp.end;
+    int c = 0;
+    if (active) {
+        float x = a.x[p], y = a.y[p];
+        if (!(isfinite(x) && isfinite(y))) {
+            atomicOr(a.err, 8u);
+            x = 0.0f;
+            y = 0.0f;
+        }
+        c = R.cell0 + cell_key(x, y, R);
     }
-    const int c = R.cell0 + cell_key(x, y, R);
-    const int slot = atomicAdd(&a.count[c], 1);
-    if (slot < a.cap)
-        a.tmp[(size_t)c * a.cap + slot] = p;
-    else
-        atomicOr(a.err, 1u);
+    bin_wave(a, active, p, c);
 }
 
 // ---------------------------------------------------------------------------
-// Counting sort over all cells of all replicas.
+// Counting sort over all cells of all replicas: per-workgroup cell totals,
+// exclusive scan into cell_start, then a particle-parallel scatter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
     __shared__ int red[kScanThreads / 64];
@@ -249,16 +280,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
     }
 }
 
-// Exclusive scan -> cell_start; each cell's slot list sorted by particle
-// index (the in-cell order of femmd2D/src/nn.cpp:8-20) -> perm; positions
-// gathered into sorted order; counters zeroed for the next binning.
-__global__ __launch_bounds__(kScanThreads) void k_scan_scatter(ScanArgs a) {
-    extern __shared__ int sbuf[];  // kScanThreads * (cap + 1)
+// Exclusive scan of the (capped) cell counts -> cell_start; the other half
+// of the double-buffered counters (used by the previous binning, whose
+// scatter has completed) is zeroed for the next binning.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
     __shared__ int red[kScanThreads / 64];
     __shared__ int wsum[kScanThreads / 64];
     __shared__ int s_pre;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // prefix of all earlier scan workgroups
     int pre = 0;
     for (int k = t; k < (int)blockIdx.x; k += kScanThreads) pre += a.bsum[k];
     pre = wave_sum_int(pre);
@@ -272,7 +301,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_scatter(ScanArgs a) {
         n[q] = (c < a.ncells) ? min(a.count[c], a.cap) : 0;
         tsum += n[q];
     }
-    // block exclusive scan of the per-thread sums
     int incl = wave_incl_scan(tsum);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
@@ -285,41 +313,39 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_scatter(ScanArgs a) {
     int wbase = 0;
     for (int k = 0; k < w; ++k) wbase += wsum[k];
     int run = s_pre + wbase + incl - tsum;
-    int *mine = sbuf + t * (a.cap + 1);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int c = base + q;
         if (c >= a.ncells) break;
         a.start[c] = run;
-        const int m = n[q];
-        const int *src = a.tmp + (size_t)c * a.cap;
-        // insertion sort of the cell's particle indices (ascending)
-        for (int k = 0; k < m; ++k) {
-            int v = src[k];
-            int j = k - 1;
-            while (j >= 0 && mine[j] > v) {
-                mine[j + 1] = mine[j];
-                --j;
-            }
-            mine[j + 1] = v;
-        }
-        for (int k = 0; k < m; ++k) {
-            const int i = mine[k];
-            const int pos = run + k;
-            if (pos >= a.n) break;  // only after an aborted binning sequence
-            a.perm[pos] = i;
-            float4 e;
-            e.x = a.x[i];
-            e.y = a.y[i];
-            e.z = __int_as_float(a.grp[i]);
-            e.w = a.sig ? a.sig[i] : 0.0f;
-            a.sorted[pos] = e;
-            if (a.typ) a.sorted_typ[pos] = a.typ[i];
-        }
-        a.count[c] = 0;
-        run += m;
+        a.count_zero[c] = 0;
+        run += n[q];
         if (c == a.ncells - 1) a.start[a.ncells] = run;
     }
+}
+
+// One thread per particle: its sorted position is cell_start + its rank
+// among the cell's members by particle index (the in-cell order of
+// femmd2D/src/nn.cpp:8-20, deterministic); positions, exclusion group and
+// sigma are copied into sorted order for the force pass.
+__global__ __launch_bounds__(kScanThreads) void k_scatter(ScanArgs a) {
+    const int i = blockIdx.x * kScanThreads + (int)threadIdx.x;
+    if (i >= a.n) return;
+    const int c = a.key[i];
+    if (c < 0) return;  // dropped by a cell overflow (flagged)
+    const int m = min(a.count[c], a.cap);
+    const int *src = a.tmp + (size_t)c * a.cap;
+    int rank = 0;
+    for (int k = 0; k < m; ++k) rank += (src[k] < i);
+    const int pos = a.start[c] + rank;
+    a.perm[pos] = i;
+    float4 e;
+    e.x = a.x[i];
+    e.y = a.y[i];
+    e.z = __int_as_float(a.grp[i]);
+    e.w = a.sig ? a.sig[i] : 0.0f;
+    a.sorted[pos] = e;
+    if (a.typ) a.sorted_typ[pos] = a.typ[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -652,10 +678,20 @@ __device__ void bond_block(const ForceArgs &a, int b, double *red) {
         const int sy = (dy >= R.hLy) - (dy < -R.hLy);
         dx = shifted_diff(xi, xj, -sx, R.Lx);
         dy = shifted_diff(yi, yj, -sy, R.Ly);
-        const float r2 = dx * dx + dy * dy;
+        // r - r0 = (r^2 - r0^2) / (r + r0) with r^2 - r0^2 formed from
+        // error-free products and sums, so a bond near rest keeps its
+        // relative accuracy (r - r0 in FP32 would cancel to nothing)
+        const float p1 = dx * dx, e1 = __builtin_fmaf(dx, dx, -p1);
+        const float p2 = dy * dy, e2 = __builtin_fmaf(dy, dy, -p2);
+        const float r2 = p1 + p2;
+        const float bb = r2 - p1;
+        const float t2 = (p1 - (r2 - bb)) + (p2 - bb);
+        const float r0 = a.br0[l];
+        const float h0 = r0 * r0, l0 = __builtin_fmaf(r0, r0, -h0);
+        const float q = (r2 - h0) + ((t2 + e1 + e2) - l0);
         const float r = sqrtf(r2);
         const float k = a.bk[l];
-        const float d = r - a.br0[l];
+        const float d = q / (r + r0);
         const float kdr = -k * d / r;
         const float fx = kdr * dx, fy = kdr * dy;
         a.cf[a.cf_bond0 + 2 * l] = make_float2(fx, fy);
@@ -810,17 +846,21 @@ hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s
     if (nblk <= 0) return hipSuccess;
     if (part == 0) {
         hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(kScanThreads), 0, s, a);
+    } else if (part == 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(nblk), dim3(kScanThreads), 0, s, a);
     } else {
-        size_t lds = (size_t)kScanThreads * (a.cap + 1) * sizeof(int);
-        hipLaunchKernelGGL(k_scan_scatter, dim3(nblk), dim3(kScanThreads), lds, s, a);
+        const int np = (a.n + kScanThreads - 1) / kScanThreads;
+        if (np > 0) hipLaunchKernelGGL(k_scatter, dim3(np), dim3(kScanThreads), 0, s, a);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s) {
-    hipError_t e = launch_scan_part(0, nblk, a, s);
-    if (e != hipSuccess) return e;
-    return launch_scan_part(1, nblk, a, s);
+    for (int part = 0; part < 3; ++part) {
+        hipError_t e = launch_scan_part(part, nblk, a, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {

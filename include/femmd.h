@@ -55,7 +55,10 @@ typedef struct fm_batch fm_batch;   /* many independent replicas */
 #define FM_LEGACY_GPU 2  /* computeFeaForcesGPU energy (= 2 x SVK)     */
 
 typedef struct fm_params {
-    double Lx, Ly;  /* periodic box (MDSystem::init, md3dsystem.cpp:49-80) */
+    /* periodic box (MDSystem::init, md3dsystem.cpp:49-80); held in FP32
+     * like the reference's Box (a real=float vector, def.h:86), so every
+     * wrap and image shift uses (float)Lx, (float)Ly */
+    double Lx, Ly;
     double dt;      /* time step; reference default 0.005                  */
     int32_t device; /* HIP device ordinal                                  */
     int32_t reserved[7];
@@ -160,8 +163,8 @@ int fm_pair_sets(fm_system *h, int32_t *cnt, int64_t *s1, int64_t *s2);
 /* Time one force pass on the device (ms per pass, averaged over reps). */
 int fm_time_forces(fm_system *h, int32_t reps, double *ms);
 /* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel with HIP
- * events on the engine's stream (no graph).  ms[5] = per step: kick + bin,
- * scan reduce, scan scatter, force pass, whole step. */
+ * events on the engine's stream (no graph).  ms[6] = per step: kick + bin,
+ * scan reduce, scan apply, scatter, force pass, whole step. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
 
 /* ---- batched replicas (no reference counterpart) ---- */

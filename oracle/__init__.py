@@ -140,16 +140,24 @@ class OracleSystem:
         return self
 
 
+def box_f32(L: float) -> float:
+    """The box length as the reference holds it (Box is a real=float vector,
+    femmd2D/src/def.h:86): every image shift uses this FP32 value."""
+    return float(np.float32(L))
+
+
 def from_workload(w, x=None, y=None) -> OracleSystem:
     """Oracle system of a configs.Workload at caller-order positions (x, y),
-    default the generated ones (wrapped in FP32 as the engine stores them)."""
+    default the generated ones (wrapped in FP32 as the engine stores them).
+    The box is the FP32 one the engine and the reference compute with."""
     ga = w.global_arrays()
+    Lx, Ly = box_f32(w.Lx), box_f32(w.Ly)
     if x is None:
         xf = ga["x"].astype(np.float32)
         yf = ga["y"].astype(np.float32)
-        wrap_f32(xf, yf, w.Lx, w.Ly)
+        wrap_f32(xf, yf, Lx, Ly)
         x, y = xf, yf
-    return OracleSystem(w.Lx, w.Ly, x, y, sigma=ga["sigma"], excl=ga["excl"], eps=[w.eps],
+    return OracleSystem(Lx, Ly, x, y, sigma=ga["sigma"], excl=ga["excl"], eps=[w.eps],
                         rc=[w.rc], ushift=[w.ushift], bonds=w.bonds, tri=ga["tri"],
                         refx=ga["refx"].astype(np.float32), refy=ga["refy"].astype(np.float32),
                         E=ga["E"], nu=ga["nu"], model=w.model)
@@ -191,17 +199,32 @@ def pair_sets(osys: OracleSystem, mode: int = 1):
 
 def forces(osys: OracleSystem, pred_f32: bool = True, all_pairs: bool = False,
            scale: bool = False):
-    """FP64 forces (fx, fy), energies[8] and, with scale=True, the per-particle
-    sum of force-term magnitudes."""
+    """FP64 forces (fx, fy), energies[8] and, with scale=True, an (2, n) array:
+    row 0 the per-particle sum of force-term magnitudes, row 1 the sum of
+    r |df/dr| over those terms (see oracle.h)."""
     n = osys.n
     fx = np.empty(n)
     fy = np.empty(n)
     en = np.empty(8)
-    fs = np.empty(n) if scale else None
+    fs = np.empty(2 * n) if scale else None
     rc = lib().or_forces(C.byref(osys.s), int(pred_f32), int(all_pairs), _dp(fx), _dp(fy), _dp(en),
                          _dp(fs))
     assert rc == 0, rc
-    return (fx, fy, en, fs) if scale else (fx, fy, en)
+    return (fx, fy, en, fs.reshape(2, n)) if scale else (fx, fy, en)
+
+
+def force_error(fx, fy, rx, ry, fs):
+    """Per-particle error of an FP32 force against the FP64 oracle:
+    |f - f_ref| / max(|f_ref|, sum_j |f_ij|, 0.1 * sum_j r_ij |df_ij/dr_ij|).
+
+    The second term is the scale of the cancellation in the net force; the
+    third is the force change that a relative change of 1e-6 (16 FP32 ulps)
+    in each separation produces, times 1e-5: below it a force is not
+    resolved by FP32 inputs at all (an unstrained element, a bond at rest, a
+    pair sitting at the zero of the LJ force)."""
+    d = np.hypot(np.asarray(fx, float) - rx, np.asarray(fy, float) - ry)
+    den = np.maximum(np.maximum(np.hypot(rx, ry), fs[0]), 0.1 * fs[1])
+    return d / np.where(den > 0, den, 1.0)
 
 
 def run(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt, pred_f32=False, history=False):

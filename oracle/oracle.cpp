@@ -210,7 +210,12 @@ void lj_pair64(const or_system *s, int i, int j, int sx, int sy, double *fx,
     double fc = c * dr6i * (dr6i - 0.5) * dr2i;
     fx[i] += fc * dx;
     fy[i] += fc * dy;
-    if (fs) fs[i] += std::fabs(fc) * std::sqrt(dr2);
+    if (fs) {
+        const double r = std::sqrt(dr2);
+        fs[i] += std::fabs(fc) * r;
+        // r |dF/dr| of F(r) = 48 eps (s6^2 - s6/2) / r, s6 = (sigma/r)^6
+        fs[s->n + i] += e * std::fabs(624.0 * dr6i * dr6i - 168.0 * dr6i) / r;
+    }
     double u = 4.0 * e * dr6i * (dr6i - 1.0) + s->ushift[k];
     en[OR_U_LJ] += 0.5 * u;
     en[OR_W_LJ] += 0.5 * fc * dr2;
@@ -233,7 +238,12 @@ void bonds64(const or_system *s, const double *x, const double *y, double *fx,
         double kdr = -s->bk[b] * dr / r;
         fx[i] += kdr * dx; fy[i] += kdr * dy;
         fx[j] -= kdr * dx; fy[j] -= kdr * dy;
-        if (fs) { fs[i] += std::fabs(kdr) * r; fs[j] += std::fabs(kdr) * r; }
+        if (fs) {
+            fs[i] += std::fabs(kdr) * r;
+            fs[j] += std::fabs(kdr) * r;
+            fs[s->n + i] += s->bk[b] * r;  // r dF/dr of F = k (r - r0)
+            fs[s->n + j] += s->bk[b] * r;
+        }
         en[OR_U_BOND] += 0.5 * s->bk[b] * dr * dr;
         en[OR_W_BOND] += kdr * (dx * dx + dy * dy);
     }
@@ -328,6 +338,12 @@ void cst64(const or_system *s, const double *x, const double *y, double *fx,
             fs[n0] += std::hypot(f0x, f0y);
             fs[n1] += std::hypot(f1x, f1y);
             fs[n2] += std::hypot(f2x, f2y);
+            // element stiffness x length: (lambda + 2 mu) |A0| |Bm|_F
+            const double bn = std::sqrt(B00 * B00 + B01 * B01 + B10 * B10 + B11 * B11);
+            const double kl = (std::fabs(lam) + 2 * std::fabs(mu)) * std::fabs(vol0) * bn;
+            fs[s->n + n0] += kl;
+            fs[s->n + n1] += kl;
+            fs[s->n + n2] += kl;
         }
         en[OR_U_FEA] += psi * vol0;
         en[OR_W_FEA] += f0x * r0x + f0y * r0y + f1x * r1x + f1y * r1y +
@@ -341,7 +357,7 @@ int32_t forces_impl(const or_system *s, int pred_f32, int all_pairs,
     std::fill(fx, fx + s->n, 0.0);
     std::fill(fy, fy + s->n, 0.0);
     std::fill(en, en + 8, 0.0);
-    if (fs) std::fill(fs, fs + s->n, 0.0);
+    if (fs) std::fill(fs, fs + 2 * (size_t)s->n, 0.0);
     // force order of femmd2D/src/interactions.cpp:96-102: bonds, FEA, pairs.
     if (s->nbond > 0) bonds64(s, s->x, s->y, fx, fy, en, fs);
     if (s->ntri > 0) cst64(s, s->x, s->y, fx, fy, en, fs);

@@ -93,8 +93,11 @@ int32_t or_pair_sets_f32(const or_system *s, int32_t mode, int32_t *cnt,
 /* FP64 forces and energies at the given positions.
  * pred_f32: take cutoff / image decisions in FP32 (positions must then be
  * FP32 values).  all_pairs: 1 = O(N^2), 0 = cell walk.  fscale (may be
- * NULL) receives per particle the sum of the magnitudes of its pair, bond
- * and element force terms (the scale of the cancellation in its net force).
+ * NULL, 2n doubles) receives per particle, in [0, n), the sum of the
+ * magnitudes of its pair, bond and element force terms (the scale of the
+ * cancellation in its net force) and, in [n, 2n), the sum over those terms
+ * of r |df/dr| (the force change per relative change of each separation:
+ * what an FP32 evaluation of the same inputs cannot resolve below).
  * Returns 0 or a negative error (bad input). */
 int32_t or_forces(const or_system *s, int32_t pred_f32, int32_t all_pairs,
                   double *fx, double *fy, double *energies, double *fscale);

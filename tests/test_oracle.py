@@ -87,6 +87,29 @@ def test_lj_energy_force_consistency_fd():
             assert g == pytest.approx(f[i], rel=1e-6, abs=1e-6)
 
 
+def test_force_scales_match_fd_of_term_magnitudes():
+    # row 1 of the oracle's force scale is sum_terms r |df/dr|: check it by
+    # central differences of the single-term force magnitude
+    for r in (0.95, 1.12, 1.6, 2.3):
+        x = np.array([10.0, 10.0 + r])
+        y = np.array([20.0, 20.0])
+        s = O.OracleSystem(50.0, 50.0, x, y, eps=[1.3], rc=[RC], ushift=[0.0])
+        _, _, _, fs = O.forces(s, pred_f32=False, scale=True)
+        h = 1e-6
+        def mag(rr):
+            s2 = O.OracleSystem(50.0, 50.0, [10.0, 10.0 + rr], y, eps=[1.3], rc=[RC], ushift=[0.0])
+            return O.forces(s2, pred_f32=False)[0][0]
+        dfdr = (mag(r + h) - mag(r - h)) / (2 * h)
+        assert fs[1][0] == pytest.approx(r * abs(dfdr), rel=1e-5)
+        assert fs[0][0] == pytest.approx(abs(mag(r)), rel=1e-12)
+    # bond: r dF/dr = k r
+    s = O.OracleSystem(30.0, 30.0, [5.0, 6.2], [5.0, 5.0], lj_on=False,
+                       bonds=([0], [1], [40.0], [1.0]))
+    _, _, _, fs = O.forces(s, pred_f32=False, scale=True)
+    assert fs[1][0] == pytest.approx(40.0 * 1.2, rel=1e-12)
+    assert fs[0][0] == pytest.approx(40.0 * 0.2, rel=1e-9)
+
+
 def test_lj_newton_third_law_and_virial_theorem():
     w = configs.c1(with_disk=False)
     s = O.from_workload(w)
