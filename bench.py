@@ -55,6 +55,43 @@ def algorithmic_bytes(n, ncells, ntri, nbond):
     return b
 
 
+def batched_leg(steps: int, loop_steps: int, tune_cache):
+    """C4: 512 replicas, N ~ logU[256, 32768]; one batched, graph-captured
+    step for all of them against looping the plain step over the replicas
+    (each replica stepping loop_steps with its own captured graph)."""
+    from femmd_old_amd import Batch, configs
+    t0 = time.time()
+    ws = configs.c4()
+    ntot = sum(w.n_total for w in ws)
+    systems = [configs.build_system(w, commit=False) for w in ws]
+    b = Batch(systems)
+    t_tune = time.time()
+    b.tune(tune_cache)
+    t_tune = time.time() - t_tune
+    b.run(20)
+    tb0 = time.perf_counter()
+    b.run(steps)
+    tb1 = time.perf_counter()
+    batched = ntot * steps / (tb1 - tb0)
+    for s in systems:  # warm-up also captures each replica's step graph
+        s.allocate_cuda()
+        s.run(loop_steps)
+    tl0 = time.perf_counter()
+    for s in systems:
+        s.run(loop_steps)
+    tl1 = time.perf_counter()
+    looped = ntot * loop_steps / (tl1 - tl0)
+    errs = b.device_errors()
+    print(f"[bench] c4: {len(ws)} replicas, {ntot} particles, setup {t_tune and time.time() - t0:.1f}s, "
+          f"tune {t_tune:.1f}s", file=sys.stderr, flush=True)
+    return {"replicas": len(ws), "particles": ntot, "steps": steps,
+            "batched_particle_steps_per_s": batched,
+            "batched_ms_per_step": 1e3 * (tb1 - tb0) / steps,
+            "looped_particle_steps_per_s": looped,
+            "looped_ms_per_step": 1e3 * (tl1 - tl0) / loop_steps,
+            "speedup": batched / looped, "device_errors": errs}
+
+
 def cpu_baseline(w, steps, threads):
     import numpy as np
     import oracle as O
@@ -80,6 +117,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests)")
     args = ap.parse_args()
 
@@ -140,6 +180,10 @@ def main():
         "energy_drift": (E1 - E0) / abs(E0),
         "roofline": roof,
     }
+    if not args.no_batched:
+        del s
+        out["batched"] = batched_leg(args.batched_steps, max(4, args.batched_steps // 10),
+                                     args.tune_cache)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, 1)
     print(json.dumps(out), flush=True)

@@ -408,9 +408,13 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
         int j = a0;
         if (j == b0) j = a1;
         if (j == b1) j = a2;
-        float fx = 0.f, fy = 0.f, uacc = 0.f, vir = 0.f, ush = 0.f;
+        // single-type accumulators: A = sum s12, B = sum s6 (s6 = (sigma/r)^6)
+        // give U = 4 eps (A - B) + n shift and W = 48 eps (A - B/2); the
+        // typed path accumulates eps-weighted terms per pair instead
+        float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f, ush = 0.f;
         int nint = 0;
         long long s1 = 0, s2 = 0;
+        const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
         for (int t = 0; t < total; ++t) {
             const float4 pj = L.ent[j];
             const int gj = __float_as_int(pj.z);
@@ -423,47 +427,57 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
                 if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
             }
             const float dr2 = dx * dx + dy * dy;
-            float rc2, s2ij, e48 = 0.f, e4 = 0.f, shift = 0.f;
+            float s2ij, e48 = 0.f, e4 = 0.f, shift = 0.f;
+            unsigned rc2m1;
             if (kTypes) {
                 const PairEntry pe = L.pt[ti * nt + (kTypes ? L.typ[j] : 0)];
                 e48 = pe.e48; e4 = pe.e4; shift = pe.shift;
                 const float sij = kSigma ? (si + pj.w) * 0.5f : si;
-                rc2 = pe.rc * pe.rc * sij * sij;
+                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
                 s2ij = sij * sij;
             } else if (kSigma) {
                 const float sij = (si + pj.w) * 0.5f;
                 const float rc = L.pt[0].rc;
-                rc2 = rc * rc * sij * sij;
+                rc2m1 = __float_as_uint(rc * rc * sij * sij) - 1u;
                 s2ij = sij * sij;
             } else {
-                rc2 = R.rc2_uniform;
+                rc2m1 = rc2m1_u;
                 s2ij = R.s2_uniform;
             }
-            const bool hit = (dr2 < rc2) & (j != me) & (gj != gix);
+            // 0 < dr2 < rc2 on the bit patterns of non-negative floats: the
+            // cutoff predicate dr2 < rc2 exactly, and the pair with itself
+            // (dr2 == 0, the only zero-distance pair of distinct slots in a
+            // valid configuration) falls out with it
+            const bool hit = ((__float_as_uint(dr2) - 1u) < rc2m1) & (gj != gix);
             if (hit) {
                 // force from the image-exact difference: the far coordinate
                 // is shifted by L before subtracting (exact by Sterbenz)
-                float dxa = dx, dya = dy;
+                float dxa = dx, dya = dy, r2a = dr2;
                 if (kWrap) {
                     if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
                     if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);
+                    r2a = __builtin_fmaf(dxa, dxa, dya * dya);
                 }
-                const float r2a = __builtin_fmaf(dxa, dxa, dya * dya);
                 const float inv = __builtin_amdgcn_rcpf(r2a);
                 const float sr2 = s2ij * inv;
-                const float sr6 = sr2 * sr2 * sr2;
-                const float fcv = sr6 * (sr6 - 0.5f) * inv;
-                const float fc = kTypes ? e48 * fcv : fcv;
-                fx = __builtin_fmaf(fc, dxa, fx);
-                fy = __builtin_fmaf(fc, dya, fy);
-                const float ue = sr6 * (sr6 - 1.0f);
+                const float sr4 = sr2 * sr2;
+                const float sr6 = sr4 * sr2;
+                const float s12 = sr6 * sr6;
+                const float tv = __builtin_fmaf(-0.5f, sr6, s12);  // s6 (s6 - 1/2)
                 if (kTypes) {
-                    uacc = __builtin_fmaf(e4, ue, uacc);
+                    const float fc = e48 * tv * inv;
+                    fx = __builtin_fmaf(fc, dxa, fx);
+                    fy = __builtin_fmaf(fc, dya, fy);
+                    accA = __builtin_fmaf(e4, s12 - sr6, accA);  // energy
+                    accB = __builtin_fmaf(e48, tv, accB);        // virial
                     ush += shift;
                 } else {
-                    uacc += ue;
+                    const float fc = tv * inv;
+                    fx = __builtin_fmaf(fc, dxa, fx);
+                    fy = __builtin_fmaf(fc, dya, fy);
+                    accA += s12;
+                    accB += sr6;
                 }
-                vir = __builtin_fmaf(fc, r2a, vir);
                 ++nint;
                 if (kDebug) {
                     const long long v = (long long)a.uid[a.perm[L.sp[j]]] + 1;
@@ -475,19 +489,23 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             if (j == b0) j = a1;
             if (j == b1) j = a2;
         }
+        double Ui, Wi;
         if (!kTypes) {
             const PairEntry pe = L.pt[0];
             fx *= pe.e48;
             fy *= pe.e48;
-            vir *= pe.e48;
-            uacc = pe.e4 * uacc;
-            ush = (float)nint * pe.shift;
+            const double A = accA, B = accB;
+            Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
+            Wi = (double)pe.e48 * (A - 0.5 * B);
+        } else {
+            Ui = (double)accA + (double)ush;
+            Wi = (double)accB;
         }
         const int id = a.perm[sbase + k];
         a.fx[id] = fx;
         a.fy[id] = fy;
-        Ublk += 0.5 * ((double)uacc + (double)ush);
-        Wblk += 0.5 * (double)vir;
+        Ublk += 0.5 * Ui;
+        Wblk += 0.5 * Wi;
         if (kDebug) {
             const int u = a.uid[id];
             a.dbg_cnt[u] = nint;
