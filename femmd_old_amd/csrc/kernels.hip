@@ -452,11 +452,14 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             if (hit) {
                 // force from the image-exact difference: the far coordinate
                 // is shifted by L before subtracting (exact by Sterbenz)
+                // (same unfused expression as dr2: an unshifted pair gets the
+                // same bits on interior and boundary tiles, so results do not
+                // depend on the tile shape)
                 float dxa = dx, dya = dy, r2a = dr2;
                 if (kWrap) {
                     if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
                     if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);
-                    r2a = __builtin_fmaf(dxa, dxa, dya * dya);
+                    r2a = dxa * dxa + dya * dya;
                 }
                 const float inv = __builtin_amdgcn_rcpf(r2a);
                 const float sr2 = s2ij * inv;
