@@ -37,6 +37,10 @@ def _ip(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+def _dp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
 def _f32(a, n=None) -> np.ndarray:
     a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
     if n is not None and a.shape[0] != n:
@@ -69,6 +73,7 @@ class System:
         self._dt = float(dt)
         self._kbond = 0.0
         self._committed = False
+        self._nelem = 0
 
     # -- plumbing -------------------------------------------------------------
     def _check(self, rc: int, h="self"):
@@ -140,7 +145,12 @@ class System:
             self._h, nv, _fp(v), nt, _ip(t), int(index_base), float(x), float(y), int(typ),
             float(sig), float(node_mass), float(mass), int(bool(exclude)), _fp(Ea), _fp(nua),
             C.byref(first)))
+        self._nelem += nt
         return first.value
+
+    @property
+    def n_elements(self) -> int:
+        return self._nelem
 
     def set_kBond(self, k: float):
         """System.set_kBond (femmd2D/src/interface.cpp:455-465): the stiffness
@@ -308,6 +318,39 @@ class System:
         self._check(self._lib.fm_profile_step(self._h, int(nsteps), ms))
         return dict(zip(("kick", "scan_reduce", "scan_apply", "scatter", "forces", "step"),
                         list(ms)))
+
+    # -- adjoint (no reference counterpart) -----------------------------------
+    def force_jvp(self, wx, wy):
+        """(jx, jy, dE, deps, dsig): j = (df/dx) w at the current positions and
+        the parameter terms w . df/dE_e (per element), w . df/deps,
+        w . df/dsigma (common pair eps and sigma)."""
+        n = self.N
+        wx, wy = _f32(wx, n), _f32(wy, n)
+        jx, jy = np.empty(n, np.float32), np.empty(n, np.float32)
+        dE = np.empty(self._nelem, np.float64)
+        deps, dsig = C.c_double(), C.c_double()
+        self._check(self._lib.fm_force_jvp(self._h, _fp(wx), _fp(wy), _fp(jx), _fp(jy),
+                                           _dp(dE), C.byref(deps), C.byref(dsig)))
+        return jx, jy, dE, deps.value, dsig.value
+
+    def adjoint(self, nsteps: int, ids, targets, ckpt: int = 0) -> dict:
+        """Runs nsteps forward and returns the loss
+        (1/K) sum_k |x_{ids[k]}(T) - targets[k]|^2 with its gradient
+        {"loss", "dE" (per element), "deps", "dsigma"} from the discrete
+        adjoint (checkpoints every ckpt steps, default ~sqrt(nsteps))."""
+        ids = _i32(ids)
+        t = np.asarray(targets, np.float64).reshape(-1, 2)
+        if t.shape[0] != ids.shape[0]:
+            raise ValueError("one target per loss particle")
+        tx, ty = _f32(t[:, 0]), _f32(t[:, 1])
+        if ckpt <= 0:
+            ckpt = max(1, int(round(np.sqrt(nsteps))))
+        loss, deps, dsig = C.c_double(), C.c_double(), C.c_double()
+        dE = np.empty(self._nelem, np.float64)
+        self._check(self._lib.fm_adjoint(self._h, int(nsteps), int(ckpt), ids.shape[0], _ip(ids),
+                                         _fp(tx), _fp(ty), C.byref(loss), _dp(dE),
+                                         C.byref(deps), C.byref(dsig)))
+        return {"loss": loss.value, "dE": dE, "deps": deps.value, "dsigma": dsig.value}
 
 
 class Batch:

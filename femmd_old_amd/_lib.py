@@ -98,6 +98,9 @@ PROTOTYPES = {
     "fm_pair_sets": (C.c_int, [_h, _i, _l, _l]),
     "fm_time_forces": (C.c_int, [_h, C.c_int32, _d]),
     "fm_profile_step": (C.c_int, [_h, C.c_int32, _d]),
+    "fm_force_jvp": (C.c_int, [_h, _f, _f, _f, _f, _d, _d, _d]),
+    "fm_adjoint": (C.c_int, [_h, C.c_int32, C.c_int32, C.c_int32, _i, _f, _f, _d, _d, _d,
+                             _d]),
     "fm_batch_create": (C.c_int, [C.c_int32, C.POINTER(_h), C.POINTER(_h)]),
     "fm_batch_destroy": (None, [_h]),
     "fm_batch_last_error": (C.c_char_p, [_h]),

@@ -1,0 +1,292 @@
+// adjoint.hip -- discrete adjoint of the KDK step with checkpointed
+// recomputation (no reference counterpart).
+//
+// Forward step n:  v' = v_n + h M^-1 f(x_n);  x_{n+1} = x_n + dt v';
+//                  v_{n+1} = v' + h M^-1 f(x_{n+1}),   h = dt / 2.
+// With xb = dL/dx, vb = dL/dv and the two half-kicks that share f(x_n)
+// merged, the backward sweep from xb = dL/dx_N, vb = 0 is, for n = N-1..0:
+//     vb += dt xb                          (drift of step n)
+//     w   = c_n M^-1 vb,  c_n = dt (n > 0), h (n = 0)
+//     xb += J(x_n) w,  theta_b += (df/dtheta)(x_n)^T w
+// J = df/dx is symmetric, so J^T w = J w.  The wrap is the identity
+// almost everywhere.  Positions x_n are recomputed per segment from
+// checkpoints (x, v) taken every `ckpt` steps; parameter gradients
+// accumulate in FP64 (per element for E, per force tile for eps, sigma).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace fm {
+
+int Engine::adj_alloc(std::string &err) {
+    if (R_ != 1) { err = "the adjoint is available for single systems"; return FM_ERR_STATE; }
+    if (types_) { err = "the adjoint supports one particle type"; return FM_ERR_STATE; }
+    if ((size_t)lds_cap_ * (sizeof(float4) + sizeof(float2)) > 65536) {
+        err = "adjoint tiles exceed the LDS: choose a smaller tile width";
+        return FM_ERR_STATE;
+    }
+    hipError_t he;
+    const size_t ntiles = std::max<size_t>(tiles_h_.size(), 1);
+    if (pacc_.n != 2 * ntiles && (he = pacc_.alloc(2 * ntiles)) != hipSuccess)
+        return check(he, "alloc", err);
+    if (w_.p) return FM_OK;
+    const size_t N = N_;
+#define FM_AL(buf, cnt) \
+    if ((he = buf.alloc(cnt)) != hipSuccess) return check(he, "alloc " #buf, err)
+    FM_AL(w_, N);
+    FM_AL(jx_, N);
+    FM_AL(jy_, N);
+    FM_AL(xbx_, N);
+    FM_AL(xby_, N);
+    FM_AL(vbx_, N);
+    FM_AL(vby_, N);
+    FM_AL(cfa_, std::max(1, 3 * ntri_ + 2 * nbond_));
+    FM_AL(dE_, std::max(1, ntri_));
+#undef FM_AL
+    // particles outside every force tile (LJ off) keep a zero pair part
+    if ((he = hipMemsetAsync(jx_.p, 0, N * sizeof(float), stream_)) != hipSuccess)
+        return check(he, "memset", err);
+    if ((he = hipMemsetAsync(jy_.p, 0, N * sizeof(float), stream_)) != hipSuccess)
+        return check(he, "memset", err);
+    return FM_OK;
+}
+
+int Engine::adj_pass(std::string &err) {
+    if (launch_bin()) return check(hipGetLastError(), "bin", err);
+    if (launch_sort()) return check(hipGetLastError(), "sort", err);
+    AdjArgs A{};
+    A.f = force_args();
+    A.w = w_.p;
+    A.jx = jx_.p;
+    A.jy = jy_.p;
+    A.cfa = cfa_.p;
+    A.eE = eE_.p;
+    A.dE = dE_.p;
+    A.pacc = pacc_.p;
+    if (launch_adj_forces(sigma_, block_, A, stream_) != hipSuccess)
+        return check(hipGetLastError(), "adjoint forces", err);
+    return FM_OK;
+}
+
+int Engine::adj_kick(bool accumulate, bool drift, bool make_w, float c) {
+    AdjKickArgs k{};
+    k.reps = reps_.p;
+    k.spans = kspans_.p;
+    k.xbx = xbx_.p;
+    k.xby = xby_.p;
+    k.vbx = vbx_.p;
+    k.vby = vby_.p;
+    k.jx = jx_.p;
+    k.jy = jy_.p;
+    k.cfa = cfa_.p;
+    k.goff = gather_ ? goff_.p : nullptr;
+    k.gent = gent_.p;
+    k.inv_m = inv_m_.p;
+    k.w = w_.p;
+    k.c = c;
+    k.accumulate = accumulate;
+    k.drift = drift;
+    k.make_w = make_w;
+    return launch_adj_kick((int)kspans_h_.size(), k, stream_) == hipSuccess ? 0 : -1;
+}
+
+int Engine::save_state(float *dst) {
+    const size_t b = (size_t)N_ * sizeof(float);
+    const float *src[4] = {x_.p, y_.p, vx_.p, vy_.p};
+    for (int q = 0; q < 4; ++q)
+        if (hipMemcpyAsync(dst + (size_t)q * N_, src[q], b, hipMemcpyDeviceToDevice, stream_) !=
+            hipSuccess)
+            return -1;
+    return 0;
+}
+
+int Engine::load_state(const float *src) {
+    const size_t b = (size_t)N_ * sizeof(float);
+    float *dst[4] = {x_.p, y_.p, vx_.p, vy_.p};
+    for (int q = 0; q < 4; ++q)
+        if (hipMemcpyAsync(dst[q], src + (size_t)q * N_, b, hipMemcpyDeviceToDevice, stream_) !=
+            hipSuccess)
+            return -1;
+    primed_ = false;
+    return 0;
+}
+
+namespace {
+double sum_pairs(const std::vector<double> &v, int which) {
+    double s = 0.0;
+    for (size_t k = which; k < v.size(); k += 2) s += v[k];
+    return s;
+}
+}  // namespace
+
+int Engine::adjoint_op(const float *wx, const float *wy, float *jx, float *jy, double *dE,
+                       double *deps, double *dsig, std::string &err) {
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = adj_alloc(err);
+    if (rc) return rc;
+    const int N = N_;
+    const std::vector<int32_t> &iid = iid_h_[0];
+    std::vector<float2> wh(N);
+    for (int u = 0; u < N; ++u) wh[iid[u]] = make_float2(wx[u], wy[u]);
+    hipError_t he;
+    if ((he = hipMemcpyAsync(w_.p, wh.data(), N * sizeof(float2), hipMemcpyHostToDevice,
+                             stream_)) != hipSuccess)
+        return check(he, "upload w", err);
+    if ((he = hipMemsetAsync(dE_.p, 0, dE_.n * sizeof(double), stream_)) != hipSuccess)
+        return check(he, "memset", err);
+    if ((he = hipMemsetAsync(pacc_.p, 0, pacc_.n * sizeof(double), stream_)) != hipSuccess)
+        return check(he, "memset", err);
+    rc = adj_pass(err);
+    if (rc) return rc;
+    primed_ = false;  // the forward force buffers were not refreshed
+    if (launch_total_force(N, jx_.p, jy_.p, cfa_.p, gather_ ? goff_.p : nullptr, gent_.p,
+                           outx_.p, outy_.p, stream_) != hipSuccess)
+        return check(hipGetLastError(), "total", err);
+    std::vector<float> bx(N), by(N);
+    std::vector<double> de(dE_.n), pa(pacc_.n);
+    (void)hipMemcpyAsync(bx.data(), outx_.p, N * sizeof(float), hipMemcpyDeviceToHost, stream_);
+    (void)hipMemcpyAsync(by.data(), outy_.p, N * sizeof(float), hipMemcpyDeviceToHost, stream_);
+    (void)hipMemcpyAsync(de.data(), dE_.p, de.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_);
+    (void)hipMemcpyAsync(pa.data(), pacc_.p, pa.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_);
+    rc = sync(err);
+    if (rc) return rc;
+    for (int u = 0; u < N; ++u) {
+        if (jx) jx[u] = bx[iid[u]];
+        if (jy) jy[u] = by[iid[u]];
+    }
+    if (dE)
+        for (int e = 0; e < ntri_; ++e) dE[ecaller_h_[e]] = de[e];
+    if (deps) *deps = sum_pairs(pa, 0);
+    if (dsig) *dsig = sum_pairs(pa, 1);
+    return FM_OK;
+}
+
+int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
+                        const float *ty, double *loss, double *dE, double *deps, double *dsig,
+                        std::string &err) {
+    if (nsteps < 1 || ckpt < 1 || nloss < 1 || !ids || !tx || !ty) {
+        err = "nsteps, ckpt and the loss set must be non-empty";
+        return FM_ERR_ARG;
+    }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = adj_alloc(err);
+    if (rc) return rc;
+    const int N = N_;
+    const std::vector<int32_t> &iid = iid_h_[0];
+    std::vector<int32_t> lid(nloss);
+    std::vector<float2> tgt(nloss);
+    for (int k = 0; k < nloss; ++k) {
+        if (ids[k] < 0 || ids[k] >= N) { err = "loss particle out of range"; return FM_ERR_ARG; }
+        lid[k] = iid[ids[k]];
+        tgt[k] = make_float2(tx[k], ty[k]);
+    }
+    const int S = (nsteps + ckpt - 1) / ckpt;
+    hipError_t he;
+    if (ckpt_.n < (size_t)(S + 1) * 4 * N && (he = ckpt_.alloc((size_t)(S + 1) * 4 * N)) != hipSuccess)
+        return check(he, "alloc checkpoints", err);
+    if (hist_.n < (size_t)ckpt * 2 * N && (he = hist_.alloc((size_t)ckpt * 2 * N)) != hipSuccess)
+        return check(he, "alloc history", err);
+    if ((he = loss_ids_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
+    if ((he = loss_t_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
+    const int nlb = (nloss + 255) / 256;
+    if ((he = loss_part_.alloc(nlb)) != hipSuccess) return check(he, "alloc", err);
+    (void)hipMemcpyAsync(loss_ids_.p, lid.data(), nloss * sizeof(int32_t), hipMemcpyHostToDevice,
+                         stream_);
+    (void)hipMemcpyAsync(loss_t_.p, tgt.data(), nloss * sizeof(float2), hipMemcpyHostToDevice,
+                         stream_);
+    rc = ensure_primed(err);
+    if (rc) return rc;
+    // ---- forward, checkpoint at every segment start and at the end ----
+    for (int s = 0; s < S; ++s) {
+        if (save_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "ckpt", err);
+        const int len = std::min(ckpt, nsteps - s * ckpt);
+        if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
+        for (int k = 1; k < len; ++k)
+            if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+        if (launch_step_body(true, false)) return check(hipGetLastError(), "kick", err);
+    }
+    steps_ += nsteps;
+    float *final_state = ckpt_.p + (size_t)S * 4 * N;
+    if (save_state(final_state)) return check(hipGetLastError(), "ckpt", err);
+    // ---- loss and its gradient ----
+    const size_t fb = (size_t)N * sizeof(float);
+    (void)hipMemsetAsync(xbx_.p, 0, fb, stream_);
+    (void)hipMemsetAsync(xby_.p, 0, fb, stream_);
+    (void)hipMemsetAsync(vbx_.p, 0, fb, stream_);
+    (void)hipMemsetAsync(vby_.p, 0, fb, stream_);
+    (void)hipMemsetAsync(dE_.p, 0, dE_.n * sizeof(double), stream_);
+    (void)hipMemsetAsync(pacc_.p, 0, pacc_.n * sizeof(double), stream_);
+    LossArgs la{};
+    la.reps = reps_.p;
+    la.n = nloss;
+    la.ids = loss_ids_.p;
+    la.target = loss_t_.p;
+    la.x = x_.p;
+    la.y = y_.p;
+    la.xbx = xbx_.p;
+    la.xby = xby_.p;
+    la.scale = 2.0f / (float)nloss;
+    la.part = loss_part_.p;
+    if (launch_adj_loss(la, stream_) != hipSuccess) return check(hipGetLastError(), "loss", err);
+    std::vector<double> lp(nlb);
+    (void)hipMemcpyAsync(lp.data(), loss_part_.p, nlb * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_);
+    // ---- backward sweep, segment by segment ----
+    const float dt = reps_h_[0].dt, h = reps_h_[0].hdt;
+    bool first = true;
+    for (int s = S - 1; s >= 0; --s) {
+        const int len = std::min(ckpt, nsteps - s * ckpt);
+        if (load_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "restore", err);
+        rc = ensure_primed(err);
+        if (rc) return rc;
+        // recompute x_{sC} .. x_{sC+len-1} of this segment
+        for (int k = 0; k < len; ++k) {
+            if (k > 0 && launch_step_body(k > 1, true)) return check(hipGetLastError(), "step", err);
+            float *hx = hist_.p + (size_t)k * 2 * N;
+            if (hipMemcpyAsync(hx, x_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess ||
+                hipMemcpyAsync(hx + N, y_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess)
+                return check(hipGetLastError(), "history", err);
+        }
+        for (int k = len - 1; k >= 0; --k) {
+            const int n = s * ckpt + k;
+            if (first) {
+                if (adj_kick(false, true, true, n > 0 ? dt : h)) return check(hipGetLastError(), "adj kick", err);
+                first = false;
+            }
+            xalt_ = hist_.p + (size_t)k * 2 * N;
+            yalt_ = xalt_ + N;
+            rc = adj_pass(err);
+            xalt_ = yalt_ = nullptr;
+            if (rc) return rc;
+            const bool more = n > 0;
+            if (adj_kick(true, more, more, (n - 1 > 0) ? dt : h))
+                return check(hipGetLastError(), "adj kick", err);
+        }
+    }
+    // ---- restore the state at the end of the forward pass ----
+    if (load_state(final_state)) return check(hipGetLastError(), "restore", err);
+    rc = ensure_primed(err);
+    if (rc) return rc;
+    std::vector<double> de(dE_.n), pa(pacc_.n);
+    (void)hipMemcpyAsync(de.data(), dE_.p, de.size() * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    (void)hipMemcpyAsync(pa.data(), pacc_.p, pa.size() * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    rc = sync(err);
+    if (rc) return rc;
+    double l = 0.0;
+    for (double v : lp) l += v;
+    if (loss) *loss = l / nloss;
+    if (dE)
+        for (int e = 0; e < ntri_; ++e) dE[ecaller_h_[e]] = de[e];
+    if (deps) *deps = sum_pairs(pa, 0);
+    if (dsig) *dsig = sum_pairs(pa, 1);
+    return FM_OK;
+}
+
+}  // namespace fm

@@ -406,6 +406,24 @@ int fm_profile_step(fm_system *h, int32_t nsteps, double *ms) {
     FM_END(h)
 }
 
+int fm_force_jvp(fm_system *h, const float *wx, const float *wy, float *jx, float *jy,
+                 double *dE, double *deps, double *dsig) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    if (!wx || !wy) return fail(h, FM_ERR_ARG, "wx and wy are required");
+    return h->eng->adjoint_op(wx, wy, jx, jy, dE, deps, dsig, h->err);
+    FM_END(h)
+}
+
+int fm_adjoint(fm_system *h, int32_t nsteps, int32_t ckpt, int32_t nloss, const int32_t *ids,
+               const float *tx, const float *ty, double *loss, double *dE, double *deps,
+               double *dsig) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->adjoint_run(nsteps, ckpt, nloss, ids, tx, ty, loss, dE, deps, dsig, h->err);
+    FM_END(h)
+}
+
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------

@@ -35,7 +35,8 @@ struct alignas(16) RepDesc {
     int32_t ntypes;
     float rc2_uniform; // ((rc*rc)*s)*s for single-type uniform-sigma systems
     float s2_uniform;  // sigma^2 for uniform sigma
-    int32_t pad[3];
+    float s_uniform;   // sigma for uniform sigma
+    int32_t pad[2];
 };
 
 // One pair-table entry: 48 eps, 4 eps, additive shift, cutoff in sigma units.
@@ -124,6 +125,46 @@ struct ForceArgs {
     int32_t *dbg_cnt;
     long long *dbg_s1, *dbg_s2;
     uint32_t *err;
+};
+
+// Adjoint pass: J(x) w (J = df/dx, the negative Hessian of the potential)
+// and the parameter terms (df/dtheta)^T w at one configuration.
+struct AdjArgs {
+    ForceArgs f;          // forward inputs (binning, tiles, elements, bonds)
+    const float2 *w;      // w per particle, internal order
+    float *jx, *jy;       // pair part of J w, internal order
+    float2 *cfa;          // element / bond corner parts of J w
+    const float *eE;      // Young's modulus per element (forces are linear in E)
+    double *dE;           // per element: += w . f_e / E_e
+    double *pacc;         // per tile: += sum_i w_i . f_i / eps, += w . df/dsigma
+};
+
+// Backward half-kicks / drift: xb += J w (accumulate), vb += dt xb (drift),
+// w = c inv_m vb (make_w); one workgroup per kick span.
+struct AdjKickArgs {
+    const RepDesc *reps;
+    const Span *spans;
+    float *xbx, *xby, *vbx, *vby;
+    const float *jx, *jy;
+    const float2 *cfa;
+    const int32_t *goff, *gent;
+    const float *inv_m;
+    float2 *w;
+    float c;
+    int32_t accumulate, drift, make_w;
+};
+
+// Loss = (1/K) sum_k |x_k - t_k|^2 over K particles (minimum image);
+// seeds xb = dL/dx (other particles keep xb = 0).
+struct LossArgs {
+    const RepDesc *reps;
+    int32_t n;
+    const int32_t *ids;   // internal ids of the loss particles
+    const float2 *target;
+    const float *x, *y;
+    float *xbx, *xby;
+    float scale;          // 2 / K
+    double *part;         // per workgroup partial sum of |x - t|^2
 };
 
 }  // namespace fm

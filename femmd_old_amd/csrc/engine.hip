@@ -157,6 +157,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         const float rcf = S.rc[0];
         D.rc2_uniform = rcf * rcf * s0 * s0;  // reference order ((rc*rc)*s)*s
         D.s2_uniform = s0 * s0;
+        D.s_uniform = s0;
         amax = std::max(amax, (double)(D.Lx / D.ncx) * (D.Ly / D.ncy));
         rho_cp = std::max(rho_cp, 1.16 / ((double)lo * lo));
     }
@@ -171,6 +172,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     std::vector<int32_t> ht0, ht1, ht2, hbi, hbj;
     std::vector<float4> hdmbm;
     std::vector<float2> halm;
+    std::vector<float> heE;
     std::vector<float> hbk, hbr0;
     std::vector<Span> ks, es, bs;
     std::vector<int32_t> erange(2 * R), brange(2 * R);
@@ -251,6 +253,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             hdmbm.push_back(make_float4((float)B00, (float)B01, (float)B10, (float)B11));
             const double Ee = S.tE[e] >= 0 ? S.tE[e] : S.E;
             const double nue = S.tnu[e] >= 0 ? S.tnu[e] : S.nu;
+            E->ecaller_h_.push_back(e);
+            heE.push_back((float)Ee);
             // femmd2D/src/interactions.cpp:44-45
             double lam = Ee * nue / ((1 - 2 * nue) * (1 + nue));
             double mu = Ee / (2 * (1 + nue));
@@ -334,6 +338,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_UP(t2_, ht2);
     FM_UP(dmbm_, hdmbm);
     FM_UP(alm_, halm);
+    FM_UP(eE_, heE);
     FM_UP(bi_, hbi);
     FM_UP(bj_, hbj);
     FM_UP(bk_, hbk);
@@ -492,8 +497,8 @@ int Engine::launch_bin() {
     KickArgs a{};
     a.reps = reps_.p;
     a.spans = kspans_.p;
-    a.x = x_.p;
-    a.y = y_.p;
+    a.x = xcur();
+    a.y = ycur();
     a.count = count_.p + (size_t)parity_ * ncells_;
     a.key = key_.p;
     a.tmp = tmp_.p;
@@ -559,19 +564,31 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     return launch_forces_only(false);
 }
 
-int Engine::launch_forces_only(bool debug) {
-    hipError_t e;
+// Counting sort of the last binning (scan reduce, scan apply, scatter).
+int Engine::launch_sort() {
     const int nscan = (ncells_ + kScanCells - 1) / kScanCells;
     ScanArgs s = make_scan(N_, ncells_, cap_, count_.p + (size_t)parity_ * ncells_,
-                           count_.p + (size_t)(1 - parity_) * ncells_, key_.p, tmp_.p, bsum_.p, start_.p, perm_.p, x_.p,
-                           y_.p, grp_.p, sigma_ ? sig_.p : nullptr, types_ ? typ_.p : nullptr,
-                           sorted_.p, types_ ? sorted_typ_.p : nullptr);
+                           count_.p + (size_t)(1 - parity_) * ncells_, key_.p, tmp_.p, bsum_.p,
+                           start_.p, perm_.p, xcur(), ycur(), grp_.p, sigma_ ? sig_.p : nullptr,
+                           types_ ? typ_.p : nullptr, sorted_.p,
+                           types_ ? sorted_typ_.p : nullptr);
     for (int part = 0; part < 3; ++part) {
-        e = launch_scan_part(part, nscan, s, stream_);
-        if (e != hipSuccess) return -1;
+        if (launch_scan_part(part, nscan, s, stream_) != hipSuccess) return -1;
         if (prof_ev_) (void)hipEventRecord(prof_ev_[2 + part], stream_);
     }
     parity_ ^= 1;  // the next binning uses the other (now zeroed) counters
+    return 0;
+}
+
+int Engine::launch_forces_only(bool debug) {
+    if (launch_sort()) return -1;
+    const ForceArgs f = force_args();
+    const hipError_t e = launch_forces(sigma_, types_, debug, block_, f, stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
+    return e == hipSuccess ? 0 : -1;
+}
+
+ForceArgs Engine::force_args() {
     ForceArgs f{};
     f.reps = reps_.p;
     f.ptab = ptab_.p;
@@ -593,8 +610,8 @@ int Engine::launch_forces_only(bool debug) {
     f.dmbm = dmbm_.p;
     f.alm = alm_.p;
     f.model = (model_ == FM_NEO_HOOKEAN) ? 0 : 1;
-    f.x = x_.p;
-    f.y = y_.p;
+    f.x = xcur();
+    f.y = ycur();
     f.cf = cf_.p;
     f.fea_part = fea_part_.p;
     f.bspans = bspans_.p;
@@ -610,9 +627,7 @@ int Engine::launch_forces_only(bool debug) {
     f.dbg_s1 = dbg_s1_.p;
     f.dbg_s2 = dbg_s2_.p;
     f.err = err_.p;
-    e = launch_forces(sigma_, types_, debug, block_, f, stream_);
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
-    return e == hipSuccess ? 0 : -1;
+    return f;
 }
 
 // Per-kernel device time of nsteps step bodies (no graph): HIP events on

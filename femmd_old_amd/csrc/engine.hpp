@@ -84,6 +84,14 @@ public:
     int pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err);
     int time_forces(int reps, double *ms, std::string &err);
     int profile(int nsteps, double *out, std::string &err);
+    // adjoint (adjoint.hip): J(x) w and (df/dtheta)^T w at the current
+    // positions, caller order; and a checkpointed forward + backward sweep
+    int adjoint_op(const float *wx, const float *wy, float *jx, float *jy, double *dE,
+                   double *deps, double *dsig, std::string &err);
+    int adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
+                    const float *ty, double *loss, double *dE, double *deps, double *dsig,
+                    std::string &err);
+    int ntri() const { return ntri_; }
     int set_tuning(int tile_w, int block, int use_graph, std::string &err);
     uint32_t errors();
     int clear_errors();
@@ -105,6 +113,17 @@ private:
     int launch_bin();
     int launch_step_body(bool kick2, bool drift);
     int launch_forces_only(bool debug);
+    int launch_sort();
+    ForceArgs force_args();
+    // positions the binning / force kernels read: the live state, or a
+    // stored configuration during the adjoint sweep
+    float *xcur() const { return xalt_ ? xalt_ : x_.p; }
+    float *ycur() const { return yalt_ ? yalt_ : y_.p; }
+    int adj_alloc(std::string &err);
+    int adj_pass(std::string &err);  // bin + sort + adjoint force kernel at xcur
+    int adj_kick(bool accumulate, bool drift, bool make_w, float c);
+    int save_state(float *dst);
+    int load_state(const float *src);
     int build_tiles(const std::vector<int> &tile_w, std::string &err);
     int ensure_primed(std::string &err);
     int check(hipError_t e, const char *what, std::string &err);
@@ -158,6 +177,17 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     int graph_steps_ = 0;
+
+    // adjoint state
+    float *xalt_ = nullptr, *yalt_ = nullptr;
+    std::vector<int32_t> ecaller_h_;  // internal element -> caller element index
+    DBuf<float> eE_;                  // Young's modulus per element
+    DBuf<float2> w_, cfa_;
+    DBuf<float> jx_, jy_, xbx_, xby_, vbx_, vby_;
+    DBuf<double> dE_, pacc_, loss_part_;
+    DBuf<float> ckpt_, hist_;
+    DBuf<int32_t> loss_ids_;
+    DBuf<float2> loss_t_;
 };
 
 // launchers (kernels.hip)
@@ -173,5 +203,8 @@ hipError_t launch_total_force(int n, const float *fx, const float *fy, const flo
 hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
                          const double *fea, const double *bond, double *out, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
+hipError_t launch_adj_forces(bool sigma, int block, const AdjArgs &A, hipStream_t s);
+hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s);
+hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s);
 
 }  // namespace fm

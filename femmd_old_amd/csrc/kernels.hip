@@ -523,16 +523,16 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
     }
 }
 
-template <bool kSigma, bool kTypes, bool kDebug>
-__device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
-    const Tile T = a.tiles[tile];
-    const RepDesc R = a.reps[T.rep];
-    const int W = T.w_flags & 0xffff;
-    const bool wrapT = (T.w_flags >> 16) & 1;
+// Stages the 3 x (W+2) cells of a tile into LDS: rows cy-1, cy, cy+1 and
+// columns cx0-1 .. cx0+W with the periodic wrap of femmd2D/src/nnCuda.cu:
+// 26-37; sorted positions are contiguous per cell.  With wsrc set (the
+// adjoint pass) the per-particle vector w is staged beside each entry.
+// Returns the number of staged particles, or -1 if they exceed the LDS.
+template <bool kTypes, bool kDebug>
+__device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
+                          PairLds &L, const float2 *wsrc, float2 *lw) {
     const int row = W + 2, S = 3 * row;
     const int tid = threadIdx.x;
-    // slot table: rows cy-1, cy, cy+1; columns cx0-1 .. cx0+W (periodic,
-    // femmd2D/src/nnCuda.cu:26-37)
     for (int s = tid; s < S; s += blockDim.x) {
         const int r = s / row, col = s - r * row;
         int cy = T.cy + r - 1;
@@ -569,15 +569,7 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
     }
     __syncthreads();
     const int M = L.off[S];
-    if (M > a.lds_cap) {
-        if (tid == 0) {
-            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
-            a.lj_part[2 * tile] = 0.0;
-            a.lj_part[2 * tile + 1] = 0.0;
-        }
-        return;
-    }
-    // stage the 3 x (W+2) cells: sorted positions are contiguous per cell
+    if (M > a.lds_cap) return -1;
     for (int e = tid; e < M; e += blockDim.x) {
         int lo = 0, hi = S;
         while (hi - lo > 1) {
@@ -588,8 +580,26 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
         L.ent[e] = a.sorted[p];
         if (kTypes) L.typ[e] = a.sorted_typ[p];
         if (kDebug) L.sp[e] = p;
+        if (wsrc) lw[e] = wsrc[a.perm[p]];
     }
     __syncthreads();
+    return M;
+}
+
+template <bool kSigma, bool kTypes, bool kDebug>
+__device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    const bool wrapT = (T.w_flags >> 16) & 1;
+    if (stage_tile<kTypes, kDebug>(a, T, R, W, L, nullptr, nullptr) < 0) {
+        if (threadIdx.x == 0) {
+            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+            a.lj_part[2 * tile] = 0.0;
+            a.lj_part[2 * tile + 1] = 0.0;
+        }
+        return;
+    }
     if (wrapT)
         pair_tile_body<true, kSigma, kTypes, kDebug>(a, R, W, tile, L);
     else
@@ -835,9 +845,372 @@ __global__ void k_fill_int(int32_t *p, int32_t v, size_t n) {
     if (i < n) p[i] = v;
 }
 
+// ===========================================================================
+// Adjoint pass (no reference counterpart).  With f_i = sum_j g(s) d_ij,
+// s = |d_ij|^2, the directional derivative of the pair forces is
+//   (J w)_i = sum_j [ g (w_i - w_j) + 2 g'(s) d_ij (d_ij . (w_i - w_j)) ]
+// (the full-list loop of the forward pass, with w staged beside x).  For
+// LJ, with u = sigma^2 / s, s6 = u^3, s12 = s6^2:
+//   g = 48 eps (s12 - s6/2) / s,  g' = 48 eps (2 s6 - 7 s12) / s^2,
+//   dg/dsigma = 48 eps (12 s12 - 3 s6) / (s sigma),  dg/deps = g / eps.
+// ===========================================================================
+template <bool kWrap, bool kSigma>
+__device__ void adj_pair_body(const AdjArgs &A, const RepDesc &R, int W, int tile, PairLds &L,
+                              const float2 *lw) {
+    const ForceArgs &a = A.f;
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const PairEntry pe = L.pt[0];
+    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    double peps = 0.0, psig = 0.0;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int me = c_lo + k;
+        int lo = row + 1, hi = row + W + 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= me) lo = mid; else hi = mid;
+        }
+        const int col = lo - row;
+        const float4 pi = L.ent[me];
+        const float2 wi = lw[me];
+        const float xi = pi.x, yi = pi.y, xim = xi - Lx, yim = yi - Ly;
+        const int gi = __float_as_int(pi.z);
+        const int gix = gi >= 0 ? gi : -2;
+        const float si = pi.w;
+        const int a0 = L.off[col - 1], b0 = L.off[col + 2];
+        const int a1 = L.off[row + col - 1], b1 = L.off[row + col + 2];
+        const int a2 = L.off[2 * row + col - 1], b2 = L.off[2 * row + col + 2];
+        const int total = (b0 - a0) + (b1 - a1) + (b2 - a2);
+        int j = a0;
+        if (j == b0) j = a1;
+        if (j == b1) j = a2;
+        float jx = 0.f, jy = 0.f, fx = 0.f, fy = 0.f, gx = 0.f, gy = 0.f;
+        for (int t = 0; t < total; ++t) {
+            const float4 pj = L.ent[j];
+            const int gj = __float_as_int(pj.z);
+            float dx = xi - pj.x;
+            float dy = yi - pj.y;
+            int sx = 0, sy = 0;
+            if (kWrap) {
+                if (dx >= hLx) { dx -= Lx; sx = 1; } else if (dx < -hLx) { dx += Lx; sx = -1; }
+                if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
+            }
+            const float dr2 = dx * dx + dy * dy;
+            float s2ij, sij;
+            unsigned rc2m1;
+            if (kSigma) {
+                sij = (si + pj.w) * 0.5f;
+                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                s2ij = sij * sij;
+            } else {
+                sij = R.s_uniform;
+                rc2m1 = rc2m1_u;
+                s2ij = R.s2_uniform;
+            }
+            const bool hit = ((__float_as_uint(dr2) - 1u) < rc2m1) & (gj != gix);
+            if (hit) {
+                float dxa = dx, dya = dy, r2a = dr2;
+                if (kWrap) {
+                    if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
+                    if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);
+                    r2a = dxa * dxa + dya * dya;
+                }
+                const float inv = __builtin_amdgcn_rcpf(r2a);
+                const float u = s2ij * inv;
+                const float s6 = u * u * u;
+                const float s12 = s6 * s6;
+                const float g = __builtin_fmaf(-0.5f, s6, s12) * inv;
+                const float gp = __builtin_fmaf(2.0f, s6, -7.0f * s12) * inv * inv;
+                const float gs = __builtin_fmaf(12.0f, s12, -3.0f * s6) * inv / sij;
+                const float2 wj = lw[j];
+                const float dwx = wi.x - wj.x, dwy = wi.y - wj.y;
+                const float ddw = __builtin_fmaf(dxa, dwx, dya * dwy);
+                const float g2 = 2.0f * gp * ddw;
+                jx = __builtin_fmaf(g, dwx, __builtin_fmaf(g2, dxa, jx));
+                jy = __builtin_fmaf(g, dwy, __builtin_fmaf(g2, dya, jy));
+                fx = __builtin_fmaf(g, dxa, fx);
+                fy = __builtin_fmaf(g, dya, fy);
+                gx = __builtin_fmaf(gs, dxa, gx);
+                gy = __builtin_fmaf(gs, dya, gy);
+            }
+            ++j;
+            if (j == b0) j = a1;
+            if (j == b1) j = a2;
+        }
+        const int id = a.perm[sbase + k];
+        A.jx[id] = pe.e48 * jx;
+        A.jy[id] = pe.e48 * jy;
+        // w . f_lj / eps = 48 w . (f / 48 eps); w . df/dsigma
+        peps += 48.0 * ((double)wi.x * fx + (double)wi.y * fy);
+        psig += (double)pe.e48 * ((double)wi.x * gx + (double)wi.y * gy);
+    }
+    block_sum2(peps, psig, L.red);
+    if (threadIdx.x == 0) {
+        A.pacc[2 * tile] += peps;
+        A.pacc[2 * tile + 1] += psig;
+    }
+}
+
+template <bool kSigma>
+__device__ void adj_pair_tile(const AdjArgs &A, int tile, PairLds &L, float2 *lw) {
+    const ForceArgs &a = A.f;
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    const bool wrapT = (T.w_flags >> 16) & 1;
+    if (stage_tile<false, false>(a, T, R, W, L, A.w, lw) < 0) {
+        if (threadIdx.x == 0) atomicOr(a.err, 2u);
+        return;
+    }
+    if (wrapT)
+        adj_pair_body<true, kSigma>(A, R, W, tile, L, lw);
+    else
+        adj_pair_body<false, kSigma>(A, R, W, tile, L, lw);
+}
+
+// Element part of J w: dP (the directional derivative of the first
+// Piola-Kirchhoff stress along dF = dDs Bm) pushed through H = -A0 P Bm^T;
+// and dL/dE_e += w . f_e / E_e (P is linear in (lambda, mu), both
+// proportional to E at fixed nu).
+__device__ void adj_elem_block(const AdjArgs &A, int b, double *red) {
+    const ForceArgs &a = A.f;
+    const Span sp = a.espans[b];
+    const RepDesc R = a.reps[sp.rep];
+    for (int e = sp.begin + (int)threadIdx.x; e < sp.end; e += blockDim.x) {
+        const int n0 = a.t0[e], n1 = a.t1[e], n2 = a.t2[e];
+        const float x0 = a.x[n0], y0 = a.y[n0];
+        const float x1 = a.x[n1], y1 = a.y[n1];
+        const float x2 = a.x[n2], y2 = a.y[n2];
+        const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+        const float dx01 = x0 - x1, dx02 = x0 - x2, dy01 = y0 - y1, dy02 = y0 - y2;
+        const int s1x = (dx01 > hLx) - (dx01 < -hLx);
+        const int s2x = (dx02 > hLx) - (dx02 < -hLx);
+        const int s1y = (dy01 > hLy) - (dy01 < -hLy);
+        const int s2y = (dy02 > hLy) - (dy02 < -hLy);
+        const float Ds00 = shifted_diff(x0, x2, -s2x, Lx);
+        const float Ds10 = shifted_diff(y0, y2, -s2y, Ly);
+        const float Ds01 = shifted_diff(x1, x2, s1x - s2x, Lx);
+        const float Ds11 = shifted_diff(y1, y2, s1y - s2y, Ly);
+        const float4 Dm = a.dmbm[2 * e];
+        const float4 B = a.dmbm[2 * e + 1];
+        const float2 alm = a.alm[e];
+        const float al = alm.x, am = alm.y;
+        const float G00 = (Ds00 - Dm.x) * B.x + (Ds01 - Dm.y) * B.z;
+        const float G10 = (Ds10 - Dm.z) * B.x + (Ds11 - Dm.w) * B.z;
+        const float G01 = (Ds00 - Dm.x) * B.y + (Ds01 - Dm.y) * B.w;
+        const float G11 = (Ds10 - Dm.z) * B.y + (Ds11 - Dm.w) * B.w;
+        const float F00 = 1.0f + G00, F11 = 1.0f + G11, F01 = G01, F10 = G10;
+        const float2 w0 = A.w[n0], w1 = A.w[n1], w2 = A.w[n2];
+        const float d00 = w0.x - w2.x, d10 = w0.y - w2.y, d01 = w1.x - w2.x, d11 = w1.y - w2.y;
+        const float dF00 = d00 * B.x + d01 * B.z;
+        const float dF10 = d10 * B.x + d11 * B.z;
+        const float dF01 = d00 * B.y + d01 * B.w;
+        const float dF11 = d10 * B.y + d11 * B.w;
+        float P00, P01, P10, P11, Q00, Q01, Q10, Q11;
+        if (a.model == 0) {
+            const float tJ = (G00 + G11) + (G00 * G11 - G01 * G10);
+            const float lJ = log1pf(tJ);
+            const float iJ = 1.0f / (1.0f + tJ);
+            const float A00 = (G00 - G11) + tJ * F00;
+            const float A01 = G01 + tJ * G01 + G10;
+            const float A10 = G10 + tJ * G10 + G01;
+            const float A11 = (G11 - G00) + tJ * F11;
+            const float ll = al * lJ;
+            P00 = (am * A00 + ll * F11) * iJ;
+            P01 = (am * A01 - ll * G10) * iJ;
+            P10 = (am * A10 - ll * G01) * iJ;
+            P11 = (am * A11 + ll * F00) * iJ;
+            // F^-T = cof(F) / J
+            const float c00 = F11 * iJ, c01 = -F10 * iJ, c10 = -F01 * iJ, c11 = F00 * iJ;
+            const float tr = (F11 * dF00 - F01 * dF10 - F10 * dF01 + F00 * dF11) * iJ;
+            // M = F^-T dF^T F^-T
+            const float q00 = dF00 * c00 + dF10 * c10, q01 = dF00 * c01 + dF10 * c11;
+            const float q10 = dF01 * c00 + dF11 * c10, q11 = dF01 * c01 + dF11 * c11;
+            const float M00 = c00 * q00 + c01 * q10, M01 = c00 * q01 + c01 * q11;
+            const float M10 = c10 * q00 + c11 * q10, M11 = c10 * q01 + c11 * q11;
+            const float k1 = am - ll, k2 = al * tr;
+            Q00 = am * dF00 + k1 * M00 + k2 * c00;
+            Q01 = am * dF01 + k1 * M01 + k2 * c01;
+            Q10 = am * dF10 + k1 * M10 + k2 * c10;
+            Q11 = am * dF11 + k1 * M11 + k2 * c11;
+        } else {
+            const float E00 = G00 + 0.5f * (G00 * G00 + G10 * G10);
+            const float E11 = G11 + 0.5f * (G01 * G01 + G11 * G11);
+            const float E01 = 0.5f * (G01 + G10) + 0.5f * (G00 * G01 + G10 * G11);
+            const float trE = E00 + E11;
+            const float S00 = 2.0f * am * E00 + al * trE;
+            const float S11 = 2.0f * am * E11 + al * trE;
+            const float S01 = 2.0f * am * E01;
+            P00 = F00 * S00 + F01 * S01;
+            P01 = F00 * S01 + F01 * S11;
+            P10 = F10 * S00 + F11 * S01;
+            P11 = F10 * S01 + F11 * S11;
+            // dE = (dF^T F + F^T dF) / 2, dS = 2 mu dE + lambda tr(dE) I
+            const float dE00 = dF00 * F00 + dF10 * F10;
+            const float dE11 = dF01 * F01 + dF11 * F11;
+            const float dE01 = 0.5f * ((dF00 * F01 + dF10 * F11) + (F00 * dF01 + F10 * dF11));
+            const float dtr = dE00 + dE11;
+            const float dS00 = 2.0f * am * dE00 + al * dtr;
+            const float dS11 = 2.0f * am * dE11 + al * dtr;
+            const float dS01 = 2.0f * am * dE01;
+            Q00 = dF00 * S00 + dF01 * S01 + F00 * dS00 + F01 * dS01;
+            Q01 = dF00 * S01 + dF01 * S11 + F00 * dS01 + F01 * dS11;
+            Q10 = dF10 * S00 + dF11 * S01 + F10 * dS00 + F11 * dS01;
+            Q11 = dF10 * S01 + dF11 * S11 + F10 * dS01 + F11 * dS11;
+        }
+        const float f0x = -(P00 * B.x + P01 * B.y), f0y = -(P10 * B.x + P11 * B.y);
+        const float f1x = -(P00 * B.z + P01 * B.w), f1y = -(P10 * B.z + P11 * B.w);
+        const float g0x = -(Q00 * B.x + Q01 * B.y), g0y = -(Q10 * B.x + Q11 * B.y);
+        const float g1x = -(Q00 * B.z + Q01 * B.w), g1y = -(Q10 * B.z + Q11 * B.w);
+        A.cfa[3 * e] = make_float2(g0x, g0y);
+        A.cfa[3 * e + 1] = make_float2(g1x, g1y);
+        A.cfa[3 * e + 2] = make_float2(-g0x - g1x, -g0y - g1y);
+        const double wf = (double)(w0.x * f0x + w0.y * f0y) + (double)(w1.x * f1x + w1.y * f1y) -
+                          (double)(w2.x * (f0x + f1x) + w2.y * (f0y + f1y));
+        A.dE[e] += wf / (double)A.eE[e];
+    }
+    (void)red;
+}
+
+// Bond part of J w: g = -k (r - r0) / r, g' = dg/ds = -k r0 / (2 r^3).
+__device__ void adj_bond_block(const AdjArgs &A, int b) {
+    const ForceArgs &a = A.f;
+    const Span sp = a.bspans[b];
+    const RepDesc R = a.reps[sp.rep];
+    for (int l = sp.begin + (int)threadIdx.x; l < sp.end; l += blockDim.x) {
+        const int i = a.bi[l], j = a.bj[l];
+        const float xi = a.x[i], yi = a.y[i], xj = a.x[j], yj = a.y[j];
+        float dx = xi - xj, dy = yi - yj;
+        const int sx = (dx >= R.hLx) - (dx < -R.hLx);
+        const int sy = (dy >= R.hLy) - (dy < -R.hLy);
+        dx = shifted_diff(xi, xj, -sx, R.Lx);
+        dy = shifted_diff(yi, yj, -sy, R.Ly);
+        const float r2 = dx * dx + dy * dy;
+        const float r = sqrtf(r2);
+        const float k = a.bk[l], r0 = a.br0[l];
+        const float g = -k * (r - r0) / r;
+        const float gp = -k * r0 / (2.0f * r2 * r);
+        const float2 wi = A.w[i], wj = A.w[j];
+        const float dwx = wi.x - wj.x, dwy = wi.y - wj.y;
+        const float g2 = 2.0f * gp * (dx * dwx + dy * dwy);
+        const float jx = g * dwx + g2 * dx, jy = g * dwy + g2 * dy;
+        A.cfa[a.cf_bond0 + 2 * l] = make_float2(jx, jy);
+        A.cfa[a.cf_bond0 + 2 * l + 1] = make_float2(-jx, -jy);
+    }
+}
+
+template <bool kSigma>
+__global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
+    extern __shared__ float4 dyn[];
+    __shared__ int s_cstart[kMaxSlots];
+    __shared__ int s_off[kMaxSlots + 1];
+    __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
+    __shared__ double s_red[2 * 4];
+    const ForceArgs &a = A.f;
+    int b = blockIdx.x;
+    if (b < a.ntiles) {
+        PairLds L;
+        L.ent = dyn;
+        L.typ = nullptr;
+        L.sp = nullptr;
+        L.cstart = s_cstart;
+        L.off = s_off;
+        L.pt = s_pt;
+        L.red = s_red;
+        float2 *lw = reinterpret_cast<float2 *>(dyn + a.lds_cap);
+        adj_pair_tile<kSigma>(A, xcd_remap(b, a.ntiles), L, lw);
+        return;
+    }
+    b -= a.ntiles;
+    if (b < a.nespans) {
+        adj_elem_block(A, b, s_red);
+        return;
+    }
+    b -= a.nespans;
+    if (b < a.nbspans) adj_bond_block(A, b);
+}
+
+__global__ __launch_bounds__(kKickThreads) void k_adj_kick(AdjKickArgs a) {
+    const Span sp = a.spans[blockIdx.x];
+    const RepDesc R = a.reps[sp.rep];
+    const int p = sp.begin + (int)threadIdx.x;
+    if (p >= sp.end) return;
+    float xbx = a.xbx[p], xby = a.xby[p];
+    if (a.accumulate) {
+        float jx = a.jx[p], jy = a.jy[p];
+        if (a.goff) {
+            for (int e = a.goff[p]; e < a.goff[p + 1]; ++e) {
+                const float2 c = a.cfa[a.gent[e]];
+                jx += c.x;
+                jy += c.y;
+            }
+        }
+        xbx += jx;
+        xby += jy;
+        a.xbx[p] = xbx;
+        a.xby[p] = xby;
+    }
+    float vbx = a.vbx[p], vby = a.vby[p];
+    if (a.drift) {
+        vbx = __builtin_fmaf(R.dt, xbx, vbx);
+        vby = __builtin_fmaf(R.dt, xby, vby);
+        a.vbx[p] = vbx;
+        a.vby[p] = vby;
+    }
+    if (a.make_w) {
+        const float s = a.c * a.inv_m[p];
+        a.w[p] = make_float2(s * vbx, s * vby);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_loss(LossArgs a) {
+    __shared__ double red[8];
+    const int k = blockIdx.x * blockDim.x + (int)threadIdx.x;
+    double l = 0.0, z = 0.0;
+    if (k < a.n) {
+        const RepDesc R = a.reps[0];
+        const int p = a.ids[k];
+        const float2 t = a.target[k];
+        float dx = a.x[p] - t.x, dy = a.y[p] - t.y;
+        if (dx >= R.hLx) dx -= R.Lx; else if (dx < -R.hLx) dx += R.Lx;
+        if (dy >= R.hLy) dy -= R.Ly; else if (dy < -R.hLy) dy += R.Ly;
+        a.xbx[p] = a.scale * dx;
+        a.xby[p] = a.scale * dy;
+        l = (double)dx * dx + (double)dy * dy;
+    }
+    block_sum2(l, z, red);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = l;
+}
+
 // ---------------------------------------------------------------------------
 // launchers (host side)
 // ---------------------------------------------------------------------------
+hipError_t launch_adj_forces(bool sigma, int block, const AdjArgs &A, hipStream_t s) {
+    const ForceArgs &a = A.f;
+    const int nblk = a.ntiles + a.nespans + a.nbspans;
+    if (nblk <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.lds_cap * (sizeof(float4) + sizeof(float2));
+    if (sigma)
+        hipLaunchKernelGGL((k_adj_forces<true>), dim3(nblk), dim3(block), lds, s, A);
+    else
+        hipLaunchKernelGGL((k_adj_forces<false>), dim3(nblk), dim3(block), lds, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_kick, dim3(nblk), dim3(kKickThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_loss, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,

@@ -167,6 +167,24 @@ int fm_time_forces(fm_system *h, int32_t reps, double *ms);
  * scan reduce, scan apply, scatter, force pass, whole step. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
 
+/* ---- adjoint (no reference counterpart; single-type systems) ----
+ * Directional derivative of the forces at the current positions: for a
+ * per-particle vector w (caller order), j = (df/dx) w, and the parameter
+ * terms dE[e] = w . df/dE_e (caller element order), deps = w . df/deps,
+ * dsig = w . df/dsigma for the common pair eps and sigma.  Any output may
+ * be NULL.  The state is left unchanged. */
+int fm_force_jvp(fm_system *h, const float *wx, const float *wy, float *jx,
+                 float *jy, double *dE, double *deps, double *dsig);
+/* Discrete adjoint of nsteps KDK steps from the current state: runs them
+ * forward (the state advances as with fm_step), evaluates
+ * loss = (1/K) sum_k |x_{ids[k]} - (tx[k], ty[k])|^2 at the end (minimum
+ * image, K = nloss) and sweeps backward, recomputing positions from
+ * checkpoints taken every ckpt steps.  Returns loss, dL/dE per element
+ * (caller order) and dL/deps, dL/dsigma; gradient outputs may be NULL. */
+int fm_adjoint(fm_system *h, int32_t nsteps, int32_t ckpt, int32_t nloss,
+               const int32_t *ids, const float *tx, const float *ty,
+               double *loss, double *dE, double *deps, double *dsig);
+
 /* ---- batched replicas (no reference counterpart) ---- */
 int fm_batch_create(int32_t n, fm_system *const *systems, fm_batch **out);
 void fm_batch_destroy(fm_batch *b);

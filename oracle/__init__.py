@@ -146,10 +146,12 @@ def box_f32(L: float) -> float:
     return float(np.float32(L))
 
 
-def from_workload(w, x=None, y=None) -> OracleSystem:
+def from_workload(w, x=None, y=None, *, E=None, eps=None, sigma=None) -> OracleSystem:
     """Oracle system of a configs.Workload at caller-order positions (x, y),
     default the generated ones (wrapped in FP32 as the engine stores them).
-    The box is the FP32 one the engine and the reference compute with."""
+    The box is the FP32 one the engine and the reference compute with.
+    E (per element), eps and sigma (per particle) override the workload's
+    parameters."""
     ga = w.global_arrays()
     Lx, Ly = box_f32(w.Lx), box_f32(w.Ly)
     if x is None:
@@ -157,10 +159,11 @@ def from_workload(w, x=None, y=None) -> OracleSystem:
         yf = ga["y"].astype(np.float32)
         wrap_f32(xf, yf, Lx, Ly)
         x, y = xf, yf
-    return OracleSystem(Lx, Ly, x, y, sigma=ga["sigma"], excl=ga["excl"], eps=[w.eps],
+    return OracleSystem(Lx, Ly, x, y, sigma=ga["sigma"] if sigma is None else sigma,
+                        excl=ga["excl"], eps=[w.eps if eps is None else eps],
                         rc=[w.rc], ushift=[w.ushift], bonds=w.bonds, tri=ga["tri"],
                         refx=ga["refx"].astype(np.float32), refy=ga["refy"].astype(np.float32),
-                        E=ga["E"], nu=ga["nu"], model=w.model)
+                        E=ga["E"] if E is None else E, nu=ga["nu"], model=w.model)
 
 
 # ---------------------------------------------------------------------------
