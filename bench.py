@@ -92,6 +92,63 @@ def batched_leg(steps: int, loop_steps: int, tune_cache):
             "speedup": batched / looped, "device_errors": errs}
 
 
+def adjoint_leg(steps: int):
+    """C5: elastic sheet (128 x 256 nodes, E in [20, 120], bottom row pinned)
+    in 200,000 WCA solvent.  Discrete adjoint of `steps` steps (forward,
+    backward sweep) of the loss over the free sheet nodes, against the same
+    steps forward: with the whole trajectory stored (it fits in HBM), and
+    with checkpoints every sqrt(steps) steps and recomputation."""
+    import numpy as np
+    from femmd_old_amd import configs
+    w = configs.c5()
+    s = configs.build_system(w)
+    ga = w.global_arrays()
+    nodes = np.arange(w.n_solvent, w.n_total)
+    ids = nodes[np.isfinite(ga["mass"][nodes])].astype(np.int32)
+    x, y, _, _ = s.get_state()
+    tgt = np.stack([x[ids] + 0.3, y[ids]], 1)
+    ck_sqrt = max(1, int(round(steps ** 0.5)))
+    s.run(steps)  # warm-up: graph capture
+    s.adjoint(steps, ids, tgt, ckpt=ck_sqrt)  # warm-up: adjoint buffers
+    t0 = time.perf_counter()
+    s.run(steps)
+    t1 = time.perf_counter()
+    g = s.adjoint(steps, ids, tgt)  # whole trajectory stored (fits in HBM)
+    t2 = time.perf_counter()
+    s.adjoint(steps, ids, tgt, ckpt=ck_sqrt)  # sqrt(steps) checkpoints, recompute
+    t3 = time.perf_counter()
+    fwd, adj, adj_ck = t1 - t0, t2 - t1, t3 - t2
+    errs = s.device_errors()
+    print(f"[bench] c5 adjoint: N={w.n_total} tri={w.n_tri} {steps} steps: forward {fwd:.3f}s, "
+          f"adjoint {adj:.3f}s, with checkpoints every {ck_sqrt} {adj_ck:.3f}s",
+          file=sys.stderr, flush=True)
+    return {"workload": "c5: 128x256-node CST sheet (64,770 tri) in 200,000 WCA solvent",
+            "particles": w.n_total, "steps": steps,
+            "adjoint_particle_steps_per_s": w.n_total * steps / adj,
+            "adjoint_ms_per_step": 1e3 * adj / steps,
+            "forward_ms_per_step": 1e3 * fwd / steps,
+            "adjoint_over_forward": adj / fwd,
+            "checkpointed": {"checkpoint_every": ck_sqrt,
+                             "adjoint_ms_per_step": 1e3 * adj_ck / steps,
+                             "adjoint_over_forward": adj_ck / fwd},
+            "loss": g["loss"], "grad_norm_E": float(np.linalg.norm(g["dE"])),
+            "device_errors": errs}
+
+
+def copy_probe(nbytes: int):
+    """Reads nbytes and writes nbytes with one elementwise kernel (5 launches):
+    what FETCH_SIZE / WRITE_SIZE report for it scales the counters of the
+    engine's kernels to HBM bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {5} launches of {n * 4} bytes read + written", file=sys.stderr)
+
+
 def cpu_baseline(w, steps, threads):
     import numpy as np
     import oracle as O
@@ -120,6 +177,11 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=None)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (5 launches), the "
+                         "calibration of the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests)")
     args = ap.parse_args()
 
@@ -127,6 +189,10 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
                                    "note only (bench.py --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     from femmd_old_amd import configs
 
@@ -184,6 +250,8 @@ def main():
         del s
         out["batched"] = batched_leg(args.batched_steps, max(4, args.batched_steps // 10),
                                      args.tune_cache)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, 1)
     print(json.dumps(out), flush=True)

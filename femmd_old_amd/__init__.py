@@ -337,14 +337,13 @@ class System:
         """Runs nsteps forward and returns the loss
         (1/K) sum_k |x_{ids[k]}(T) - targets[k]|^2 with its gradient
         {"loss", "dE" (per element), "deps", "dsigma"} from the discrete
-        adjoint (checkpoints every ckpt steps, default ~sqrt(nsteps))."""
+        adjoint (checkpoints every ckpt steps; 0 lets the engine store as
+        much of the trajectory as a quarter of the free HBM holds)."""
         ids = _i32(ids)
         t = np.asarray(targets, np.float64).reshape(-1, 2)
         if t.shape[0] != ids.shape[0]:
             raise ValueError("one target per loss particle")
         tx, ty = _f32(t[:, 0]), _f32(t[:, 1])
-        if ckpt <= 0:
-            ckpt = max(1, int(round(np.sqrt(nsteps))))
         loss, deps, dsig = C.c_double(), C.c_double(), C.c_double()
         dE = np.empty(self._nelem, np.float64)
         self._check(self._lib.fm_adjoint(self._h, int(nsteps), int(ckpt), ids.shape[0], _ip(ids),

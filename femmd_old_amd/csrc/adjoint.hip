@@ -10,8 +10,10 @@
 //     xb += J(x_n) w,  theta_b += (df/dtheta)(x_n)^T w
 // J = df/dx is symmetric, so J^T w = J w.  The wrap is the identity
 // almost everywhere.  Positions x_n are recomputed per segment from
-// checkpoints (x, v) taken every `ckpt` steps; parameter gradients
-// accumulate in FP64 (per element for E, per force tile for eps, sigma).
+// checkpoints (x, v) taken every `ckpt` steps (the last segment is recorded
+// during the forward pass, so with ckpt >= nsteps nothing is recomputed);
+// parameter gradients accumulate in FP64 (per element for E, per force tile
+// for eps, sigma).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -170,8 +172,8 @@ int Engine::adjoint_op(const float *wx, const float *wy, float *jx, float *jy, d
 int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
                         const float *ty, double *loss, double *dE, double *deps, double *dsig,
                         std::string &err) {
-    if (nsteps < 1 || ckpt < 1 || nloss < 1 || !ids || !tx || !ty) {
-        err = "nsteps, ckpt and the loss set must be non-empty";
+    if (nsteps < 1 || nloss < 1 || !ids || !tx || !ty) {
+        err = "nsteps and the loss set must be non-empty";
         return FM_ERR_ARG;
     }
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
@@ -179,6 +181,18 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
     rc = adj_alloc(err);
     if (rc) return rc;
     const int N = N_;
+    if (ckpt <= 0) {
+        // keep as many stored configurations as a quarter of the free HBM
+        // (at most 64 GiB) holds: the whole trajectory when it fits, so the
+        // backward sweep recomputes nothing
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+        const size_t budget = std::min(fr / 4, (size_t)64 << 30);
+        const size_t per = (size_t)2 * N * sizeof(float);
+        const size_t fit = budget / per;
+        ckpt = (int)std::max<size_t>(1, std::min<size_t>(fit, (size_t)nsteps));
+    }
+    ckpt = std::min(ckpt, nsteps);
     const std::vector<int32_t> &iid = iid_h_[0];
     std::vector<int32_t> lid(nloss);
     std::vector<float2> tgt(nloss);
@@ -203,20 +217,33 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
                          stream_);
     rc = ensure_primed(err);
     if (rc) return rc;
-    // ---- forward, checkpoint at every segment start and at the end ----
+    const size_t fb = (size_t)N * sizeof(float);
+    // positions x_n of step k of a segment -> hist slot k
+    auto record = [&](int k) -> int {
+        float *hx = hist_.p + (size_t)k * 2 * N;
+        if (hipMemcpyAsync(hx, x_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess ||
+            hipMemcpyAsync(hx + N, y_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess)
+            return -1;
+        return 0;
+    };
+    // ---- forward, checkpoint at every segment start and at the end; the
+    // last segment records its positions as it goes ----
     for (int s = 0; s < S; ++s) {
         if (save_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "ckpt", err);
         const int len = std::min(ckpt, nsteps - s * ckpt);
+        const bool last = s == S - 1;
+        if (last && record(0)) return check(hipGetLastError(), "history", err);
         if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
-        for (int k = 1; k < len; ++k)
+        for (int k = 1; k < len; ++k) {
+            if (last && record(k)) return check(hipGetLastError(), "history", err);
             if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+        }
         if (launch_step_body(true, false)) return check(hipGetLastError(), "kick", err);
     }
     steps_ += nsteps;
     float *final_state = ckpt_.p + (size_t)S * 4 * N;
     if (save_state(final_state)) return check(hipGetLastError(), "ckpt", err);
     // ---- loss and its gradient ----
-    const size_t fb = (size_t)N * sizeof(float);
     (void)hipMemsetAsync(xbx_.p, 0, fb, stream_);
     (void)hipMemsetAsync(xby_.p, 0, fb, stream_);
     (void)hipMemsetAsync(vbx_.p, 0, fb, stream_);
@@ -243,16 +270,15 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
     bool first = true;
     for (int s = S - 1; s >= 0; --s) {
         const int len = std::min(ckpt, nsteps - s * ckpt);
-        if (load_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "restore", err);
-        rc = ensure_primed(err);
-        if (rc) return rc;
-        // recompute x_{sC} .. x_{sC+len-1} of this segment
-        for (int k = 0; k < len; ++k) {
-            if (k > 0 && launch_step_body(k > 1, true)) return check(hipGetLastError(), "step", err);
-            float *hx = hist_.p + (size_t)k * 2 * N;
-            if (hipMemcpyAsync(hx, x_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess ||
-                hipMemcpyAsync(hx + N, y_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess)
-                return check(hipGetLastError(), "history", err);
+        if (s < S - 1) {
+            // recompute x_{sC} .. x_{sC+len-1} of this segment
+            if (load_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "restore", err);
+            rc = ensure_primed(err);
+            if (rc) return rc;
+            for (int k = 0; k < len; ++k) {
+                if (k > 0 && launch_step_body(k > 1, true)) return check(hipGetLastError(), "step", err);
+                if (record(k)) return check(hipGetLastError(), "history", err);
+            }
         }
         for (int k = len - 1; k >= 0; --k) {
             const int n = s * ckpt + k;

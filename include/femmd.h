@@ -179,8 +179,10 @@ int fm_force_jvp(fm_system *h, const float *wx, const float *wy, float *jx,
  * forward (the state advances as with fm_step), evaluates
  * loss = (1/K) sum_k |x_{ids[k]} - (tx[k], ty[k])|^2 at the end (minimum
  * image, K = nloss) and sweeps backward, recomputing positions from
- * checkpoints taken every ckpt steps.  Returns loss, dL/dE per element
- * (caller order) and dL/deps, dL/dsigma; gradient outputs may be NULL. */
+ * checkpoints taken every ckpt steps (ckpt <= 0: as many stored steps as a
+ * quarter of the free device memory holds, the whole trajectory when it
+ * fits).  Returns loss, dL/dE per element (caller order) and dL/deps,
+ * dL/dsigma; gradient outputs may be NULL. */
 int fm_adjoint(fm_system *h, int32_t nsteps, int32_t ckpt, int32_t nloss,
                const int32_t *ids, const float *tx, const float *ty,
                double *loss, double *dE, double *deps, double *dsig);
