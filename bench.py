@@ -55,6 +55,20 @@ def algorithmic_bytes(n, ncells, ntri, nbond):
     return b
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the full c2 workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by
+    a plain copy of known size); None when absent."""
+    p = ROOT / "profiles" / "k_forces_c2_hbm_traffic.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def batched_leg(steps: int, loop_steps: int, tune_cache):
     """C4: 512 replicas, N ~ logU[256, 32768]; one batched, graph-captured
     step for all of them against looping the plain step over the replicas
@@ -219,7 +233,9 @@ def main():
     achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": f"k_{kern}", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": ab[kern],
+            "traffic": measured_traffic(f"k_{kern}") if args.scale == 1.0 else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)",
+            "bytes_per_launch": ab[kern],
             "launch_ms": round(prof[kern], 5),
             "step_kernels_ms": {k: round(v, 5) for k, v in prof.items()},
             "step_frac": round(ab["step"] / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
