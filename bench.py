@@ -6,9 +6,12 @@ Workload (the headline configuration): 1,048,576 LJ particles at density
 energy sums, dt = 0.005, synthetic seeded inputs (femmd_old_amd.configs.c2).
 
 A step is one pass of the hot path over the whole system: half-kick + drift
-+ wrap + cell binning, counting sort, LJ / CST forces, and the second
++ wrap, the neighbour-list update (binning, counting sort and list build
+on the steps where the displacement bound crosses skin / 2, the
+reference's rule with its skin 0.4), LJ / CST forces, and the second
 half-kick with its kinetic-energy reduction (fused into the next step's
-first kernel).  Inputs are resident in HBM when the timed region starts.
+first kernel).  --skin 0 walks the cells every step instead.  Inputs are
+resident in HBM when the timed region starts.
 
 Prints one JSON line: value = particle-steps/s; roofline for the dominant
 kernel (live HIP-event timing on the engine's stream, algorithmic bytes);
@@ -30,39 +33,66 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 
-def algorithmic_bytes(n, ncells, ntri, nbond):
-    """Bytes each kernel must move per launch (each input array read once,
-    each output written once; FP32 values, int32 indices).
+def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
+    """Bytes each kernel must move per executed launch (each input array
+    read once, each output written once; FP32 values, int32 indices).
 
+    Per-step cell walk (list_len None):
     kick     r x,y,vx,vy,fx,fy,inv_m,m,goff; w x,y,vx,vy,slot,key: 60/particle;
              counter read-modify-write 8/cell; gather 12/corner
     reduce   r count: 4/cell
-    apply    r count, w start, zero the other counters: 12/cell
+    apply    r count, w start, w count (zeroed): 12/cell
     scatter  r key, own slot entry, x,y,group; w perm, sorted (16): 40/particle;
-             r count, start: 8/cell
+             r start (2 entries): 8/cell
     forces   r sorted (16), perm; w fx,fy: 28/particle; r start: 4/cell;
              CST: r 3 ids, Dm, Bm, A0*lambda, A0*mu, 3 node x,y; w 3 corners:
              100/element; bond: r ids, k, r0, 2 node x,y; w 2 ends: 48/bond
+    Neighbour lists (list_len = mean list length L):
+    kick     as above without slot and key: 52/particle
+    bin      r x,y; w slot, key: 16/particle; counters 8/cell  (rebuild steps)
+    list     r sorted (16), perm; w L entries, length: 24 + 4L/particle;
+             r start: 4/cell                                   (rebuild steps)
+    forces   r x,y, length, L entries; w fx,fy: 20 + 4L/particle; elements
+             and bonds as above
     """
-    b = {
-        "kick": 60 * n + 8 * ncells + 36 * ntri + 24 * nbond,
-        "scan_reduce": 4 * ncells,
-        "scan_apply": 12 * ncells,
-        "scatter": 40 * n + 8 * ncells,
-        "forces": 28 * n + 4 * ncells + 100 * ntri + 48 * nbond,
-    }
-    b["step"] = sum(b.values())
+    corners = 36 * ntri + 24 * nbond
+    elems = 100 * ntri + 48 * nbond
+    if list_len is None:
+        b = {
+            "kick": 60 * n + 8 * ncells + corners,
+            "bin": 0,
+            "scan_reduce": 4 * ncells,
+            "scan_apply": 12 * ncells,
+            "scatter": 40 * n + 8 * ncells,
+            "list_build": 0,
+            "forces": 28 * n + 4 * ncells + elems,
+        }
+    else:
+        L = float(list_len)
+        b = {
+            "kick": 52 * n + corners,
+            "bin": 16 * n + 8 * ncells,
+            "scan_reduce": 4 * ncells,
+            "scan_apply": 12 * ncells,
+            "scatter": 40 * n + 8 * ncells,
+            "list_build": int((24 + 4 * L) * n) + 4 * ncells,
+            "forces": int((20 + 4 * L) * n) + elems,
+        }
     return b
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on the full c2 workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by
-    a plain copy of known size); None when absent."""
+REBUILD_KERNELS = ("bin", "scan_reduce", "scan_apply", "scatter", "list_build")
+
+
+def measured_traffic(kernel: str, mode: str):
+    """HBM bytes per launch of `kernel` on the full c2 workload in `mode`
+    ("list" or "cells"), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counters scaled by a plain copy of known size);
+    None when absent."""
     p = ROOT / "profiles" / "k_forces_c2_hbm_traffic.md"
     try:
         for line in p.read_text().splitlines():
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}[{mode}]:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -191,6 +221,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=None)
+    ap.add_argument("--skin", type=float, default=0.4,
+                    help="neighbour-list skin (0: walk the cells every step)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (5 launches), the "
@@ -212,33 +244,43 @@ def main():
 
     t_gen = time.time()
     w = configs.c2(scale=args.scale)
-    s = configs.build_system(w)
+    s = configs.build_system(w, skin=args.skin)
     print(f"[bench] workload c2: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} L={w.Lx:.2f} "
           f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
     ncx, ncy, _, _, _ = s.cells()
 
     s.run(args.warmup)
     e0 = s.energies()
+    ls0 = s.list_stats()
     t0 = time.perf_counter()
     s.run(args.steps)  # returns after a stream synchronise
     t1 = time.perf_counter()
     e1 = s.energies()
+    ls1 = s.list_stats()
     secs = t1 - t0
     value = w.n_total * args.steps / secs
-    print(f"[bench] timed {args.steps} steps in {secs:.3f}s", file=sys.stderr, flush=True)
+    list_mode = ls1["rebuilds"] > 0
+    rebuild_frac = (ls1["rebuilds"] - ls0["rebuilds"]) / args.steps if list_mode else 1.0
+    print(f"[bench] timed {args.steps} steps in {secs:.3f}s; lists {ls1}", file=sys.stderr,
+          flush=True)
 
     prof = s.profile(args.profile_steps)
-    ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds)
-    kern = max(("kick", "scan_reduce", "scan_apply", "scatter", "forces"), key=lambda k: prof[k])
+    ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds,
+                           ls1["mean"] if list_mode else None)
+    kern = "forces"
+    assert prof["forces"] >= max(prof[k] for k in ("kick",) + REBUILD_KERNELS)
     achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
+    step_bytes = ab["kick"] + ab["forces"] + rebuild_frac * sum(ab[k] for k in REBUILD_KERNELS)
+    mode = "list" if list_mode else "cells"
     roof = {"bound": "hbm", "kernel": f"k_{kern}", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(f"k_{kern}") if args.scale == 1.0 else None,
+            "traffic": measured_traffic(f"k_{kern}", mode) if args.scale == 1.0 else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)",
             "bytes_per_launch": ab[kern],
             "launch_ms": round(prof[kern], 5),
             "step_kernels_ms": {k: round(v, 5) for k, v in prof.items()},
-            "step_frac": round(ab["step"] / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+            "step_bytes": int(step_bytes),
+            "step_frac": round(step_bytes / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
 
     E0 = e0["ke"] + e0["u_lj"] + e0["u_fea"] + e0["u_bond"]
     E1 = e1["ke"] + e1["u_lj"] + e1["u_fea"] + e1["u_bond"]
@@ -258,7 +300,10 @@ def main():
         "config": {"workload": "c2: 1,048,576 LJ + 1,024 CST disks r=5, rho 0.8, rc 2.5, "
                                "dt 0.005, periodic", "n_particles": w.n_total,
                    "n_triangles": w.n_tri, "cells": [ncx, ncy], "box": w.Lx,
-                   "reductions": "f64"},
+                   "reductions": "f64", "pair_path": mode,
+                   "skin": args.skin if list_mode else 0.0,
+                   "list_mean_len": round(ls1["mean"], 2),
+                   "steps_per_rebuild": round(1.0 / rebuild_frac, 2) if rebuild_frac else None},
         "energy_drift": (E1 - E0) / abs(E0),
         "roofline": roof,
     }

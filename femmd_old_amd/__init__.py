@@ -314,10 +314,22 @@ class System:
 
     def profile(self, nsteps: int = 10) -> dict:
         """Per-step device ms of each kernel (HIP events, engine stream)."""
-        ms = (C.c_double * 6)()
+        ms = (C.c_double * 8)()
         self._check(self._lib.fm_profile_step(self._h, int(nsteps), ms))
-        return dict(zip(("kick", "scan_reduce", "scan_apply", "scatter", "forces", "step"),
-                        list(ms)))
+        return dict(zip(("kick", "bin", "scan_reduce", "scan_apply", "scatter", "list_build",
+                         "forces", "step"), list(ms)))
+
+    def set_skin(self, skin: float):
+        """Neighbour-list skin (reference default 0.4, femmd2D/src/nn.h:63);
+        0 walks the cells every step.  Before allocate_cuda."""
+        self._check(self._lib.fm_set_skin(self._h, float(skin)))
+
+    def list_stats(self) -> dict:
+        """{"mean", "max", "rebuilds"} of the neighbour lists (zeros when
+        the cells are walked every step)."""
+        m, mx, nb = C.c_double(), C.c_int32(), C.c_int64()
+        self._check(self._lib.fm_list_stats(self._h, C.byref(m), C.byref(mx), C.byref(nb)))
+        return {"mean": m.value, "max": mx.value, "rebuilds": nb.value}
 
     # -- adjoint (no reference counterpart) -----------------------------------
     def force_jvp(self, wx, wy):

@@ -23,6 +23,7 @@ FM_DEV_CELL_OVERFLOW = 1
 FM_DEV_TILE_OVERFLOW = 2
 FM_DEV_INVERTED = 4
 FM_DEV_NONFINITE = 8
+FM_DEV_LIST_OVERFLOW = 16
 
 FM_NEO_HOOKEAN = 0
 FM_SVK = 1
@@ -81,6 +82,8 @@ PROTOTYPES = {
     "fm_set_dt": (C.c_int, [_h, C.c_double]),
     "fm_set_lj": (C.c_int, [_h, C.c_int32]),
     "fm_set_exclusion": (C.c_int, [_h, C.c_int32, C.c_int32]),
+    "fm_set_skin": (C.c_int, [_h, C.c_float]),
+    "fm_list_stats": (C.c_int, [_h, _d, _i, _l]),
     "fm_set_mass": (C.c_int, [_h, C.c_int32, _i, _f]),
     "fm_commit": (C.c_int, [_h]),
     "fm_step": (C.c_int, [_h, C.c_int32]),

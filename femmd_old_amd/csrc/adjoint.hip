@@ -229,6 +229,13 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
     // ---- forward, checkpoint at every segment start and at the end; the
     // last segment records its positions as it goes ----
     for (int s = 0; s < S; ++s) {
+        if (list_ && S > 1) {
+            // a fresh neighbour list at every checkpoint, as the recomputation
+            // from that checkpoint will have (same summation order)
+            primed_ = false;
+            rc = ensure_primed(err);
+            if (rc) return rc;
+        }
         if (save_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "ckpt", err);
         const int len = std::min(ckpt, nsteps - s * ckpt);
         const bool last = s == S - 1;

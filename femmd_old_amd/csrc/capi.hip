@@ -234,6 +234,15 @@ int fm_set_lj(fm_system *h, int32_t on) {
     FM_END(h)
 }
 
+int fm_set_skin(fm_system *h, float skin) {
+    FM_GUARD(h)
+    if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
+    if (!(skin >= 0.0f) || !std::isfinite(skin)) return fail(h, FM_ERR_ARG, "skin must be >= 0");
+    h->hs.skin = skin;
+    return FM_OK;
+    FM_END(h)
+}
+
 int fm_set_exclusion(fm_system *h, int32_t i, int32_t group) {
     FM_GUARD(h)
     if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
@@ -401,8 +410,15 @@ int fm_time_forces(fm_system *h, int32_t reps, double *ms) {
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms) {
     FM_GUARD(h)
     FM_NEED_ENGINE(h);
-    if (nsteps <= 0 || !ms) return fail(h, FM_ERR_ARG, "nsteps > 0 and ms[5] required");
+    if (nsteps <= 0 || !ms) return fail(h, FM_ERR_ARG, "nsteps > 0 and ms[8] required");
     return h->eng->profile(nsteps, ms, h->err);
+    FM_END(h)
+}
+
+int fm_list_stats(fm_system *h, double *mean_len, int32_t *max_len, int64_t *rebuilds) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->list_stats(0, mean_len, max_len, rebuilds, h->err);
     FM_END(h)
 }
 

@@ -36,7 +36,22 @@ struct alignas(16) RepDesc {
     float rc2_uniform; // ((rc*rc)*s)*s for single-type uniform-sigma systems
     float s2_uniform;  // sigma^2 for uniform sigma
     float s_uniform;   // sigma for uniform sigma
-    int32_t pad[2];
+    float skin;        // neighbour-list skin (0: no list, cells every step)
+    float rl2_uniform; // list radius^2, (rc*s + skin)^2 with a rounding margin
+    int32_t ks0, nks;  // this replica's kick spans [ks0, ks0 + nks)
+    float dslack;      // per-step rounding allowance of the displacement bound
+};
+
+// Per-replica control words of the neighbour-list mode.  The kick kernel's
+// last workgroup of a replica turns the step's largest displacement into
+// the rebuild decision (femmd2D/src/nn.cpp:114-134: dispHi > skin / 2).
+struct alignas(16) Ctl {
+    int32_t flag;    // rebuild the list in this step
+    float disp;      // sum of per-step maximum displacements since the build
+    uint32_t dmax;   // bits of this step's max squared displacement
+    uint32_t done;   // kick workgroups of this replica finished this step
+    uint32_t nbuild; // rebuilds so far
+    int32_t pad[3];
 };
 
 // One pair-table entry: 48 eps, 4 eps, additive shift, cutoff in sigma units.
@@ -69,14 +84,19 @@ struct KickArgs {
     int32_t cap;
     double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
     uint32_t *err;
+    Ctl *ctl;                  // list mode: per-replica control (else null)
+    int32_t *any;              // list mode: some replica rebuilds this step
 };
 
 struct ScanArgs {
     int32_t ncells;
     int32_t cap;
     int32_t n;                 // particles
-    const int32_t *count;      // counters of this binning
-    int32_t *count_zero;       // counters of the previous binning, zeroed
+    int32_t *count;            // per-cell counters, zeroed once scanned
+    const Span *spans;         // kick spans (one scatter workgroup each)
+    int32_t nspans;
+    const Ctl *ctl;            // list mode gating (null: always run)
+    const int32_t *any;
     const int32_t *key;
     const int32_t *tmp;
     int32_t *bsum;             // per scan workgroup totals
@@ -120,6 +140,16 @@ struct ForceArgs {
     const float *bk, *br0;
     int32_t cf_bond0;          // first corner slot of the bonds
     double *bond_part;         // 2 per bond workgroup: U, W
+    // neighbour-list mode: pair role over kick spans, one lane per particle
+    const Span *lspans;
+    int32_t nlspans;
+    int32_t nbr_cap;           // list capacity per particle
+    int32_t *nbr;              // nbr[k * n + i]: k-th neighbour of particle i
+    int32_t *nnum;             // list length per particle
+    const float *sig;          // per-particle sigma (list mode, sigma path)
+    const Ctl *ctl;            // list build gating (null: always build)
+    int32_t *any;              // reset by the force pass after the rebuild
+    int32_t n;                 // particles (stride of nbr)
     // diagnostics (pair-set dump); null in production
     const int32_t *uid;        // internal -> caller id
     int32_t *dbg_cnt;

@@ -18,7 +18,6 @@ namespace fm {
 namespace {
 
 constexpr int kGraphSteps = 16;  // step bodies per captured graph
-static_assert(kGraphSteps % 2 == 0, "graph bodies must return to their counter parity");
 
 float wrap_host(float r, float L) {  // femmd2D/src/def.h:112, FP32
     float q = r / L;
@@ -60,6 +59,7 @@ std::string dev_error_text(uint32_t e) {
     if (e & FM_DEV_TILE_OVERFLOW) s += " cell tile exceeded LDS capacity;";
     if (e & FM_DEV_INVERTED) s += " inverted element (det F <= 0);";
     if (e & FM_DEV_NONFINITE) s += " non-finite coordinate;";
+    if (e & FM_DEV_LIST_OVERFLOW) s += " neighbour list capacity exceeded;";
     return s;
 }
 
@@ -108,8 +108,21 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->block_ = reps[0]->block;
     E->use_graph_ = reps[0]->use_graph;
 
+    // neighbour-list mode: one skin for the batch, single-type pair tables
+    const float skin = reps[0]->skin;
+    bool any_types = false, any_lj = false;
+    for (int r = 0; r < R; ++r) {
+        if (reps[r]->skin != skin) { err = "all replicas of a batch must use one skin"; return FM_ERR_ARG; }
+        any_types |= reps[r]->ntypes > 1;
+        any_lj |= reps[r]->lj_on;
+    }
+    if (!(skin >= 0.0f) || !std::isfinite(skin)) { err = "skin must be >= 0"; return FM_ERR_ARG; }
+    E->list_ = skin > 0.0f && !any_types && any_lj;
+    E->skin_ = E->list_ ? skin : 0.0f;
+
     // ---- pass 1: validation and cell geometry per replica ----
     std::vector<RepDesc> rd(R);
+    double rlmax = 0.0;
     std::vector<float> smin(R), smax(R);
     double amax = 0.0, rho_cp = 0.0;
     for (int r = 0; r < R; ++r) {
@@ -133,8 +146,10 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         if (S.ntypes > 1) E->types_ = true;
         double rcmax = 0;
         for (float v : S.rc) rcmax = std::max(rcmax, (double)v);
-        const double rc_eff = rcmax * hi;
-        // cell side >= the largest cutoff plus the FP32 slop of the key
+        const double rc_eff = rcmax * hi + E->skin_;
+        rlmax = std::max(rlmax, rc_eff);
+        // cell side >= the largest cutoff (plus skin) plus the FP32 slop of
+        // the key
         const double cmin = rc_eff * (1.0 + 1e-5) + 1e-6 * std::max(S.Lx, S.Ly);
         RepDesc &D = rd[r];
         std::memset(&D, 0, sizeof(D));
@@ -145,7 +160,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         D.ncx = (int32_t)std::floor(S.Lx / cmin);
         D.ncy = (int32_t)std::floor(S.Ly / cmin);
         if (D.ncx < 3 || D.ncy < 3) {
-            err = "box too small: fewer than 3 cells of side >= rc*sigma per axis";
+            err = "box too small: fewer than 3 cells of side >= rc*sigma + skin per axis";
             return FM_ERR_GEOMETRY;
         }
         D.fncx = (float)D.ncx;
@@ -158,12 +173,26 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         D.rc2_uniform = rcf * rcf * s0 * s0;  // reference order ((rc*rc)*s)*s
         D.s2_uniform = s0 * s0;
         D.s_uniform = s0;
+        D.skin = E->skin_;
+        {
+            // 4 ulp of the box: one rounding of x + dt v and one of the wrap
+            // per coordinate, both coordinates
+            const float Lm = std::max(D.Lx, D.Ly);
+            D.dslack = 4.0f * (std::nextafter(Lm, INFINITY) - Lm);
+        }
+        const double rl = (double)rcf * s0 + E->skin_;
+        D.rl2_uniform = (float)(rl * rl * 1.00002);
         amax = std::max(amax, (double)(D.Lx / D.ncx) * (D.Ly / D.ncy));
         rho_cp = std::max(rho_cp, 1.16 / ((double)lo * lo));
     }
     // per-cell list capacity: twice the close-packed count of a cell
     E->rho_cp_ = rho_cp;
     E->cap_ = std::min(60, std::max(32, (int)std::ceil((2.0 * rho_cp * amax + 4.0) / 8.0) * 8));
+    // neighbour-list capacity: 1.3 x the close-packed count of the list disc
+    if (E->list_) {
+        const double m = 1.3 * M_PI * rlmax * rlmax * rho_cp;
+        E->nbr_cap_ = std::min(512, std::max(32, (int)std::ceil(m / 8.0) * 8));
+    }
 
     // ---- pass 2: layout ----
     std::vector<float> hx, hy, hvx, hvy, hm, him, hs;
@@ -174,7 +203,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     std::vector<float2> halm;
     std::vector<float> heE;
     std::vector<float> hbk, hbr0;
-    std::vector<Span> ks, es, bs;
+    std::vector<Span> ks, es, bs, ls;
     std::vector<int32_t> erange(2 * R), brange(2 * R);
     E->iid_h_.resize(R);
     E->lj_on_.resize(R);
@@ -224,8 +253,12 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             ht.push_back(S.type[u]);
             huid.push_back(u);
         }
-        for (int b = p0; b < p0 + n; b += kKickThreads)
+        D.ks0 = (int32_t)ks.size();
+        for (int b = p0; b < p0 + n; b += kKickThreads) {
             ks.push_back(Span{r, b, std::min(b + kKickThreads, p0 + n), 0});
+            if (S.lj_on) ls.push_back(ks.back());
+        }
+        D.nks = (int32_t)ks.size() - D.ks0;
         // elements, ordered by their lowest internal node for locality
         const int nt = (int)S.tri.size() / 3;
         std::vector<int> eorder(nt);
@@ -289,6 +322,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->kspans_h_ = ks;
     E->espans_h_ = es;
     E->bspans_h_ = bs;
+    E->lspans_h_ = ls;
 
     // gather CSR: per particle, its corner slots in ascending slot order
     std::vector<int32_t> goff, gent;
@@ -333,6 +367,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_UP(kspans_, ks);
     FM_UP(espans_, es);
     FM_UP(bspans_, bs);
+    FM_UP(lspans_, ls);
     FM_UP(t0_, ht0);
     FM_UP(t1_, ht1);
     FM_UP(t2_, ht2);
@@ -352,7 +387,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(fy_, N);
     FM_AL(outx_, N);
     FM_AL(outy_, N);
-    FM_AL(count_, 2 * (size_t)E->ncells_);
+    FM_AL(count_, (size_t)E->ncells_);
     FM_AL(key_, N);
     FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
     const int nscan = (E->ncells_ + kScanCells - 1) / kScanCells;
@@ -367,9 +402,21 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(bond_part_, 2 * std::max<size_t>(bs.size(), 1));
     FM_AL(ene_, 8 * R);
     FM_AL(err_, 1);
+    if (E->list_) {
+        FM_AL(nbr_, (size_t)E->nbr_cap_ * N);
+        FM_AL(nnum_, N);
+        FM_AL(ctl_, R);
+        FM_AL(any_, 1);
+    }
 #undef FM_AL
-    if ((he = hipMemsetAsync(E->count_.p, 0, 2 * sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
+    if ((he = hipMemsetAsync(E->count_.p, 0, sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
         return E->check(he, "memset", err);
+    if (E->list_) {
+        if ((he = hipMemsetAsync(E->nnum_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess ||
+            (he = hipMemsetAsync(E->ctl_.p, 0, sizeof(Ctl) * R, st)) != hipSuccess ||
+            (he = hipMemsetAsync(E->any_.p, 0, sizeof(int32_t), st)) != hipSuccess)
+            return E->check(he, "memset", err);
+    }
     if ((he = hipMemsetAsync(E->err_.p, 0, sizeof(uint32_t), st)) != hipSuccess)
         return E->check(he, "memset", err);
     if ((he = hipMemsetAsync(E->fx_.p, 0, sizeof(float) * N, st)) != hipSuccess)
@@ -429,8 +476,18 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
         g[0] = kb;
         while (kb < (int)kspans_h_.size() && kspans_h_[kb].rep == r) ++kb;
         g[1] = kb;
-        g[2] = t_begin;
-        g[3] = (int)tiles_h_.size();
+        if (list_) {
+            // list mode: the pair partials are per list span of the replica
+            int lb = 0;
+            while (lb < (int)lspans_h_.size() && lspans_h_[lb].rep < r) ++lb;
+            int le = lb;
+            while (le < (int)lspans_h_.size() && lspans_h_[le].rep == r) ++le;
+            g[2] = lb;
+            g[3] = le;
+        } else {
+            g[2] = t_begin;
+            g[3] = (int)tiles_h_.size();
+        }
         g[4] = eb;
         while (eb < (int)espans_h_.size() && espans_h_[eb].rep == r) ++eb;
         g[5] = eb;
@@ -445,7 +502,8 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
     hipError_t he;
     if ((he = upload(tiles_, tiles_h_, stream_)) != hipSuccess) return check(he, "tiles", err);
     if ((he = upload(rng_, rng_h_, stream_)) != hipSuccess) return check(he, "ranges", err);
-    if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
+    if ((he = lj_part_.alloc(2 * std::max<size_t>(std::max(tiles_h_.size(), lspans_h_.size()), 1))) !=
+        hipSuccess)
         return check(he, "lj_part", err);
     if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess)
         return check(he, "memset", err);
@@ -493,22 +551,25 @@ int Engine::set_tuning(int tile_w, int block, int use_graph, std::string &err) {
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
-int Engine::launch_bin() {
+// Binning of the positions the force pass reads (xcur); gated = list mode,
+// only the replicas whose rebuild flag is set.
+int Engine::launch_bin(bool gated) {
     KickArgs a{};
     a.reps = reps_.p;
     a.spans = kspans_.p;
     a.x = xcur();
     a.y = ycur();
-    a.count = count_.p + (size_t)parity_ * ncells_;
+    a.count = count_.p;
     a.key = key_.p;
     a.tmp = tmp_.p;
     a.cap = cap_;
     a.err = err_.p;
+    a.ctl = gated ? ctl_.p : nullptr;
     return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
-static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *count_zero,
-                          const int32_t *key, int32_t *tmp, int32_t *bsum,
+static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, const int32_t *key,
+                          int32_t *tmp, int32_t *bsum,
                           int32_t *start, int32_t *perm, float *x, float *y, int32_t *grp,
                           float *sig, int32_t *typ, float4 *sorted, int32_t *sorted_typ) {
     ScanArgs s{};
@@ -516,7 +577,6 @@ static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, int32_t *c
     s.n = n;
     s.cap = cap;
     s.count = count;
-    s.count_zero = count_zero;
     s.key = key;
     s.tmp = tmp;
     s.bsum = bsum;
@@ -549,41 +609,75 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.gent = gather_ ? gent_.p : nullptr;
     k.mass = mass_.p;
     k.inv_m = inv_m_.p;
-    k.count = count_.p + (size_t)parity_ * ncells_;
+    k.count = count_.p;
     k.key = key_.p;
     k.tmp = tmp_.p;
     k.cap = cap_;
     k.ke_part = ke_part_.p;
     k.err = err_.p;
+    if (list_) {
+        k.ctl = ctl_.p;
+        k.any = any_.p;
+    }
     const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
     if (prof_ev_) (void)hipEventRecord(prof_ev_[0], stream_);
     e = launch_kick(mode, gather_, (int)kspans_h_.size(), k, stream_);
     if (e != hipSuccess) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[1], stream_);
     if (!drift) return 0;
-    return launch_forces_only(false);
+    if (!list_) return launch_forces_only(false);
+    // list mode: binning, sort and list build run only for the replicas
+    // whose displacement bound crossed skin / 2 this step
+    if (launch_bin(true)) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
+    if (launch_sort(true)) return -1;
+    if (launch_list_build(true)) return -1;
+    return launch_pair_pass();
 }
 
 // Counting sort of the last binning (scan reduce, scan apply, scatter).
-int Engine::launch_sort() {
+int Engine::launch_sort(bool gated) {
     const int nscan = (ncells_ + kScanCells - 1) / kScanCells;
-    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p + (size_t)parity_ * ncells_,
-                           count_.p + (size_t)(1 - parity_) * ncells_, key_.p, tmp_.p, bsum_.p,
-                           start_.p, perm_.p, xcur(), ycur(), grp_.p, sigma_ ? sig_.p : nullptr,
+    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p, key_.p, tmp_.p, bsum_.p, start_.p,
+                           perm_.p, xcur(), ycur(), grp_.p, sigma_ ? sig_.p : nullptr,
                            types_ ? typ_.p : nullptr, sorted_.p,
                            types_ ? sorted_typ_.p : nullptr);
+    s.spans = kspans_.p;
+    s.nspans = (int)kspans_h_.size();
+    if (gated) {
+        s.ctl = ctl_.p;
+        s.any = any_.p;
+    }
     for (int part = 0; part < 3; ++part) {
         if (launch_scan_part(part, nscan, s, stream_) != hipSuccess) return -1;
-        if (prof_ev_) (void)hipEventRecord(prof_ev_[2 + part], stream_);
+        if (prof_ev_) (void)hipEventRecord(prof_ev_[3 + part], stream_);
     }
-    parity_ ^= 1;  // the next binning uses the other (now zeroed) counters
     return 0;
 }
 
-int Engine::launch_forces_only(bool debug) {
-    if (launch_sort()) return -1;
+// Neighbour lists of the replicas flagged for a rebuild (gated), or of all.
+int Engine::launch_list_build(bool gated) {
+    ForceArgs f = force_args();
+    if (gated) f.ctl = ctl_.p;
+    const hipError_t e = ::fm::launch_list_build(sigma_, block_, f, stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[6], stream_);
+    return e == hipSuccess ? 0 : -1;
+}
+
+int Engine::launch_pair_pass() {
     const ForceArgs f = force_args();
-    const hipError_t e = launch_forces(sigma_, types_, debug, block_, f, stream_);
+    const hipError_t e = launch_forces(sigma_, types_, false, true, block_, f, stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
+    return e == hipSuccess ? 0 : -1;
+}
+
+// Cell-walk force pass over the current binning (sort first).
+int Engine::launch_forces_only(bool debug) {
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);  // binning is in the kick
+    if (launch_sort()) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[6], stream_);  // no list build
+    const ForceArgs f = force_args();
+    const hipError_t e = launch_forces(sigma_, types_, debug, false, block_, f, stream_);
     if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
@@ -622,6 +716,14 @@ ForceArgs Engine::force_args() {
     f.br0 = br0_.p;
     f.cf_bond0 = 3 * ntri_;
     f.bond_part = bond_part_.p;
+    f.lspans = lspans_.p;
+    f.nlspans = (int)lspans_h_.size();
+    f.nbr_cap = nbr_cap_;
+    f.nbr = nbr_.p;
+    f.nnum = nnum_.p;
+    f.sig = sig_.p;
+    f.any = list_ ? any_.p : nullptr;
+    f.n = N_;
     f.uid = uid_.p;
     f.dbg_cnt = dbg_cnt_.p;
     f.dbg_s1 = dbg_s1_.p;
@@ -631,8 +733,10 @@ ForceArgs Engine::force_args() {
 }
 
 // Per-kernel device time of nsteps step bodies (no graph): HIP events on
-// the engine's stream around each launch.  out[0..5] = kick+bin,
-// scan_reduce, scan_apply, scatter, forces, whole body, in ms per step.
+// the engine's stream around each launch.  out[0..7] = kick (+ bin in cell
+// mode), bin, scan_reduce, scan_apply, scatter, list build, forces, whole
+// body, in ms per step (list mode: the rebuild kernels are gated, so their
+// time is the average over rebuild and skip steps).
 int Engine::profile(int nsteps, double *out, std::string &err) {
     if (nsteps <= 0) { err = "nsteps must be > 0"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
@@ -664,7 +768,7 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
             acc[k] += t;
         }
         (void)hipEventElapsedTime(&t, e[0], e[K - 1]);
-        acc[K - 1] += t;
+        acc[K - 1] += t;  // out[K - 1] = whole body (from the first event)
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < K; ++k) out[k] = acc[k] / nsteps;
@@ -674,7 +778,15 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
 int Engine::ensure_primed(std::string &err) {
     if (primed_) return FM_OK;
     if (launch_bin()) return check(hipGetLastError(), "bin", err);
-    if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
+    if (list_) {
+        // a fresh list epoch at the current positions
+        if (launch_sort()) return check(hipGetLastError(), "sort", err);
+        if (launch_list_build(false)) return check(hipGetLastError(), "list build", err);
+        if (launch_ctl_reset(ctl_.p, R_, stream_) != hipSuccess) return check(hipGetLastError(), "ctl", err);
+        if (launch_pair_pass()) return check(hipGetLastError(), "forces", err);
+    } else {
+        if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
+    }
     primed_ = true;
     return FM_OK;
 }
@@ -692,16 +804,12 @@ int Engine::step(int nsteps, std::string &err) {
     int rem = nsteps - 1;
     if (use_graph_ && rem >= kGraphSteps + 1) {
         if (!graph_exec_) {
-            // the captured bodies alternate the counter halves; an even
-            // number of them returns to the parity they started from
-            graph_parity_ = parity_;
             rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture", err);
             if (rc) return rc;
             int bad = 0;
             for (int s = 0; s < kGraphSteps; ++s) bad |= launch_step_body(true, true);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(stream_, &g);
-            parity_ = graph_parity_;
             if (bad || ce != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
                 return check(ce != hipSuccess ? ce : hipErrorLaunchFailure, "capture step", err);
@@ -710,10 +818,6 @@ int Engine::step(int nsteps, std::string &err) {
             rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate", err);
             if (rc) { drop_graph(); return rc; }
             graph_steps_ = kGraphSteps;
-        }
-        if (parity_ != graph_parity_) {
-            if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
-            --rem;
         }
         while (rem >= graph_steps_) {
             rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);
@@ -884,6 +988,8 @@ int Engine::get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
     if (rc) return rc;
+    // bin and sort the current positions (list mode sorts only on rebuilds)
+    if (launch_bin() || launch_sort()) return check(hipGetLastError(), "cells", err);
     const RepDesc &D = reps_h_[r];
     if (ncx) *ncx = D.ncx;
     if (ncy) *ncy = D.ncy;
@@ -941,6 +1047,7 @@ int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string
     dbg_cnt_.release();
     dbg_s1_.release();
     dbg_s2_.release();
+    primed_ = false;  // the dump pass overwrote the pair partials
     return FM_OK;
 }
 
@@ -953,7 +1060,13 @@ int Engine::time_forces(int reps, double *ms, std::string &err) {
     (void)hipEventRecord(a, stream_);
     for (int k = 0; k < reps; ++k) {
         if (launch_bin()) return check(hipGetLastError(), "bin", err);
-        if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
+        if (list_) {
+            // a rebuild (sort + list build) and the list pair pass
+            if (launch_sort() || launch_list_build(false) || launch_pair_pass())
+                return check(hipGetLastError(), "forces", err);
+        } else if (launch_forces_only(false)) {
+            return check(hipGetLastError(), "forces", err);
+        }
     }
     (void)hipEventRecord(b, stream_);
     rc = check(hipEventSynchronize(b), "time", err);
@@ -964,6 +1077,38 @@ int Engine::time_forces(int reps, double *ms, std::string &err) {
     *ms = t / std::max(reps, 1);
     if (rc) return rc;
     return sync(err);
+}
+
+int Engine::list_stats(int r, double *mean, int32_t *maxlen, int64_t *nbuild, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    if (mean) *mean = 0.0;
+    if (maxlen) *maxlen = 0;
+    if (nbuild) *nbuild = 0;
+    if (!list_) return FM_OK;
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    const RepDesc &D = reps_h_[r];
+    const int n = D.p1 - D.p0;
+    std::vector<int32_t> nn(n);
+    Ctl c{};
+    rc = check(hipMemcpyAsync(nn.data(), nnum_.p + D.p0, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              stream_), "list stats", err);
+    if (rc) return rc;
+    rc = check(hipMemcpyAsync(&c, ctl_.p + r, sizeof(Ctl), hipMemcpyDeviceToHost, stream_),
+               "list stats", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    double s = 0.0;
+    int32_t m = 0;
+    for (int32_t v : nn) {
+        s += v;
+        m = std::max(m, v);
+    }
+    if (mean) *mean = n ? s / n : 0.0;
+    if (maxlen) *maxlen = m;
+    if (nbuild) *nbuild = c.nbuild;
+    return FM_OK;
 }
 
 }  // namespace fm

@@ -39,6 +39,8 @@ struct HostSystem {
     std::vector<int32_t> bi, bj;
     std::vector<float> bk, br0;
     int32_t next_group = 0;
+    // neighbour-list skin (femmd2D/src/nn.h:63); 0 = cell walk every step
+    float skin = 0.4f;
     // launch tuning
     int32_t tile_w = 0, block = 256, use_graph = 1;
 };
@@ -84,6 +86,10 @@ public:
     int pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err);
     int time_forces(int reps, double *ms, std::string &err);
     int profile(int nsteps, double *out, std::string &err);
+    // neighbour-list statistics of replica r: mean and max list length,
+    // rebuilds so far (0 / 0 / 0 when the engine walks cells every step)
+    int list_stats(int r, double *mean, int32_t *maxlen, int64_t *nbuild, std::string &err);
+    bool list_mode() const { return list_; }
     // adjoint (adjoint.hip): J(x) w and (df/dtheta)^T w at the current
     // positions, caller order; and a checkpointed forward + backward sweep
     int adjoint_op(const float *wx, const float *wy, float *jx, float *jy, double *dE,
@@ -110,10 +116,12 @@ public:
 
 private:
     Engine() = default;
-    int launch_bin();
+    int launch_bin(bool gated = false);
     int launch_step_body(bool kick2, bool drift);
     int launch_forces_only(bool debug);
-    int launch_sort();
+    int launch_sort(bool gated = false);
+    int launch_list_build(bool gated);
+    int launch_pair_pass();  // list mode: list pair pass + elements + bonds
     ForceArgs force_args();
     // positions the binning / force kernels read: the live state, or a
     // stored configuration during the adjoint sweep
@@ -134,6 +142,9 @@ private:
     int R_ = 0, N_ = 0, ncells_ = 0, ntri_ = 0, nbond_ = 0, cap_ = 32, lds_cap_ = 2048;
     int block_ = 256, use_graph_ = 1;
     bool sigma_ = false, types_ = false, gather_ = false;
+    bool list_ = false;  // neighbour-list mode
+    float skin_ = 0.0f;
+    int nbr_cap_ = 0;
     bool primed_ = false;
     std::vector<char> lj_on_;
     int64_t steps_ = 0;
@@ -144,7 +155,7 @@ private:
     std::vector<int32_t> uid_h_;               // internal -> caller id (per replica)
     std::vector<std::vector<int32_t>> iid_h_;  // per replica: caller -> internal
     std::vector<int> tile_w_;                  // per replica tile width in use
-    std::vector<Span> kspans_h_, espans_h_, bspans_h_;
+    std::vector<Span> kspans_h_, espans_h_, bspans_h_, lspans_h_;
     std::vector<Tile> tiles_h_;
     std::vector<int32_t> rng_h_;  // per replica 8 ints (energy ranges)
     std::vector<double> ene_h_;
@@ -155,7 +166,9 @@ private:
     DBuf<int32_t> grp_, typ_, uid_;
     DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_, key_;
     DBuf<float4> sorted_;
-    DBuf<Span> kspans_, espans_, bspans_;
+    DBuf<Span> kspans_, espans_, bspans_, lspans_;
+    DBuf<int32_t> nbr_, nnum_, any_;
+    DBuf<Ctl> ctl_;
     DBuf<Tile> tiles_;
     DBuf<int32_t> t0_, t1_, t2_, bi_, bj_, goff_, gent_;
     DBuf<float4> dmbm_;
@@ -168,12 +181,10 @@ private:
     DBuf<int32_t> dbg_cnt_;
     DBuf<long long> dbg_s1_, dbg_s2_;
 
-    // profiling: events before the kick, after kick, reduce, apply,
-    // scatter and forces (set only while profiling)
-    static constexpr int kProfEvents = 6;
+    // profiling: events before the kick and after kick, bin, scan reduce,
+    // scan apply, scatter, list build and forces (set only while profiling)
+    static constexpr int kProfEvents = 8;
     hipEvent_t *prof_ev_ = nullptr;
-    int parity_ = 0;        // which half of the double-buffered cell counters
-    int graph_parity_ = 0;  // parity the captured step graph starts from
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     int graph_steps_ = 0;
@@ -194,8 +205,10 @@ private:
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s);
 hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s);
-hipError_t launch_forces(bool sigma, bool types, bool debug, int block, const ForceArgs &a,
-                         hipStream_t s);
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int block,
+                         const ForceArgs &a, hipStream_t s);
+hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
                               const int32_t *goff, const int32_t *gent, float *ox, float *oy,

@@ -146,7 +146,41 @@ __device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, 
 //   kDrift: v += dt/2 a, x += dt v, wrap, key, bin
 // a = (f_pair + sum of gathered corner forces) * inv_m.
 // ---------------------------------------------------------------------------
-template <bool kKick2, bool kDrift, bool kGather>
+// List mode: the last kick workgroup of a replica to finish adds the
+// step's largest displacement to the replica's running bound and decides
+// the rebuild (femmd2D/src/nn.cpp:114-134, with the drift of this step
+// counted before the decision rather than after it).  d2 = this lane's
+// squared displacement; redd holds one float per wavefront.
+__device__ void list_decide(const KickArgs &a, const Span &sp, const RepDesc &R, float d2,
+                            float *redd) {
+    const float m = wave_max(d2);
+    if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    float mm = 0.0f;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mm = fmaxf(mm, redd[k]);
+    Ctl *c = a.ctl + sp.rep;
+    atomicMax(&c->dmax, __float_as_uint(mm));  // non-negative floats order as uints
+    __threadfence();
+    const uint32_t t = atomicAdd(&c->done, 1u);
+    if (t != (uint32_t)(R.nks - 1)) return;
+    __threadfence();
+    const float dm = __uint_as_float(atomicOr(&c->dmax, 0u));
+    // the stored positions move by the rounded sums x + dt v (and the
+    // wrap): dslack bounds what rounding adds to the exact displacement
+    const float disp = c->disp + sqrtf(dm) + R.dslack;
+    const int flag = disp > 0.5f * R.skin;
+    c->flag = flag;
+    c->disp = flag ? 0.0f : disp;
+    c->dmax = 0u;
+    c->done = 0u;
+    if (flag) {
+        c->nbuild += 1u;
+        atomicOr(a.any, 1);
+    }
+}
+
+template <bool kKick2, bool kDrift, bool kGather, bool kList>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     __shared__ double red[2 * (kKickThreads / 64)];
     __shared__ float redm[kKickThreads / 64];
@@ -154,7 +188,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const RepDesc R = a.reps[sp.rep];
     const int p = sp.begin + (int)threadIdx.x;
     double ke = 0.0;
-    float vvmax = 0.0f;
+    float vvmax = 0.0f, d2 = 0.0f;
     int cell = 0;
     if (p < sp.end) {
         float fx = a.fx[p], fy = a.fy[p];
@@ -179,8 +213,10 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         if (kDrift) {
             vx = vx + R.hdt * ax;
             vy = vy + R.hdt * ay;
-            float x = a.x[p] + R.dt * vx;
-            float y = a.y[p] + R.dt * vy;
+            const float dxp = R.dt * vx, dyp = R.dt * vy;
+            float x = a.x[p] + dxp;
+            float y = a.y[p] + dyp;
+            if (kList) d2 = dxp * dxp + dyp * dyp;
             x = wrap1(x, R.Lx);
             y = wrap1(y, R.Ly);
             if (!(isfinite(x) && isfinite(y))) {
@@ -190,12 +226,16 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             }
             a.x[p] = x;
             a.y[p] = y;
-            cell = R.cell0 + cell_key(x, y, R);
+            if (!kList) cell = R.cell0 + cell_key(x, y, R);
         }
         a.vx[p] = vx;
         a.vy[p] = vy;
     }
-    if (kDrift) bin_wave(a, p < sp.end, p, cell);
+    if (kDrift && !kList) bin_wave(a, p < sp.end, p, cell);
+    if (kDrift && kList) {
+        __shared__ float redd[kKickThreads / 64];
+        list_decide(a, sp, R, d2, redd);
+    }
     if (kKick2) {
         double vm = 0.0;
         // max via a float wave reduction (exact, order independent)
@@ -241,6 +281,7 @@ __global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
 // fm_set_state / fm_commit and by fm_compute_forces).
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
     const Span sp = a.spans[blockIdx.x];
+    if (a.ctl && !a.ctl[sp.rep].flag) return;  // list mode, no rebuild
     const RepDesc R = a.reps[sp.rep];
     const int p = sp.begin + (int)threadIdx.x;
     const bool active = p < sp.end;
@@ -263,6 +304,7 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
     __shared__ int red[kScanThreads / 64];
+    if (a.any && !*a.any) return;  // list mode, no replica rebuilds
     const int base = blockIdx.x * kScanCells + 4 * (int)threadIdx.x;
     int s = 0;
 #pragma unroll
@@ -280,13 +322,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
     }
 }
 
-// Exclusive scan of the (capped) cell counts -> cell_start; the other half
-// of the double-buffered counters (used by the previous binning, whose
-// scatter has completed) is zeroed for the next binning.
+// Exclusive scan of the (capped) cell counts -> cell_start; each counter is
+// zeroed once read (the scatter takes counts from cell_start), ready for
+// the next binning.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
     __shared__ int red[kScanThreads / 64];
     __shared__ int wsum[kScanThreads / 64];
     __shared__ int s_pre;
+    if (a.any && !*a.any) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     int pre = 0;
     for (int k = t; k < (int)blockIdx.x; k += kScanThreads) pre += a.bsum[k];
@@ -318,7 +361,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
         const int c = base + q;
         if (c >= a.ncells) break;
         a.start[c] = run;
-        a.count_zero[c] = 0;
+        a.count[c] = 0;
         run += n[q];
         if (c == a.ncells - 1) a.start[a.ncells] = run;
     }
@@ -329,15 +372,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
 // femmd2D/src/nn.cpp:8-20, deterministic); positions, exclusion group and
 // sigma are copied into sorted order for the force pass.
 __global__ __launch_bounds__(kScanThreads) void k_scatter(ScanArgs a) {
-    const int i = blockIdx.x * kScanThreads + (int)threadIdx.x;
-    if (i >= a.n) return;
+    const Span sp = a.spans[blockIdx.x];
+    if (a.ctl && !a.ctl[sp.rep].flag) return;  // list mode, no rebuild
+    const int i = sp.begin + (int)threadIdx.x;
+    if (i >= sp.end) return;
     const int c = a.key[i];
     if (c < 0) return;  // dropped by a cell overflow (flagged)
-    const int m = min(a.count[c], a.cap);
+    const int st = a.start[c];
+    const int m = a.start[c + 1] - st;
     const int *src = a.tmp + (size_t)c * a.cap;
     int rank = 0;
     for (int k = 0; k < m; ++k) rank += (src[k] < i);
-    const int pos = a.start[c] + rank;
+    const int pos = st + rank;
     a.perm[pos] = i;
     float4 e;
     e.x = a.x[i];
@@ -528,7 +574,7 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
 // 26-37; sorted positions are contiguous per cell.  With wsrc set (the
 // adjoint pass) the per-particle vector w is staged beside each entry.
 // Returns the number of staged particles, or -1 if they exceed the LDS.
-template <bool kTypes, bool kDebug>
+template <bool kTypes, bool kDebug, bool kPid = false>
 __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
                           PairLds &L, const float2 *wsrc, float2 *lw) {
     const int row = W + 2, S = 3 * row;
@@ -580,6 +626,7 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
         L.ent[e] = a.sorted[p];
         if (kTypes) L.typ[e] = a.sorted_typ[p];
         if (kDebug) L.sp[e] = p;
+        if (kPid) L.sp[e] = a.perm[p];  // internal id (list build)
         if (wsrc) lw[e] = wsrc[a.perm[p]];
     }
     __syncthreads();
@@ -604,6 +651,180 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
         pair_tile_body<true, kSigma, kTypes, kDebug>(a, R, W, tile, L);
     else
         pair_tile_body<false, kSigma, kTypes, kDebug>(a, R, W, tile, L);
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list mode (femmd2D/src/nn.cpp:22-62, re-designed): a tile of
+// cells of side >= rc*sigma + skin is staged as in the pair pass; every
+// centre particle records the partners within rc*sigma + skin (exclusion
+// rule applied here, once) in column-major order, nbr[k * n + i], so that a
+// wavefront of consecutive particles reads one list slot coalesced.
+// ---------------------------------------------------------------------------
+template <bool kWrap, bool kSigma>
+__device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L) {
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const PairEntry pe = L.pt[0];
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int me = c_lo + k;
+        int lo = row + 1, hi = row + W + 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= me) lo = mid; else hi = mid;
+        }
+        const int col = lo - row;
+        const float4 pi = L.ent[me];
+        const float xi = pi.x, yi = pi.y;
+        const int gi = __float_as_int(pi.z);
+        const int gix = gi >= 0 ? gi : -2;
+        const float si = pi.w;
+        const int a0 = L.off[col - 1], b0 = L.off[col + 2];
+        const int a1 = L.off[row + col - 1], b1 = L.off[row + col + 2];
+        const int a2 = L.off[2 * row + col - 1], b2 = L.off[2 * row + col + 2];
+        const int total = (b0 - a0) + (b1 - a1) + (b2 - a2);
+        int j = a0;
+        if (j == b0) j = a1;
+        if (j == b1) j = a2;
+        const int id = a.perm[sbase + k];
+        int32_t *dst = a.nbr + id;
+        int cnt = 0;
+        for (int t = 0; t < total; ++t) {
+            const float4 pj = L.ent[j];
+            const int gj = __float_as_int(pj.z);
+            float dx = xi - pj.x;
+            float dy = yi - pj.y;
+            if (kWrap) {
+                if (dx >= hLx) dx -= Lx; else if (dx < -hLx) dx += Lx;
+                if (dy >= hLy) dy -= Ly; else if (dy < -hLy) dy += Ly;
+            }
+            const float dr2 = dx * dx + dy * dy;
+            float rl2;
+            if (kSigma) {
+                const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
+                rl2 = (rl * rl) * 1.00002f;
+            } else {
+                rl2 = R.rl2_uniform;
+            }
+            if ((dr2 < rl2) & (gj != gix) & (j != me)) {
+                if (cnt < a.nbr_cap) dst[(size_t)cnt * a.n] = L.sp[j];
+                ++cnt;
+            }
+            ++j;
+            if (j == b0) j = a1;
+            if (j == b1) j = a2;
+        }
+        a.nnum[id] = min(cnt, a.nbr_cap);
+        if (cnt > a.nbr_cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
+    }
+}
+
+template <bool kSigma>
+__global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
+    extern __shared__ float4 dyn[];
+    __shared__ int s_cstart[kMaxSlots];
+    __shared__ int s_off[kMaxSlots + 1];
+    __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
+    const int tile = xcd_remap(blockIdx.x, a.ntiles);
+    const Tile T = a.tiles[tile];
+    if (a.ctl && !a.ctl[T.rep].flag) return;
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    PairLds L;
+    L.ent = dyn;
+    L.typ = nullptr;
+    L.sp = reinterpret_cast<int *>(dyn + a.lds_cap);
+    L.cstart = s_cstart;
+    L.off = s_off;
+    L.pt = s_pt;
+    L.red = nullptr;
+    if (stage_tile<false, false, true>(a, T, R, W, L, nullptr, nullptr) < 0) {
+        if (threadIdx.x == 0) atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+        return;
+    }
+    if ((T.w_flags >> 16) & 1)
+        list_tile_body<true, kSigma>(a, R, W, L);
+    else
+        list_tile_body<false, kSigma>(a, R, W, L);
+}
+
+// LJ over the neighbour lists: one lane per particle of a kick span, the
+// pair maths and accumulation of pair_tile_body (FP32 cutoff predicate on
+// the current positions, so the interacting set is the same as the cell
+// walk's whenever the list is complete).
+template <bool kSigma>
+__device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
+    const Span sp = a.lspans[b];
+    const RepDesc R = a.reps[sp.rep];
+    const PairEntry pe = a.ptab[R.pt];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    double Ublk = 0.0, Wblk = 0.0;
+    for (int i = sp.begin + (int)threadIdx.x; i < sp.end; i += blockDim.x) {
+        const float xi = a.x[i], yi = a.y[i];
+        const float xim = xi - Lx, yim = yi - Ly;
+        const float si = kSigma ? a.sig[i] : 0.0f;
+        const int n = a.nnum[i];
+        const int32_t *lst = a.nbr + i;
+        float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
+        int nint = 0;
+        for (int k = 0; k < n; ++k) {
+            const int j = lst[(size_t)k * a.n];
+            const float xj = a.x[j], yj = a.y[j];
+            float dx = xi - xj;
+            float dy = yi - yj;
+            int sx = 0, sy = 0;
+            if (dx >= hLx) { dx -= Lx; sx = 1; } else if (dx < -hLx) { dx += Lx; sx = -1; }
+            if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
+            const float dr2 = dx * dx + dy * dy;
+            float s2ij;
+            unsigned rc2m1;
+            if (kSigma) {
+                const float sij = (si + a.sig[j]) * 0.5f;
+                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                s2ij = sij * sij;
+            } else {
+                rc2m1 = rc2m1_u;
+                s2ij = R.s2_uniform;
+            }
+            if ((__float_as_uint(dr2) - 1u) < rc2m1) {
+                float dxa = dx, dya = dy, r2a = dr2;
+                if (sx | sy) {
+                    if (sx > 0) dxa = xim - xj; else if (sx < 0) dxa = xi - (xj - Lx);
+                    if (sy > 0) dya = yim - yj; else if (sy < 0) dya = yi - (yj - Ly);
+                    r2a = dxa * dxa + dya * dya;
+                }
+                const float inv = __builtin_amdgcn_rcpf(r2a);
+                const float sr2 = s2ij * inv;
+                const float sr4 = sr2 * sr2;
+                const float sr6 = sr4 * sr2;
+                const float s12 = sr6 * sr6;
+                const float tv = __builtin_fmaf(-0.5f, sr6, s12);
+                const float fc = tv * inv;
+                fx = __builtin_fmaf(fc, dxa, fx);
+                fy = __builtin_fmaf(fc, dya, fy);
+                accA += s12;
+                accB += sr6;
+                ++nint;
+            }
+        }
+        fx *= pe.e48;
+        fy *= pe.e48;
+        const double A = accA, B = accB;
+        const double Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
+        const double Wi = (double)pe.e48 * (A - 0.5 * B);
+        a.fx[i] = fx;
+        a.fy[i] = fy;
+        Ublk += 0.5 * Ui;
+        Wblk += 0.5 * Wi;
+    }
+    block_sum2(Ublk, Wblk, red);
+    if (threadIdx.x == 0) {
+        a.lj_part[2 * b] = Ublk;
+        a.lj_part[2 * b + 1] = Wblk;
+    }
 }
 
 // CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
@@ -737,7 +958,7 @@ __device__ void bond_block(const ForceArgs &a, int b, double *red) {
     }
 }
 
-template <bool kSigma, bool kTypes, bool kDebug>
+template <bool kSigma, bool kTypes, bool kDebug, bool kList>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
@@ -745,7 +966,15 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
     __shared__ double s_red[2 * 4];
     int b = blockIdx.x;
-    if (b < a.ntiles) {
+    // the rebuild of this step (if any) is complete: clear its summary
+    if (kList && a.any && b == 0 && threadIdx.x == 0) *a.any = 0;
+    if (kList) {
+        if (b < a.nlspans) {
+            list_pair_block<kSigma>(a, b, s_red);
+            return;
+        }
+        b -= a.nlspans;
+    } else if (b < a.ntiles) {
         PairLds L;
         L.ent = dyn;
         int *tail = reinterpret_cast<int *>(dyn + a.lds_cap);
@@ -757,8 +986,9 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         L.red = s_red;
         pair_tile<kSigma, kTypes, kDebug>(a, xcd_remap(b, a.ntiles), L);
         return;
+    } else {
+        b -= a.ntiles;
     }
-    b -= a.ntiles;
     if (b < a.nespans) {
         elem_block(a, b, s_red);
         return;
@@ -837,6 +1067,24 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
         a.out[8 * r + 1] = m;
     }
+}
+
+// Start a new list epoch (after a forced rebuild): running bound, step max
+// and arrival counter cleared; the forced rebuild is counted.
+__global__ void k_ctl_reset(Ctl *ctl, int nrep) {
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+        ctl[r].flag = 0;
+        ctl[r].disp = 0.0f;
+        ctl[r].dmax = 0u;
+        ctl[r].done = 0u;
+        ctl[r].nbuild += 1u;
+    }
+}
+
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s) {
+    if (nrep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, nrep);
+    return hipGetLastError();
 }
 
 // Zero an int array (graph-safe replacement of a memset node).
@@ -1214,25 +1462,31 @@ hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
-    // 4 = kinetic-energy partials only
+    // 4 = kinetic-energy partials only; a.ctl set = list mode (the drift
+    // decides the rebuild instead of binning)
     if (nblk <= 0) return hipSuccess;
     dim3 g(nblk), b(kKickThreads);
+    const bool list = a.ctl != nullptr;
+#define FM_KICK(K2, D, G) \
+    do { \
+        if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true>), g, b, 0, s, a); \
+        else hipLaunchKernelGGL((k_kick<K2, D, G, false>), g, b, 0, s, a); \
+    } while (0)
     switch (mode) {
         case 0: hipLaunchKernelGGL(k_bin, g, b, 0, s, a); break;
         case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
         case 1:
-            if (gather) hipLaunchKernelGGL((k_kick<false, true, true>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_kick<false, true, false>), g, b, 0, s, a);
+            if (gather) FM_KICK(false, true, true); else FM_KICK(false, true, false);
             break;
         case 2:
-            if (gather) hipLaunchKernelGGL((k_kick<true, false, true>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_kick<true, false, false>), g, b, 0, s, a);
+            if (gather) hipLaunchKernelGGL((k_kick<true, false, true, false>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_kick<true, false, false, false>), g, b, 0, s, a);
             break;
         default:
-            if (gather) hipLaunchKernelGGL((k_kick<true, true, true>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_kick<true, true, false>), g, b, 0, s, a);
+            if (gather) FM_KICK(true, true, true); else FM_KICK(true, true, false);
             break;
     }
+#undef FM_KICK
     return hipGetLastError();
 }
 
@@ -1243,8 +1497,8 @@ hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s
     } else if (part == 1) {
         hipLaunchKernelGGL(k_scan_apply, dim3(nblk), dim3(kScanThreads), 0, s, a);
     } else {
-        const int np = (a.n + kScanThreads - 1) / kScanThreads;
-        if (np > 0) hipLaunchKernelGGL(k_scatter, dim3(np), dim3(kScanThreads), 0, s, a);
+        if (a.nspans > 0)
+            hipLaunchKernelGGL(k_scatter, dim3(a.nspans), dim3(kScanThreads), 0, s, a);
     }
     return hipGetLastError();
 }
@@ -1262,13 +1516,23 @@ size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
            (debug ? lds_cap * sizeof(int) : 0);
 }
 
-hipError_t launch_forces(bool sigma, bool types, bool debug, int block,
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int block,
                          const ForceArgs &a, hipStream_t s) {
+    if (list) {
+        // neighbour lists: single type, no dump (the caller checks)
+        const int nblk = a.nlspans + a.nespans + a.nbspans;
+        if (nblk <= 0) return hipSuccess;
+        if (sigma)
+            hipLaunchKernelGGL((k_forces<true, false, false, true>), dim3(nblk), dim3(block), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_forces<false, false, false, true>), dim3(nblk), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
     const int nblk = a.ntiles + a.nespans + a.nbspans;
     if (nblk <= 0) return hipSuccess;
     const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
     dim3 g(nblk), b(block);
-#define FM_LAUNCH(S, T, D) hipLaunchKernelGGL((k_forces<S, T, D>), g, b, lds, s, a)
+#define FM_LAUNCH(S, T, D) hipLaunchKernelGGL((k_forces<S, T, D, false>), g, b, lds, s, a)
     if (debug) {
         if (sigma) { if (types) FM_LAUNCH(true, true, true); else FM_LAUNCH(true, false, true); }
         else { if (types) FM_LAUNCH(false, true, true); else FM_LAUNCH(false, false, true); }
@@ -1277,6 +1541,16 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, int block,
         else { if (types) FM_LAUNCH(false, true, false); else FM_LAUNCH(false, false, false); }
     }
 #undef FM_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s) {
+    if (a.ntiles <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.lds_cap * (sizeof(float4) + sizeof(int));
+    if (sigma)
+        hipLaunchKernelGGL((k_list_build<true>), dim3(a.ntiles), dim3(block), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_list_build<false>), dim3(a.ntiles), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ typedef struct fm_batch fm_batch;   /* many independent replicas */
 #define FM_DEV_TILE_OVERFLOW 2u  /* a cell tile did not fit in LDS         */
 #define FM_DEV_INVERTED 4u       /* an element with det F <= 0             */
 #define FM_DEV_NONFINITE 8u      /* a non-finite coordinate after a drift  */
+#define FM_DEV_LIST_OVERFLOW 16u /* more neighbours than the list capacity */
 
 /* ---- material models (femmd2D/src/interactions.cpp:411-689) ---- */
 #define FM_NEO_HOOKEAN 0 /* calcFeaForcesCPU2, svk = 0 (reference CPU) */
@@ -125,6 +126,14 @@ int fm_set_lj(fm_system *h, int32_t on);
  * femmd2D/src/particles.cpp:289-293); -1 = none.  Before fm_commit. */
 int fm_set_exclusion(fm_system *h, int32_t i, int32_t group);
 
+/* Neighbour-list skin (femmd2D/src/nn.h:63, default 0.4), before fm_commit.
+ * skin > 0: pair forces run over Verlet lists of radius rc*sigma + skin,
+ * rebuilt (binning, counting sort, list build) only when the running sum of
+ * per-step maximum displacements exceeds skin / 2 (femmd2D/src/nn.cpp:
+ * 114-134), so most steps skip the sort.  skin = 0: the cells are walked
+ * every step.  Typed pair tables (ntypes > 1) always walk the cells. */
+int fm_set_skin(fm_system *h, float skin);
+
 /* Masses of the given particles (Particles::mass, femmd2D/src/particles.h:56).
  * A zero, negative or infinite mass pins the particle (it never moves).
  * Before fm_commit. */
@@ -163,9 +172,14 @@ int fm_pair_sets(fm_system *h, int32_t *cnt, int64_t *s1, int64_t *s2);
 /* Time one force pass on the device (ms per pass, averaged over reps). */
 int fm_time_forces(fm_system *h, int32_t reps, double *ms);
 /* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel with HIP
- * events on the engine's stream (no graph).  ms[6] = per step: kick + bin,
- * scan reduce, scan apply, scatter, force pass, whole step. */
+ * events on the engine's stream (no graph).  ms[8] = per step: kick (with
+ * the binning when cells are walked every step), binning (list mode),
+ * scan reduce, scan apply, scatter, list build, force pass, whole step. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
+/* Neighbour-list statistics: mean and max list length at the last build,
+ * rebuilds so far (all zero when the cells are walked every step). */
+int fm_list_stats(fm_system *h, double *mean_len, int32_t *max_len,
+                  int64_t *rebuilds);
 
 /* ---- adjoint (no reference counterpart; single-type systems) ----
  * Directional derivative of the forces at the current positions: for a

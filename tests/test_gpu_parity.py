@@ -12,6 +12,11 @@ Bars:
   * 100-step trajectories against an FP64 oracle rollout;
   * run-twice, graph/no-graph, tile-shape and batched-vs-looped runs are
     bitwise identical.
+
+Both pair paths run every case: the neighbour lists (skin 0.4, the engine
+default) and the per-step cell walk (skin 0).  Forces checked after some
+steps in list mode come from lists built steps earlier, so a list that
+missed an interacting pair fails the force bar.
 """
 import numpy as np
 import pytest
@@ -24,8 +29,11 @@ pytestmark = pytest.mark.gpu
 FORCE_TOL = 1e-5
 
 
-def engine_and_oracle(w):
-    s = configs.build_system(w)
+SKINS = [0.4, 0.0]
+
+
+def engine_and_oracle(w, skin=None):
+    s = configs.build_system(w, skin=skin)
     x, y, vx, vy = s.get_state()
     return s, O.from_workload(w, x, y), (x, y, vx, vy)
 
@@ -79,15 +87,16 @@ def test_pair_sets_bit_exact(cfg):
         s.run(40)
 
 
+@pytest.mark.parametrize("skin", SKINS)
 @pytest.mark.parametrize("cfg", ["c1", "c2s", "c3s", "c5s"])
-def test_forces_within_1e5_of_fp64(cfg):
+def test_forces_within_1e5_of_fp64(cfg, skin):
     w = {"c1": lambda: configs.c1(), "c2s": lambda: configs.c2(scale=0.01),
          "c3s": lambda: configs.c3(scale=0.005), "c5s": lambda: configs.c5(scale=0.01)}[cfg]()
-    s, osys, _ = engine_and_oracle(w)
+    s, osys, _ = engine_and_oracle(w, skin)
     for step in range(3):
         x, y, _, _ = s.get_state()
         err, en = force_err(s, osys.with_positions(x, y))
-        print(f"[{cfg} step {step * 30}] max rel force err {err.max():.3e}")
+        print(f"[{cfg} skin {skin} step {step * 30}] max rel force err {err.max():.3e}")
         assert err.max() <= FORCE_TOL
         check_energies(s, en)
         s.run(30)
@@ -116,9 +125,10 @@ def test_material_models(model):
     check_energies(s, en)
 
 
-def test_trajectory_100_steps_and_energy_drift():
+@pytest.mark.parametrize("skin", SKINS)
+def test_trajectory_100_steps_and_energy_drift(skin):
     w = configs.c1()
-    s, osys, (x0, y0, vx0, vy0) = engine_and_oracle(w)
+    s, osys, (x0, y0, vx0, vy0) = engine_and_oracle(w, skin)
     ga = w.global_arrays()
     e0 = s.energies()
     s.run(100)
@@ -142,11 +152,12 @@ def test_trajectory_100_steps_and_energy_drift():
     assert abs(E1 - oE1) / abs(E0) < 1e-4
 
 
-def test_run_twice_bitwise_and_graph_invariance():
+@pytest.mark.parametrize("skin", SKINS)
+def test_run_twice_bitwise_and_graph_invariance(skin):
     w = configs.c2(scale=0.005)
     res = []
     for graph, tile, block in [(1, 0, 256), (1, 0, 256), (0, 0, 256), (1, 8, 128), (1, 3, 64)]:
-        s = configs.build_system(w, commit=False)
+        s = configs.build_system(w, commit=False, skin=skin)
         s.set_tuning(tile, block, graph)
         s.allocate_cuda()
         s.run(37)
@@ -157,16 +168,17 @@ def test_run_twice_bitwise_and_graph_invariance():
             np.testing.assert_array_equal(a, b)
 
 
-def test_batched_equals_looped_bitwise():
+@pytest.mark.parametrize("skin", SKINS)
+def test_batched_equals_looped_bitwise(skin):
     ws = configs.c4(n_replicas=6, n_max=3000)
     singles = []
     for w in ws:
-        s = configs.build_system(w)
-        s.run(25)
+        s = configs.build_system(w, skin=skin)
+        s.run(60)
         singles.append((s.get_state(), s.energies()))
-    systems = [configs.build_system(w, commit=False) for w in ws]
+    systems = [configs.build_system(w, commit=False, skin=skin) for w in ws]
     b = Batch(systems)
-    b.run(25)
+    b.run(60)
     for r, (st, en) in enumerate(singles):
         bs = b.get_state(r)
         for a, c in zip(st, bs):
@@ -212,3 +224,42 @@ def test_inverted_element_is_reported():
     with pytest.raises(FemmdError):
         s.calc_forces()
     assert s.device_errors() & _lib.FM_DEV_INVERTED
+
+
+def test_hot_system_lists_stay_complete():
+    """Three times the thermal speed: many rebuilds; forces from lists built
+    steps earlier must still meet the bar at every check."""
+    w = configs.c2(scale=0.005)
+    s, osys, (x, y, vx, vy) = engine_and_oracle(w, 0.4)
+    s.set_state(None, None, 3.0 * vx, 3.0 * vy)
+    worst = 0.0
+    for k in range(12):
+        s.run(17)
+        x, y, _, _ = s.get_state()
+        err, _ = force_err(s, osys.with_positions(x, y))
+        worst = max(worst, err.max())
+    st = s.list_stats()
+    print(f"[hot] worst force err {worst:.3e}, lists {st}")
+    assert worst <= FORCE_TOL
+    assert 3 <= st["rebuilds"] < 12 * 17
+    assert st["mean"] > 5 and st["max"] <= 512
+    assert s.device_errors() == 0
+
+
+def test_list_and_cell_paths_agree():
+    """Same interacting sets, different summation order: the two paths stay
+    within FP32 rounding of each other over a short run."""
+    w = configs.c1()
+    a = configs.build_system(w, skin=0.4)
+    b = configs.build_system(w, skin=0.0)
+    fa = a.get_forces()
+    fb = b.get_forces()
+    scale = np.abs(np.concatenate(fb)).max()
+    assert np.abs(np.concatenate(fa) - np.concatenate(fb)).max() <= 1e-5 * scale
+    a.run(50)
+    b.run(50)
+    xa, ya, _, _ = a.get_state()
+    xb, yb, _, _ = b.get_state()
+    d = np.hypot((xa - xb + w.Lx / 2) % w.Lx - w.Lx / 2, (ya - yb + w.Ly / 2) % w.Ly - w.Ly / 2)
+    assert d.max() < 1e-3
+    assert a.list_stats()["rebuilds"] >= 1 and b.list_stats()["rebuilds"] == 0
