@@ -268,7 +268,6 @@ def main():
     ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds,
                            ls1["mean"] if list_mode else None)
     kern = "forces"
-    assert prof["forces"] >= max(prof[k] for k in ("kick",) + REBUILD_KERNELS)
     achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
     step_bytes = ab["kick"] + ab["forces"] + rebuild_frac * sum(ab[k] for k in REBUILD_KERNELS)
     mode = "list" if list_mode else "cells"

@@ -39,19 +39,15 @@ struct alignas(16) RepDesc {
     float skin;        // neighbour-list skin (0: no list, cells every step)
     float rl2_uniform; // list radius^2, (rc*s + skin)^2 with a rounding margin
     int32_t ks0, nks;  // this replica's kick spans [ks0, ks0 + nks)
-    float dslack;      // per-step rounding allowance of the displacement bound
+    float dslack;      // rounding allowance of the displacement test
 };
 
-// Per-replica control words of the neighbour-list mode.  The kick kernel's
-// last workgroup of a replica turns the step's largest displacement into
-// the rebuild decision (femmd2D/src/nn.cpp:114-134: dispHi > skin / 2).
+// Per-replica control words of the neighbour-list mode.
 struct alignas(16) Ctl {
     int32_t flag;    // rebuild the list in this step
-    float disp;      // sum of per-step maximum displacements since the build
-    uint32_t dmax;   // bits of this step's max squared displacement
-    uint32_t done;   // kick workgroups of this replica finished this step
-    uint32_t nbuild; // rebuilds so far
-    int32_t pad[3];
+    float disp;      // largest displacement since the last build (this step)
+    uint32_t nbuild; // builds so far
+    int32_t pad;
 };
 
 // One pair-table entry: 48 eps, 4 eps, additive shift, cutoff in sigma units.
@@ -84,11 +80,17 @@ struct KickArgs {
     int32_t cap;
     double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
     uint32_t *err;
-    Ctl *ctl;                  // list mode: per-replica control (else null)
-    int32_t *any;              // list mode: some replica rebuilds this step
+    const Ctl *ctl;            // list mode: binning only for flagged replicas
+    float *xb, *yb;            // list mode: positions at the last list build
+    float *dpart;              // list mode: per workgroup max |x - xb|^2
+    const float2 *fs;          // list mode: pair forces in sorted order
+    const int32_t *spos;       // list mode: particle -> sorted position
+    float2 *xys;               // list mode: positions in sorted order (out)
+    int32_t nspans;            // spans (grid-stride binning)
 };
 
 struct ScanArgs {
+    const RepDesc *reps;
     int32_t ncells;
     int32_t cap;
     int32_t n;                 // particles
@@ -108,6 +110,9 @@ struct ScanArgs {
     const int32_t *typ;        // per-particle type (may be null)
     float4 *sorted;            // sorted position -> {x, y, grp bits, sigma}
     int32_t *sorted_typ;       // sorted position -> type (may be null)
+    int32_t *spos;             // list mode: particle -> sorted position
+    float2 *xys;               // list mode: positions in sorted order
+    float *sigs;               // list mode: sigma in sorted order (may be null)
 };
 
 struct ForceArgs {
@@ -144,9 +149,14 @@ struct ForceArgs {
     const Span *lspans;
     int32_t nlspans;
     int32_t nbr_cap;           // list capacity per particle
-    int32_t *nbr;              // nbr[k * n + i]: k-th neighbour of particle i
-    int32_t *nnum;             // list length per particle
-    const float *sig;          // per-particle sigma (list mode, sigma path)
+    int32_t *nbr;              // nbr[k * n + p]: k-th neighbour (internal id)
+                               // of the particle at sorted position p
+    int32_t *nnum;             // list length per sorted position
+    int32_t lbuf;              // list build: stage entries in LDS (coalesced
+                               // column writes) when set
+    const float2 *xys;         // list mode: positions in sorted order
+    float2 *fs;                // list mode: pair forces in sorted order
+    const float *sigs;         // list mode: sigma in sorted order (sigma path)
     const Ctl *ctl;            // list build gating (null: always build)
     int32_t *any;              // reset by the force pass after the rebuild
     int32_t n;                 // particles (stride of nbr)

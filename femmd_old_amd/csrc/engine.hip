@@ -407,14 +407,27 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         FM_AL(nnum_, N);
         FM_AL(ctl_, R);
         FM_AL(any_, 1);
+        FM_AL(xb_, N);
+        FM_AL(yb_, N);
+        FM_AL(dpart_, ks.size());
+        FM_AL(spos_, N);
+        FM_AL(xys_, N);
+        FM_AL(fs_, N);
+        if (E->sigma_) FM_AL(sigs_, N);
     }
 #undef FM_AL
     if ((he = hipMemsetAsync(E->count_.p, 0, sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
         return E->check(he, "memset", err);
+    // every sorted slot holds a valid particle index even where a cell
+    // overflow left it unwritten (the list pass reads perm over whole ranges)
+    if ((he = hipMemsetAsync(E->perm_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess)
+        return E->check(he, "memset", err);
     if (E->list_) {
         if ((he = hipMemsetAsync(E->nnum_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess ||
             (he = hipMemsetAsync(E->ctl_.p, 0, sizeof(Ctl) * R, st)) != hipSuccess ||
-            (he = hipMemsetAsync(E->any_.p, 0, sizeof(int32_t), st)) != hipSuccess)
+            (he = hipMemsetAsync(E->any_.p, 0, sizeof(int32_t), st)) != hipSuccess ||
+            (he = hipMemsetAsync(E->spos_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess ||
+            (he = hipMemsetAsync(E->fs_.p, 0, sizeof(float2) * N, st)) != hipSuccess)
             return E->check(he, "memset", err);
     }
     if ((he = hipMemsetAsync(E->err_.p, 0, sizeof(uint32_t), st)) != hipSuccess)
@@ -496,7 +509,8 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
         g[7] = bb;
     }
     // LDS capacity: the close-packed count of the staged cells, rounded up
-    int cap = next_pow2((int)std::ceil(worst));
+    // to a multiple of 128
+    int cap = ((int)std::ceil(worst) + 127) / 128 * 128;
     const int hard = (types_ ? 2560 : 3072);  // dynamic LDS stays <= 64 KB
     lds_cap_ = std::max(256, std::min(cap, hard));
     hipError_t he;
@@ -565,6 +579,11 @@ int Engine::launch_bin(bool gated) {
     a.cap = cap_;
     a.err = err_.p;
     a.ctl = gated ? ctl_.p : nullptr;
+    a.nspans = (int)kspans_h_.size();
+    if (list_) {
+        a.xb = xb_.p;
+        a.yb = yb_.p;
+    }
     return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
@@ -616,8 +635,12 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.ke_part = ke_part_.p;
     k.err = err_.p;
     if (list_) {
-        k.ctl = ctl_.p;
-        k.any = any_.p;
+        k.xb = xb_.p;
+        k.yb = yb_.p;
+        k.dpart = dpart_.p;
+        k.fs = fs_.p;
+        k.spos = spos_.p;
+        k.xys = xys_.p;
     }
     const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
     if (prof_ev_) (void)hipEventRecord(prof_ev_[0], stream_);
@@ -627,7 +650,8 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     if (!drift) return 0;
     if (!list_) return launch_forces_only(false);
     // list mode: binning, sort and list build run only for the replicas
-    // whose displacement bound crossed skin / 2 this step
+    // where some particle moved more than skin / 2 since the last build
+    if (launch_list_decide(R_, reps_.p, dpart_.p, ctl_.p, any_.p, stream_) != hipSuccess) return -1;
     if (launch_bin(true)) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
     if (launch_sort(true)) return -1;
@@ -642,8 +666,14 @@ int Engine::launch_sort(bool gated) {
                            perm_.p, xcur(), ycur(), grp_.p, sigma_ ? sig_.p : nullptr,
                            types_ ? typ_.p : nullptr, sorted_.p,
                            types_ ? sorted_typ_.p : nullptr);
+    s.reps = reps_.p;
     s.spans = kspans_.p;
     s.nspans = (int)kspans_h_.size();
+    if (list_) {
+        s.spos = spos_.p;
+        s.xys = xys_.p;
+        s.sigs = sigma_ ? sigs_.p : nullptr;
+    }
     if (gated) {
         s.ctl = ctl_.p;
         s.any = any_.p;
@@ -721,7 +751,12 @@ ForceArgs Engine::force_args() {
     f.nbr_cap = nbr_cap_;
     f.nbr = nbr_.p;
     f.nnum = nnum_.p;
-    f.sig = sig_.p;
+    // stage list entries in LDS when the tile and the buffer fit in 64 KB
+    // (static LDS of the kernel: ~2 KB)
+    f.lbuf = list_build_lds_bytes(lds_cap_, nbr_cap_, block_, true) <= 62 * 1024;
+    f.xys = xys_.p;
+    f.fs = fs_.p;
+    f.sigs = sigs_.p;
     f.any = list_ ? any_.p : nullptr;
     f.n = N_;
     f.uid = uid_.p;
@@ -962,7 +997,8 @@ int Engine::get_forces(int r, float *fx, float *fy, std::string &err) {
     int rc = ensure_primed(err);
     if (rc) return rc;
     if (launch_total_force(N_, fx_.p, fy_.p, cf_.p, gather_ ? goff_.p : nullptr, gent_.p,
-                           outx_.p, outy_.p, stream_) != hipSuccess)
+                           list_ ? spos_.p : nullptr, fs_.p, outx_.p, outy_.p,
+                           stream_) != hipSuccess)
         return check(hipGetLastError(), "total force", err);
     const RepDesc &D = reps_h_[r];
     const int n = D.p1 - D.p0;
@@ -988,8 +1024,11 @@ int Engine::get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
     if (rc) return rc;
-    // bin and sort the current positions (list mode sorts only on rebuilds)
+    // bin and sort the current positions (list mode sorts only on rebuilds;
+    // this sort is not followed by a list build: start a fresh list epoch
+    // on the next use)
     if (launch_bin() || launch_sort()) return check(hipGetLastError(), "cells", err);
+    if (list_) primed_ = false;
     const RepDesc &D = reps_h_[r];
     if (ncx) *ncx = D.ncx;
     if (ncy) *ncy = D.ncy;

@@ -116,6 +116,10 @@ public:
 
 private:
     Engine() = default;
+    // gated: only replicas flagged for a rebuild.  In list mode every
+    // binning records the reference positions of the displacement test and
+    // every sort writes the sorted-order state; a sort that no list build
+    // follows must leave primed_ false.
     int launch_bin(bool gated = false);
     int launch_step_body(bool kick2, bool drift);
     int launch_forces_only(bool debug);
@@ -169,6 +173,9 @@ private:
     DBuf<Span> kspans_, espans_, bspans_, lspans_;
     DBuf<int32_t> nbr_, nnum_, any_;
     DBuf<Ctl> ctl_;
+    DBuf<float> xb_, yb_, dpart_, sigs_;
+    DBuf<int32_t> spos_;
+    DBuf<float2> xys_, fs_;
     DBuf<Tile> tiles_;
     DBuf<int32_t> t0_, t1_, t2_, bi_, bj_, goff_, gent_;
     DBuf<float4> dmbm_;
@@ -208,11 +215,14 @@ hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s
 hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int block,
                          const ForceArgs &a, hipStream_t s);
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
+size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered);
 hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
+hipError_t launch_list_decide(int nrep, const RepDesc *reps, const float *dpart, Ctl *ctl,
+                              int32_t *any, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
-                              const int32_t *goff, const int32_t *gent, float *ox, float *oy,
-                              hipStream_t s);
+                              const int32_t *goff, const int32_t *gent, const int32_t *spos,
+                              const float2 *fs, float *ox, float *oy, hipStream_t s);
 hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
                          const double *fea, const double *bond, double *out, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);

@@ -20,9 +20,15 @@
 // must agree bit for bit with the reference's real=float arithmetic (wrap,
 // cell key, minimum image, cutoff predicate) is evaluated in its source
 // order; fused multiply-adds are written explicitly where wanted.
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace fm {
+
+// Grid of the gated list-mode kernels (binning, scatter, list build): 4
+// workgroups per CU loop over the work, so a skipped step costs little.
+constexpr int kGatedGrid = 1024;
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -146,40 +152,6 @@ __device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, 
 //   kDrift: v += dt/2 a, x += dt v, wrap, key, bin
 // a = (f_pair + sum of gathered corner forces) * inv_m.
 // ---------------------------------------------------------------------------
-// List mode: the last kick workgroup of a replica to finish adds the
-// step's largest displacement to the replica's running bound and decides
-// the rebuild (femmd2D/src/nn.cpp:114-134, with the drift of this step
-// counted before the decision rather than after it).  d2 = this lane's
-// squared displacement; redd holds one float per wavefront.
-__device__ void list_decide(const KickArgs &a, const Span &sp, const RepDesc &R, float d2,
-                            float *redd) {
-    const float m = wave_max(d2);
-    if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    float mm = 0.0f;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mm = fmaxf(mm, redd[k]);
-    Ctl *c = a.ctl + sp.rep;
-    atomicMax(&c->dmax, __float_as_uint(mm));  // non-negative floats order as uints
-    __threadfence();
-    const uint32_t t = atomicAdd(&c->done, 1u);
-    if (t != (uint32_t)(R.nks - 1)) return;
-    __threadfence();
-    const float dm = __uint_as_float(atomicOr(&c->dmax, 0u));
-    // the stored positions move by the rounded sums x + dt v (and the
-    // wrap): dslack bounds what rounding adds to the exact displacement
-    const float disp = c->disp + sqrtf(dm) + R.dslack;
-    const int flag = disp > 0.5f * R.skin;
-    c->flag = flag;
-    c->disp = flag ? 0.0f : disp;
-    c->dmax = 0u;
-    c->done = 0u;
-    if (flag) {
-        c->nbuild += 1u;
-        atomicOr(a.any, 1);
-    }
-}
-
 template <bool kKick2, bool kDrift, bool kGather, bool kList>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     __shared__ double red[2 * (kKickThreads / 64)];
@@ -190,8 +162,18 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     double ke = 0.0;
     float vvmax = 0.0f, d2 = 0.0f;
     int cell = 0;
+    int spi = 0;
     if (p < sp.end) {
-        float fx = a.fx[p], fy = a.fy[p];
+        float fx, fy;
+        if (kList) {
+            spi = a.spos[p];
+            const float2 f = a.fs[spi];
+            fx = f.x;
+            fy = f.y;
+        } else {
+            fx = a.fx[p];
+            fy = a.fy[p];
+        }
         if (kGather) {
             const int e0 = a.goff[p], e1 = a.goff[p + 1];
             for (int e = e0; e < e1; ++e) {
@@ -216,7 +198,6 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             const float dxp = R.dt * vx, dyp = R.dt * vy;
             float x = a.x[p] + dxp;
             float y = a.y[p] + dyp;
-            if (kList) d2 = dxp * dxp + dyp * dyp;
             x = wrap1(x, R.Lx);
             y = wrap1(y, R.Ly);
             if (!(isfinite(x) && isfinite(y))) {
@@ -227,6 +208,14 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             a.x[p] = x;
             a.y[p] = y;
             if (!kList) cell = R.cell0 + cell_key(x, y, R);
+            if (kList) {
+                a.xys[spi] = make_float2(x, y);
+                // displacement since the last list build (minimum image)
+                float ex = x - a.xb[p], ey = y - a.yb[p];
+                if (ex >= R.hLx) ex -= R.Lx; else if (ex < -R.hLx) ex += R.Lx;
+                if (ey >= R.hLy) ey -= R.Ly; else if (ey < -R.hLy) ey += R.Ly;
+                d2 = ex * ex + ey * ey;
+            }
         }
         a.vx[p] = vx;
         a.vy[p] = vy;
@@ -234,7 +223,14 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     if (kDrift && !kList) bin_wave(a, p < sp.end, p, cell);
     if (kDrift && kList) {
         __shared__ float redd[kKickThreads / 64];
-        list_decide(a, sp, R, d2, redd);
+        const float m = wave_max(d2);
+        if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float mm = 0.0f;
+            for (int k = 0; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redd[k]);
+            a.dpart[blockIdx.x] = mm;
+        }
     }
     if (kKick2) {
         double vm = 0.0;
@@ -280,8 +276,9 @@ __global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
 // Binning of the current positions without moving them (used after
 // fm_set_state / fm_commit and by fm_compute_forces).
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
-    const Span sp = a.spans[blockIdx.x];
-    if (a.ctl && !a.ctl[sp.rep].flag) return;  // list mode, no rebuild
+  for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
+    const Span sp = a.spans[b];
+    if (a.ctl && !a.ctl[sp.rep].flag) continue;  // list mode, no rebuild
     const RepDesc R = a.reps[sp.rep];
     const int p = sp.begin + (int)threadIdx.x;
     const bool active = p < sp.end;
@@ -294,8 +291,47 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
             y = 0.0f;
         }
         c = R.cell0 + cell_key(x, y, R);
+        if (a.xb) {  // list mode: the list about to be built starts here
+            a.xb[p] = x;
+            a.yb[p] = y;
+        }
     }
     bin_wave(a, active, p, c);
+  }
+}
+
+// List mode rebuild decision, one workgroup per replica: the largest
+// displacement since the last build against skin / 2 (the reference's
+// criterion, femmd2D/src/nn.cpp:114-134, with the exact displacement in
+// place of its running bound sum of sqrt(max v^2) dt).
+__global__ __launch_bounds__(256) void k_list_decide(const RepDesc *reps, const float *dpart,
+                                                     Ctl *ctl, int32_t *any) {
+    __shared__ float red[4];
+    const RepDesc R = reps[blockIdx.x];
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < R.nks; k += blockDim.x) m = fmaxf(m, dpart[R.ks0 + k]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmaxf(m, red[k]);
+    m = fmaxf(m, red[0]);
+    const float d = sqrtf(m) * 1.0001f + R.dslack;
+    const int flag = d > 0.5f * R.skin;
+    Ctl *c = ctl + blockIdx.x;
+    c->flag = flag;
+    c->disp = d;
+    if (flag) {
+        c->nbuild += 1u;
+        *any = 1;  // every writer stores the same value
+    }
+}
+
+hipError_t launch_list_decide(int nrep, const RepDesc *reps, const float *dpart, Ctl *ctl,
+                              int32_t *any, hipStream_t s) {
+    if (nrep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_list_decide, dim3(nrep), dim3(256), 0, s, reps, dpart, ctl, any);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -372,18 +408,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
 // femmd2D/src/nn.cpp:8-20, deterministic); positions, exclusion group and
 // sigma are copied into sorted order for the force pass.
 __global__ __launch_bounds__(kScanThreads) void k_scatter(ScanArgs a) {
-    const Span sp = a.spans[blockIdx.x];
-    if (a.ctl && !a.ctl[sp.rep].flag) return;  // list mode, no rebuild
+  for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
+    const Span sp = a.spans[b];
+    if (a.ctl && !a.ctl[sp.rep].flag) continue;  // list mode, no rebuild
     const int i = sp.begin + (int)threadIdx.x;
-    if (i >= sp.end) return;
+    if (i >= sp.end) continue;
     const int c = a.key[i];
-    if (c < 0) return;  // dropped by a cell overflow (flagged)
+    if (c < 0) continue;  // dropped by a cell overflow (flagged)
+    const RepDesc &R = a.reps[sp.rep];
     const int st = a.start[c];
     const int m = a.start[c + 1] - st;
     const int *src = a.tmp + (size_t)c * a.cap;
     int rank = 0;
     for (int k = 0; k < m; ++k) rank += (src[k] < i);
-    const int pos = st + rank;
+    // sorted positions of a replica stay in its own range [p0, p1) even
+    // when other replicas were not binned (gated list rebuilds)
+    const int pos = (st - a.start[R.cell0]) + R.p0 + rank;
     a.perm[pos] = i;
     float4 e;
     e.x = a.x[i];
@@ -392,6 +432,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scatter(ScanArgs a) {
     e.w = a.sig ? a.sig[i] : 0.0f;
     a.sorted[pos] = e;
     if (a.typ) a.sorted_typ[pos] = a.typ[i];
+    if (a.spos) {  // list mode: the sorted-order state the list pass reads
+        a.spos[i] = pos;
+        a.xys[pos] = make_float2(e.x, e.y);
+        if (a.sigs) a.sigs[pos] = e.w;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -579,6 +625,7 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
                           PairLds &L, const float2 *wsrc, float2 *lw) {
     const int row = W + 2, S = 3 * row;
     const int tid = threadIdx.x;
+    const int base = R.p0 - a.start[R.cell0];  // see k_scatter
     for (int s = tid; s < S; s += blockDim.x) {
         const int r = s / row, col = s - r * row;
         int cy = T.cy + r - 1;
@@ -587,7 +634,7 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
         if (cx < 0) cx += R.ncx; else if (cx >= R.ncx) cx -= R.ncx;
         const int c = R.cell0 + cy * R.ncx + cx;
         const int st = a.start[c];
-        L.cstart[s] = st;
+        L.cstart[s] = st + base;
         L.off[s + 1] = a.start[c + 1] - st;  // count, scanned below
     }
     for (int k = tid; k < R.ntypes * R.ntypes; k += blockDim.x) L.pt[k] = a.ptab[R.pt + k];
@@ -660,8 +707,9 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
 // rule applied here, once) in column-major order, nbr[k * n + i], so that a
 // wavefront of consecutive particles reads one list slot coalesced.
 // ---------------------------------------------------------------------------
-template <bool kWrap, bool kSigma>
-__device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L) {
+template <bool kWrap, bool kSigma, bool kBuf>
+__device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
+                               uint16_t *buf) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -688,8 +736,8 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         int j = a0;
         if (j == b0) j = a1;
         if (j == b1) j = a2;
-        const int id = a.perm[sbase + k];
-        int32_t *dst = a.nbr + id;
+        const int p = sbase + k;  // sorted position of this centre particle
+        int32_t *dst = a.nbr + p;
         int cnt = 0;
         for (int t = 0; t < total; ++t) {
             const float4 pj = L.ent[j];
@@ -709,14 +757,24 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 rl2 = R.rl2_uniform;
             }
             if ((dr2 < rl2) & (gj != gix) & (j != me)) {
-                if (cnt < a.nbr_cap) dst[(size_t)cnt * a.n] = L.sp[j];
+                if (cnt < a.nbr_cap) {
+                    if (kBuf) buf[cnt * blockDim.x + threadIdx.x] = (uint16_t)j;
+                    else dst[(size_t)cnt * a.n] = L.sp[j];
+                }
                 ++cnt;
             }
             ++j;
             if (j == b0) j = a1;
             if (j == b1) j = a2;
         }
-        a.nnum[id] = min(cnt, a.nbr_cap);
+        const int m = min(cnt, a.nbr_cap);
+        if (kBuf) {
+            // column by column: the lanes of a wavefront hold consecutive
+            // sorted positions, so each store is contiguous
+            for (int q = 0; q < m; ++q)
+                dst[(size_t)q * a.n] = L.sp[buf[q * blockDim.x + threadIdx.x]];
+        }
+        a.nnum[p] = m;
         if (cnt > a.nbr_cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
 }
@@ -727,27 +785,35 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     __shared__ int s_cstart[kMaxSlots];
     __shared__ int s_off[kMaxSlots + 1];
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
-    const int tile = xcd_remap(blockIdx.x, a.ntiles);
-    const Tile T = a.tiles[tile];
-    if (a.ctl && !a.ctl[T.rep].flag) return;
-    const RepDesc R = a.reps[T.rep];
-    const int W = T.w_flags & 0xffff;
     PairLds L;
     L.ent = dyn;
     L.typ = nullptr;
-    L.sp = reinterpret_cast<int *>(dyn + a.lds_cap);
+    L.sp = reinterpret_cast<int *>(dyn + a.lds_cap);  // staged sorted positions
     L.cstart = s_cstart;
     L.off = s_off;
     L.pt = s_pt;
     L.red = nullptr;
-    if (stage_tile<false, false, true>(a, T, R, W, L, nullptr, nullptr) < 0) {
-        if (threadIdx.x == 0) atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
-        return;
+    uint16_t *buf = reinterpret_cast<uint16_t *>(L.sp + a.lds_cap);
+    for (int b = blockIdx.x; b < a.ntiles; b += gridDim.x) {
+        const Tile T = a.tiles[xcd_remap(b, a.ntiles)];
+        if (a.ctl && !a.ctl[T.rep].flag) continue;  // uniform per workgroup
+        const RepDesc R = a.reps[T.rep];
+        const int W = T.w_flags & 0xffff;
+        if (stage_tile<false, true>(a, T, R, W, L, nullptr, nullptr) < 0) {
+            if (threadIdx.x == 0) atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+            __syncthreads();
+            continue;
+        }
+        const bool wrapT = (T.w_flags >> 16) & 1;
+        if (a.lbuf) {
+            if (wrapT) list_tile_body<true, kSigma, true>(a, R, W, L, buf);
+            else list_tile_body<false, kSigma, true>(a, R, W, L, buf);
+        } else {
+            if (wrapT) list_tile_body<true, kSigma, false>(a, R, W, L, buf);
+            else list_tile_body<false, kSigma, false>(a, R, W, L, buf);
+        }
+        __syncthreads();  // the next tile restages the LDS
     }
-    if ((T.w_flags >> 16) & 1)
-        list_tile_body<true, kSigma>(a, R, W, L);
-    else
-        list_tile_body<false, kSigma>(a, R, W, L);
 }
 
 // LJ over the neighbour lists: one lane per particle of a kick span, the
@@ -762,17 +828,20 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
     double Ublk = 0.0, Wblk = 0.0;
-    for (int i = sp.begin + (int)threadIdx.x; i < sp.end; i += blockDim.x) {
-        const float xi = a.x[i], yi = a.y[i];
+    // lanes walk sorted positions p (the list build's order): list slots
+    // nbr[k * n + p] are read coalesced, i = perm[p] is the particle
+    for (int p = sp.begin + (int)threadIdx.x; p < sp.end; p += blockDim.x) {
+        const float2 pi = a.xys[p];
+        const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
-        const float si = kSigma ? a.sig[i] : 0.0f;
-        const int n = a.nnum[i];
-        const int32_t *lst = a.nbr + i;
+        const float si = kSigma ? a.sigs[p] : 0.0f;
+        const int n = a.nnum[p];
+        const int32_t *lst = a.nbr + p;
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
-        for (int k = 0; k < n; ++k) {
-            const int j = lst[(size_t)k * a.n];
-            const float xj = a.x[j], yj = a.y[j];
+        auto term = [&](int q) {
+            const float2 pj = a.xys[q];
+            const float xj = pj.x, yj = pj.y;
             float dx = xi - xj;
             float dy = yi - yj;
             int sx = 0, sy = 0;
@@ -782,7 +851,7 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             float s2ij;
             unsigned rc2m1;
             if (kSigma) {
-                const float sij = (si + a.sig[j]) * 0.5f;
+                const float sij = (si + a.sigs[q]) * 0.5f;
                 rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
                 s2ij = sij * sij;
             } else {
@@ -809,14 +878,14 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 accB += sr6;
                 ++nint;
             }
-        }
+        };
+        for (int k = 0; k < n; ++k) term(lst[(size_t)k * a.n]);
         fx *= pe.e48;
         fy *= pe.e48;
         const double A = accA, B = accB;
         const double Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
         const double Wi = (double)pe.e48 * (A - 0.5 * B);
-        a.fx[i] = fx;
-        a.fy[i] = fy;
+        a.fs[p] = make_float2(fx, fy);
         Ublk += 0.5 * Ui;
         Wblk += 0.5 * Wi;
     }
@@ -1003,10 +1072,20 @@ __global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
                                                      const float2 *cf,
                                                      const int32_t *goff,
                                                      const int32_t *gent,
+                                                     const int32_t *spos,
+                                                     const float2 *fs,
                                                      float *ox, float *oy) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    float x = fx[p], y = fy[p];
+    float x, y;
+    if (spos) {  // list mode: pair forces are held in sorted order
+        const float2 f = fs[spos[p]];
+        x = f.x;
+        y = f.y;
+    } else {
+        x = fx[p];
+        y = fy[p];
+    }
     if (goff) {
         for (int e = goff[p]; e < goff[p + 1]; ++e) {
             const float2 c = cf[gent[e]];
@@ -1069,14 +1148,11 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
     }
 }
 
-// Start a new list epoch (after a forced rebuild): running bound, step max
-// and arrival counter cleared; the forced rebuild is counted.
+// Start a new list epoch (after a forced rebuild), which is counted.
 __global__ void k_ctl_reset(Ctl *ctl, int nrep) {
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
         ctl[r].flag = 0;
         ctl[r].disp = 0.0f;
-        ctl[r].dmax = 0u;
-        ctl[r].done = 0u;
         ctl[r].nbuild += 1u;
     }
 }
@@ -1462,25 +1538,29 @@ hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
-    // 4 = kinetic-energy partials only; a.ctl set = list mode (the drift
-    // decides the rebuild instead of binning)
+    // 4 = kinetic-energy partials only; a.dpart set = list mode (the drift
+    // measures the displacement since the list build instead of binning)
     if (nblk <= 0) return hipSuccess;
     dim3 g(nblk), b(kKickThreads);
-    const bool list = a.ctl != nullptr;
+    const bool list = a.dpart != nullptr;
 #define FM_KICK(K2, D, G) \
     do { \
         if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true>), g, b, 0, s, a); \
         else hipLaunchKernelGGL((k_kick<K2, D, G, false>), g, b, 0, s, a); \
     } while (0)
     switch (mode) {
-        case 0: hipLaunchKernelGGL(k_bin, g, b, 0, s, a); break;
+        case 0:
+            // gated list-mode binning: a smaller grid-stride grid keeps the
+            // skipped steps cheap
+            if (a.ctl) g = dim3(std::min(nblk, kGatedGrid));
+            hipLaunchKernelGGL(k_bin, g, b, 0, s, a);
+            break;
         case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
         case 1:
             if (gather) FM_KICK(false, true, true); else FM_KICK(false, true, false);
             break;
         case 2:
-            if (gather) hipLaunchKernelGGL((k_kick<true, false, true, false>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_kick<true, false, false, false>), g, b, 0, s, a);
+            if (gather) FM_KICK(true, false, true); else FM_KICK(true, false, false);
             break;
         default:
             if (gather) FM_KICK(true, true, true); else FM_KICK(true, true, false);
@@ -1497,8 +1577,8 @@ hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s
     } else if (part == 1) {
         hipLaunchKernelGGL(k_scan_apply, dim3(nblk), dim3(kScanThreads), 0, s, a);
     } else {
-        if (a.nspans > 0)
-            hipLaunchKernelGGL(k_scatter, dim3(a.nspans), dim3(kScanThreads), 0, s, a);
+        const int g = a.ctl ? std::min(a.nspans, kGatedGrid) : a.nspans;
+        if (g > 0) hipLaunchKernelGGL(k_scatter, dim3(g), dim3(kScanThreads), 0, s, a);
     }
     return hipGetLastError();
 }
@@ -1544,23 +1624,30 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int bloc
     return hipGetLastError();
 }
 
+size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered) {
+    return (size_t)lds_cap * (sizeof(float4) + sizeof(int)) +
+           (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) : 0);
+}
+
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.lds_cap * (sizeof(float4) + sizeof(int));
+    const size_t lds = list_build_lds_bytes(a.lds_cap, a.nbr_cap, block, a.lbuf != 0);
+    const dim3 g(a.ctl ? std::min(a.ntiles, kGatedGrid) : a.ntiles);
     if (sigma)
-        hipLaunchKernelGGL((k_list_build<true>), dim3(a.ntiles), dim3(block), lds, s, a);
+        hipLaunchKernelGGL((k_list_build<true>), g, dim3(block), lds, s, a);
     else
-        hipLaunchKernelGGL((k_list_build<false>), dim3(a.ntiles), dim3(block), lds, s, a);
+        hipLaunchKernelGGL((k_list_build<false>), g, dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_total_force(int n, const float *fx, const float *fy,
                               const float2 *cf, const int32_t *goff,
-                              const int32_t *gent, float *ox, float *oy,
+                              const int32_t *gent, const int32_t *spos,
+                              const float2 *fs, float *ox, float *oy,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_total_force, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                       fx, fy, cf, goff, gent, ox, oy);
+                       fx, fy, cf, goff, gent, spos, fs, ox, oy);
     return hipGetLastError();
 }
 

@@ -180,24 +180,44 @@ def test_adjoint_rollout_matches_fd():
     assert s.device_errors() == 0
 
 
-def test_adjoint_forward_state_and_checkpoint_invariance():
+@pytest.mark.parametrize("skin", [0.4, 0.0])
+def test_adjoint_forward_state_and_checkpoint_invariance(skin):
+    """Stored-trajectory adjoint: its forward part is the plain step (bitwise)
+    and the state afterwards steps on unchanged.  Checkpointed adjoint: the
+    cell walk recomputes each segment bit for bit, so the gradient is the
+    same; with neighbour lists every checkpoint starts a fresh list (so the
+    recomputation matches its forward segment), which changes the summation
+    order against the stored-trajectory run: the gradient agrees to FP32
+    rounding."""
     w = configs.c5(scale=0.01)
-    sa = configs.build_system(w)
-    sb = configs.build_system(w)
-    sc = configs.build_system(w)
+    sa = configs.build_system(w, skin=skin)
+    sb = configs.build_system(w, skin=skin)
+    sc = configs.build_system(w, skin=skin)
     x0, y0, _, _ = sa.get_state()
     ids = _loss_setup(w)
     tgt = np.stack([x0[ids] + 0.2, y0[ids]], 1)
-    ga = sa.adjoint(60, ids, tgt, ckpt=7)
-    gb = sb.adjoint(60, ids, tgt, ckpt=0)  # whole trajectory stored
+    ga = sa.adjoint(60, ids, tgt, ckpt=0)  # whole trajectory stored
+    gb = sb.adjoint(60, ids, tgt, ckpt=7)
     sc.run(60)
     for a, c in zip(sa.get_state(), sc.get_state()):
         np.testing.assert_array_equal(a, c)
-    np.testing.assert_array_equal(ga["dE"], gb["dE"])
-    assert ga["deps"] == gb["deps"] and ga["dsigma"] == gb["dsigma"]
-    assert ga["loss"] == gb["loss"]
-    # the engine keeps stepping normally after an adjoint sweep
+    if skin == 0.0:
+        np.testing.assert_array_equal(ga["dE"], gb["dE"])
+        assert ga["deps"] == gb["deps"] and ga["dsigma"] == gb["dsigma"]
+        assert ga["loss"] == gb["loss"]
+    else:
+        scale = np.abs(ga["dE"]).max()
+        assert np.abs(ga["dE"] - gb["dE"]).max() <= 1e-4 * scale
+        assert abs(ga["deps"] - gb["deps"]) <= 1e-4 * abs(ga["deps"]) + 1e-12
+        assert abs(ga["dsigma"] - gb["dsigma"]) <= 1e-4 * abs(ga["dsigma"]) + 1e-12
+        assert abs(ga["loss"] - gb["loss"]) <= 1e-6 * ga["loss"]
+    # the engine keeps stepping normally after an adjoint sweep (with lists
+    # it restarts from a fresh list: same interacting sets, another
+    # summation order)
     sa.run(40)
     sc.run(40)
     for a, c in zip(sa.get_state(), sc.get_state()):
-        np.testing.assert_array_equal(a, c)
+        if skin == 0.0:
+            np.testing.assert_array_equal(a, c)
+        else:
+            np.testing.assert_allclose(a, c, rtol=0, atol=1e-3)
