@@ -26,9 +26,9 @@
 
 namespace fm {
 
-// Grid of the gated list-mode kernels (binning, scatter, list build): 4
+// Grid of the gated list-mode kernels (binning, scatter, list build): 2
 // workgroups per CU loop over the work, so a skipped step costs little.
-constexpr int kGatedGrid = 1024;
+constexpr int kGatedGrid = 512;
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -716,7 +716,11 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    const int bd = blockDim.x;  // held in a register across the loops
+    const int tid = threadIdx.x;
+    const size_t N = a.n;
+    const int cap = a.nbr_cap;
+    for (int k = tid; k < nc; k += bd) {
         const int me = c_lo + k;
         int lo = row + 1, hi = row + W + 1;
         while (hi - lo > 1) {
@@ -745,8 +749,13 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
             float dx = xi - pj.x;
             float dy = yi - pj.y;
             if (kWrap) {
-                if (dx >= hLx) dx -= Lx; else if (dx < -hLx) dx += Lx;
-                if (dy >= hLy) dy -= Ly; else if (dy < -hLy) dy += Ly;
+                // minimum image as two selects per axis (no branches)
+                float ox = (dx >= hLx) ? Lx : 0.0f;
+                ox = (dx < -hLx) ? -Lx : ox;
+                float oy = (dy >= hLy) ? Ly : 0.0f;
+                oy = (dy < -hLy) ? -Ly : oy;
+                dx = dx - ox;
+                dy = dy - oy;
             }
             const float dr2 = dx * dx + dy * dy;
             float rl2;
@@ -757,9 +766,9 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 rl2 = R.rl2_uniform;
             }
             if ((dr2 < rl2) & (gj != gix) & (j != me)) {
-                if (cnt < a.nbr_cap) {
-                    if (kBuf) buf[cnt * blockDim.x + threadIdx.x] = (uint16_t)j;
-                    else dst[(size_t)cnt * a.n] = L.sp[j];
+                if (cnt < cap) {
+                    if (kBuf) buf[cnt * bd + tid] = (uint16_t)j;
+                    else dst[(size_t)cnt * N] = L.sp[j];
                 }
                 ++cnt;
             }
@@ -767,15 +776,17 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
             if (j == b0) j = a1;
             if (j == b1) j = a2;
         }
-        const int m = min(cnt, a.nbr_cap);
+        const int m = min(cnt, cap);
         if (kBuf) {
             // column by column: the lanes of a wavefront hold consecutive
             // sorted positions, so each store is contiguous
-            for (int q = 0; q < m; ++q)
-                dst[(size_t)q * a.n] = L.sp[buf[q * blockDim.x + threadIdx.x]];
+            for (int q = 0; q < m; ++q) dst[(size_t)q * N] = L.sp[buf[q * bd + tid]];
         }
+        // pad to a multiple of 4 with the particle itself (distance 0: the
+        // pair pass's cutoff test rejects it); cap is a multiple of 8
+        for (int q = m; q < ((m + 3) & ~3); ++q) dst[(size_t)q * N] = p;
         a.nnum[p] = m;
-        if (cnt > a.nbr_cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
+        if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
 }
 
@@ -827,26 +838,29 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     const PairEntry pe = a.ptab[R.pt];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    const size_t N = a.n;
     double Ublk = 0.0, Wblk = 0.0;
     // lanes walk sorted positions p (the list build's order): list slots
-    // nbr[k * n + p] are read coalesced, i = perm[p] is the particle
+    // nbr[k * n + p] are read coalesced
     for (int p = sp.begin + (int)threadIdx.x; p < sp.end; p += blockDim.x) {
         const float2 pi = a.xys[p];
         const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? a.sigs[p] : 0.0f;
-        const int n = a.nnum[p];
+        const int n4 = (a.nnum[p] + 3) >> 2;  // padded with p itself
         const int32_t *lst = a.nbr + p;
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
-        auto term = [&](int q) {
-            const float2 pj = a.xys[q];
+        auto term = [&](float2 pj, int q) {
             const float xj = pj.x, yj = pj.y;
-            float dx = xi - xj;
-            float dy = yi - yj;
-            int sx = 0, sy = 0;
-            if (dx >= hLx) { dx -= Lx; sx = 1; } else if (dx < -hLx) { dx += Lx; sx = -1; }
-            if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
+            const float ex = xi - xj, ey = yi - yj;
+            // minimum image as two selects per axis (no branches)
+            float ox = (ex >= hLx) ? Lx : 0.0f;
+            ox = (ex < -hLx) ? -Lx : ox;
+            float oy = (ey >= hLy) ? Ly : 0.0f;
+            oy = (ey < -hLy) ? -Ly : oy;
+            const float dx = ex - ox;
+            const float dy = ey - oy;
             const float dr2 = dx * dx + dy * dy;
             float s2ij;
             unsigned rc2m1;
@@ -860,9 +874,10 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             }
             if ((__float_as_uint(dr2) - 1u) < rc2m1) {
                 float dxa = dx, dya = dy, r2a = dr2;
-                if (sx | sy) {
-                    if (sx > 0) dxa = xim - xj; else if (sx < 0) dxa = xi - (xj - Lx);
-                    if (sy > 0) dya = yim - yj; else if (sy < 0) dya = yi - (yj - Ly);
+                if ((ox != 0.0f) | (oy != 0.0f)) {
+                    // image-exact difference: shift the coordinate first
+                    if (ox > 0.0f) dxa = xim - xj; else if (ox < 0.0f) dxa = xi - (xj - Lx);
+                    if (oy > 0.0f) dya = yim - yj; else if (oy < 0.0f) dya = yi - (yj - Ly);
                     r2a = dxa * dxa + dya * dya;
                 }
                 const float inv = __builtin_amdgcn_rcpf(r2a);
@@ -879,7 +894,16 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 ++nint;
             }
         };
-        for (int k = 0; k < n; ++k) term(lst[(size_t)k * a.n]);
+        for (int k = 0; k < n4; ++k) {
+            // four independent index loads, then four gathers in flight
+            const int32_t *l4 = lst + (size_t)(4 * k) * N;
+            const int q0 = l4[0], q1 = l4[N], q2 = l4[2 * N], q3 = l4[3 * N];
+            const float2 r0 = a.xys[q0], r1 = a.xys[q1], r2 = a.xys[q2], r3 = a.xys[q3];
+            term(r0, q0);
+            term(r1, q1);
+            term(r2, q2);
+            term(r3, q3);
+        }
         fx *= pe.e48;
         fy *= pe.e48;
         const double A = accA, B = accB;
