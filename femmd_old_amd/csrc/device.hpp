@@ -80,9 +80,10 @@ struct KickArgs {
     int32_t cap;
     double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
     uint32_t *err;
-    const Ctl *ctl;            // list mode: binning only for flagged replicas
+    Ctl *ctl;                  // list mode (gated binning): rebuild decision
+    int32_t *any;              // list mode: set when some replica rebuilds
     float *xb, *yb;            // list mode: positions at the last list build
-    float *dpart;              // list mode: per workgroup max |x - xb|^2
+    float *dpart;              // list mode: per kick workgroup max |x - xb|^2
     const float2 *fs;          // list mode: pair forces in sorted order
     const int32_t *spos;       // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order (out)

@@ -578,8 +578,12 @@ int Engine::launch_bin(bool gated) {
     a.tmp = tmp_.p;
     a.cap = cap_;
     a.err = err_.p;
-    a.ctl = gated ? ctl_.p : nullptr;
     a.nspans = (int)kspans_h_.size();
+    if (gated) {  // list mode: decide and bin only the replicas that rebuild
+        a.ctl = ctl_.p;
+        a.any = any_.p;
+        a.dpart = dpart_.p;
+    }
     if (list_) {
         a.xb = xb_.p;
         a.yb = yb_.p;
@@ -651,7 +655,6 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     if (!list_) return launch_forces_only(false);
     // list mode: binning, sort and list build run only for the replicas
     // where some particle moved more than skin / 2 since the last build
-    if (launch_list_decide(R_, reps_.p, dpart_.p, ctl_.p, any_.p, stream_) != hipSuccess) return -1;
     if (launch_bin(true)) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
     if (launch_sort(true)) return -1;

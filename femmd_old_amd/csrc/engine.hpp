@@ -217,8 +217,6 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int bloc
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
 size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered);
 hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
-hipError_t launch_list_decide(int nrep, const RepDesc *reps, const float *dpart, Ctl *ctl,
-                              int32_t *any, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
                               const int32_t *goff, const int32_t *gent, const int32_t *spos,

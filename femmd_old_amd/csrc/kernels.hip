@@ -26,9 +26,11 @@
 
 namespace fm {
 
-// Grid of the gated list-mode kernels (binning, scatter, list build): 2
-// workgroups per CU loop over the work, so a skipped step costs little.
-constexpr int kGatedGrid = 512;
+// Grids of the gated list-mode kernels: a few workgroups per CU loop over
+// the work, so a skipped step costs little (the binning workgroups also
+// each make the rebuild decision, so that grid is the smallest).
+constexpr int kGatedBinGrid = 1024;
+constexpr int kGatedGrid = 2048;
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -275,11 +277,49 @@ __global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
 
 // Binning of the current positions without moving them (used after
 // fm_set_state / fm_commit and by fm_compute_forces).
+// List mode rebuild decision for replica R, made by every workgroup that
+// bins one of its spans (all compute the same value from the kick's
+// per-workgroup partials); the workgroup holding the replica's first span
+// records it.  The largest displacement since the last build against
+// skin / 2 is the reference's criterion (femmd2D/src/nn.cpp:114-134) with the
+// exact displacement in place of its running bound sum sqrt(max v^2) dt.
+// All threads of the workgroup must call this.
+__device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R, float *red) {
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < R.nks; k += blockDim.x) m = fmaxf(m, a.dpart[R.ks0 + k]);
+    m = wave_max(m);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    float mm = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) mm = fmaxf(mm, red[k]);
+    const float d = sqrtf(mm) * 1.0001f + R.dslack;
+    const int flag = d > 0.5f * R.skin;
+    if (threadIdx.x == 0 && sp.begin == R.p0) {
+        Ctl *c = a.ctl + sp.rep;
+        c->flag = flag;
+        c->disp = d;
+        if (flag) {
+            c->nbuild += 1u;
+            *a.any = 1;  // every writer stores the same value
+        }
+    }
+    return flag;
+}
+
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
+  __shared__ float red[kKickThreads / 64];
+  int last_rep = -1, flag = 1;
   for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
     const Span sp = a.spans[b];
-    if (a.ctl && !a.ctl[sp.rep].flag) continue;  // list mode, no rebuild
     const RepDesc R = a.reps[sp.rep];
+    if (a.ctl) {  // list mode: bin only the replicas that rebuild this step
+        if (sp.rep != last_rep) {  // uniform per workgroup
+            flag = list_decide(a, sp, R, red);
+            last_rep = sp.rep;
+        }
+        if (!flag) continue;
+    }
     const int p = sp.begin + (int)threadIdx.x;
     const bool active = p < sp.end;
     int c = 0;
@@ -300,39 +340,6 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
   }
 }
 
-// List mode rebuild decision, one workgroup per replica: the largest
-// displacement since the last build against skin / 2 (the reference's
-// criterion, femmd2D/src/nn.cpp:114-134, with the exact displacement in
-// place of its running bound sum of sqrt(max v^2) dt).
-__global__ __launch_bounds__(256) void k_list_decide(const RepDesc *reps, const float *dpart,
-                                                     Ctl *ctl, int32_t *any) {
-    __shared__ float red[4];
-    const RepDesc R = reps[blockIdx.x];
-    float m = 0.0f;
-    for (int k = threadIdx.x; k < R.nks; k += blockDim.x) m = fmaxf(m, dpart[R.ks0 + k]);
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmaxf(m, red[k]);
-    m = fmaxf(m, red[0]);
-    const float d = sqrtf(m) * 1.0001f + R.dslack;
-    const int flag = d > 0.5f * R.skin;
-    Ctl *c = ctl + blockIdx.x;
-    c->flag = flag;
-    c->disp = d;
-    if (flag) {
-        c->nbuild += 1u;
-        *any = 1;  // every writer stores the same value
-    }
-}
-
-hipError_t launch_list_decide(int nrep, const RepDesc *reps, const float *dpart, Ctl *ctl,
-                              int32_t *any, hipStream_t s) {
-    if (nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_list_decide, dim3(nrep), dim3(256), 0, s, reps, dpart, ctl, any);
-    return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------
 // Counting sort over all cells of all replicas: per-workgroup cell totals,
@@ -507,8 +514,14 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
         int nint = 0;
         long long s1 = 0, s2 = 0;
         const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+        // the next candidate's LDS entry is read one iteration ahead
+        float4 pn = L.ent[total > 0 ? j : 0];
         for (int t = 0; t < total; ++t) {
-            const float4 pj = L.ent[j];
+            const float4 pj = pn;
+            int jn = j + 1;
+            if (jn == b0) jn = a1;
+            if (jn == b1) jn = a2;
+            pn = L.ent[(t + 1 < total) ? jn : j];
             const int gj = __float_as_int(pj.z);
             // femmd2D/src/interactionsCuda.cu:84-90, FP32, reference order
             float dx = xi - pj.x;
@@ -580,9 +593,7 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
                     s2 += v * v;
                 }
             }
-            ++j;
-            if (j == b0) j = a1;
-            if (j == b1) j = a2;
+            j = jn;
         }
         double Ui, Wi;
         if (!kTypes) {
@@ -743,8 +754,14 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int p = sbase + k;  // sorted position of this centre particle
         int32_t *dst = a.nbr + p;
         int cnt = 0;
+        // the next candidate's LDS entry is read one iteration ahead
+        float4 pn = L.ent[total > 0 ? j : 0];
         for (int t = 0; t < total; ++t) {
-            const float4 pj = L.ent[j];
+            const float4 pj = pn;
+            int jn = j + 1;
+            if (jn == b0) jn = a1;
+            if (jn == b1) jn = a2;
+            pn = L.ent[(t + 1 < total) ? jn : j];
             const int gj = __float_as_int(pj.z);
             float dx = xi - pj.x;
             float dy = yi - pj.y;
@@ -772,9 +789,7 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 }
                 ++cnt;
             }
-            ++j;
-            if (j == b0) j = a1;
-            if (j == b1) j = a2;
+            j = jn;
         }
         const int m = min(cnt, cap);
         if (kBuf) {
@@ -1236,8 +1251,14 @@ __device__ void adj_pair_body(const AdjArgs &A, const RepDesc &R, int W, int til
         if (j == b0) j = a1;
         if (j == b1) j = a2;
         float jx = 0.f, jy = 0.f, fx = 0.f, fy = 0.f, gx = 0.f, gy = 0.f;
+        // the next candidate's LDS entry is read one iteration ahead
+        float4 pn = L.ent[total > 0 ? j : 0];
         for (int t = 0; t < total; ++t) {
-            const float4 pj = L.ent[j];
+            const float4 pj = pn;
+            int jn = j + 1;
+            if (jn == b0) jn = a1;
+            if (jn == b1) jn = a2;
+            pn = L.ent[(t + 1 < total) ? jn : j];
             const int gj = __float_as_int(pj.z);
             float dx = xi - pj.x;
             float dy = yi - pj.y;
@@ -1284,9 +1305,7 @@ __device__ void adj_pair_body(const AdjArgs &A, const RepDesc &R, int W, int til
                 gx = __builtin_fmaf(gs, dxa, gx);
                 gy = __builtin_fmaf(gs, dya, gy);
             }
-            ++j;
-            if (j == b0) j = a1;
-            if (j == b1) j = a2;
+            j = jn;
         }
         const int id = a.perm[sbase + k];
         A.jx[id] = pe.e48 * jx;
@@ -1576,7 +1595,7 @@ hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
         case 0:
             // gated list-mode binning: a smaller grid-stride grid keeps the
             // skipped steps cheap
-            if (a.ctl) g = dim3(std::min(nblk, kGatedGrid));
+            if (a.ctl) g = dim3(std::min(nblk, kGatedBinGrid));
             hipLaunchKernelGGL(k_bin, g, b, 0, s, a);
             break;
         case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
