@@ -42,6 +42,12 @@ struct alignas(16) RepDesc {
     float dslack;      // rounding allowance of the displacement test
 };
 
+// Displacement maxima of a kick: each workgroup takes an atomic max into one
+// of kDispSlots slots of its replica (spread to limit contention, one slot
+// per 128-byte line); the binning reads the slots, the force pass clears them.
+constexpr int kDispSlots = 16;
+constexpr int kDispStride = 32;
+
 // Per-replica control words of the neighbour-list mode.
 struct alignas(16) Ctl {
     int32_t flag;    // rebuild the list in this step
@@ -83,7 +89,9 @@ struct KickArgs {
     Ctl *ctl;                  // list mode (gated binning): rebuild decision
     int32_t *any;              // list mode: set when some replica rebuilds
     float *xb, *yb;            // list mode: positions at the last list build
-    float *dpart;              // list mode: per kick workgroup max |x - xb|^2
+    uint32_t *dmax;            // list mode: per replica kDispSlots slots of the
+                               // step's max |x - xb|^2 (float bits), kDispStride
+                               // words apart
     const float2 *fs;          // list mode: pair forces in sorted order
     const int32_t *spos;       // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order (out)
@@ -160,6 +168,8 @@ struct ForceArgs {
     const float *sigs;         // list mode: sigma in sorted order (sigma path)
     const Ctl *ctl;            // list build gating (null: always build)
     int32_t *any;              // reset by the force pass after the rebuild
+    uint32_t *dmax;            // displacement slots, cleared by the force pass
+    int32_t nrep;
     int32_t n;                 // particles (stride of nbr)
     // diagnostics (pair-set dump); null in production
     const int32_t *uid;        // internal -> caller id

@@ -409,7 +409,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         FM_AL(any_, 1);
         FM_AL(xb_, N);
         FM_AL(yb_, N);
-        FM_AL(dpart_, ks.size());
+        FM_AL(dmax_, (size_t)R * kDispSlots * kDispStride);
         FM_AL(spos_, N);
         FM_AL(xys_, N);
         FM_AL(fs_, N);
@@ -427,7 +427,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             (he = hipMemsetAsync(E->ctl_.p, 0, sizeof(Ctl) * R, st)) != hipSuccess ||
             (he = hipMemsetAsync(E->any_.p, 0, sizeof(int32_t), st)) != hipSuccess ||
             (he = hipMemsetAsync(E->spos_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess ||
-            (he = hipMemsetAsync(E->fs_.p, 0, sizeof(float2) * N, st)) != hipSuccess)
+            (he = hipMemsetAsync(E->fs_.p, 0, sizeof(float2) * N, st)) != hipSuccess ||
+            (he = hipMemsetAsync(E->dmax_.p, 0, E->dmax_.n * sizeof(uint32_t), st)) != hipSuccess)
             return E->check(he, "memset", err);
     }
     if ((he = hipMemsetAsync(E->err_.p, 0, sizeof(uint32_t), st)) != hipSuccess)
@@ -582,7 +583,7 @@ int Engine::launch_bin(bool gated) {
     if (gated) {  // list mode: decide and bin only the replicas that rebuild
         a.ctl = ctl_.p;
         a.any = any_.p;
-        a.dpart = dpart_.p;
+        a.dmax = dmax_.p;
     }
     if (list_) {
         a.xb = xb_.p;
@@ -641,7 +642,7 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     if (list_) {
         k.xb = xb_.p;
         k.yb = yb_.p;
-        k.dpart = dpart_.p;
+        k.dmax = dmax_.p;
         k.fs = fs_.p;
         k.spos = spos_.p;
         k.xys = xys_.p;
@@ -761,6 +762,8 @@ ForceArgs Engine::force_args() {
     f.fs = fs_.p;
     f.sigs = sigs_.p;
     f.any = list_ ? any_.p : nullptr;
+    f.dmax = dmax_.p;
+    f.nrep = R_;
     f.n = N_;
     f.uid = uid_.p;
     f.dbg_cnt = dbg_cnt_.p;

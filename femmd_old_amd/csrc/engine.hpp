@@ -173,7 +173,8 @@ private:
     DBuf<Span> kspans_, espans_, bspans_, lspans_;
     DBuf<int32_t> nbr_, nnum_, any_;
     DBuf<Ctl> ctl_;
-    DBuf<float> xb_, yb_, dpart_, sigs_;
+    DBuf<float> xb_, yb_, sigs_;
+    DBuf<uint32_t> dmax_;
     DBuf<int32_t> spos_;
     DBuf<float2> xys_, fs_;
     DBuf<Tile> tiles_;

@@ -224,14 +224,13 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     }
     if (kDrift && !kList) bin_wave(a, p < sp.end, p, cell);
     if (kDrift && kList) {
-        __shared__ float redd[kKickThreads / 64];
+        // one atomic max per wavefront into a slot of the replica (relaxed:
+        // the binning reads the slots after this kernel has completed)
         const float m = wave_max(d2);
-        if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float mm = 0.0f;
-            for (int k = 0; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redd[k]);
-            a.dpart[blockIdx.x] = mm;
+        if ((threadIdx.x & 63) == 0) {
+            const int slot = (blockIdx.x * (kKickThreads / 64) + (threadIdx.x >> 6)) % kDispSlots;
+            atomicMax(a.dmax + ((size_t)sp.rep * kDispSlots + slot) * kDispStride,
+                      __float_as_uint(m));
         }
     }
     if (kKick2) {
@@ -277,22 +276,17 @@ __global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
 
 // Binning of the current positions without moving them (used after
 // fm_set_state / fm_commit and by fm_compute_forces).
-// List mode rebuild decision for replica R, made by every workgroup that
-// bins one of its spans (all compute the same value from the kick's
-// per-workgroup partials); the workgroup holding the replica's first span
-// records it.  The largest displacement since the last build against
-// skin / 2 is the reference's criterion (femmd2D/src/nn.cpp:114-134) with the
-// exact displacement in place of its running bound sum sqrt(max v^2) dt.
-// All threads of the workgroup must call this.
-__device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R, float *red) {
-    float m = 0.0f;
-    for (int k = threadIdx.x; k < R.nks; k += blockDim.x) m = fmaxf(m, a.dpart[R.ks0 + k]);
-    m = wave_max(m);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    float mm = red[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) mm = fmaxf(mm, red[k]);
+// List mode rebuild decision for replica R, made by every wavefront that
+// bins one of its spans (all read the same kDispSlots maxima); the
+// workgroup holding the replica's first span records it.  The largest
+// displacement since the last build against skin / 2 is the reference's
+// criterion (femmd2D/src/nn.cpp:114-134) with the exact displacement in
+// place of its running bound sum sqrt(max v^2) dt.
+__device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t v =
+        lane < kDispSlots ? a.dmax[((size_t)sp.rep * kDispSlots + lane) * kDispStride] : 0u;
+    const float mm = wave_max(__uint_as_float(v));
     const float d = sqrtf(mm) * 1.0001f + R.dslack;
     const int flag = d > 0.5f * R.skin;
     if (threadIdx.x == 0 && sp.begin == R.p0) {
@@ -308,14 +302,13 @@ __device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R, 
 }
 
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
-  __shared__ float red[kKickThreads / 64];
   int last_rep = -1, flag = 1;
   for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
     const Span sp = a.spans[b];
     const RepDesc R = a.reps[sp.rep];
     if (a.ctl) {  // list mode: bin only the replicas that rebuild this step
         if (sp.rep != last_rep) {  // uniform per workgroup
-            flag = list_decide(a, sp, R, red);
+            flag = list_decide(a, sp, R);
             last_rep = sp.rep;
         }
         if (!flag) continue;
@@ -1074,8 +1067,13 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
     __shared__ double s_red[2 * 4];
     int b = blockIdx.x;
-    // the rebuild of this step (if any) is complete: clear its summary
-    if (kList && a.any && b == 0 && threadIdx.x == 0) *a.any = 0;
+    // the rebuild of this step (if any) is complete: clear its summary and
+    // the displacement slots for the next kick
+    if (kList && b == 0) {
+        if (a.any && threadIdx.x == 0) *a.any = 0;
+        for (int k = threadIdx.x; k < a.nrep * kDispSlots; k += blockDim.x)
+            a.dmax[(size_t)k * kDispStride] = 0u;
+    }
     if (kList) {
         if (b < a.nlspans) {
             list_pair_block<kSigma>(a, b, s_red);
@@ -1581,11 +1579,11 @@ hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
-    // 4 = kinetic-energy partials only; a.dpart set = list mode (the drift
+    // 4 = kinetic-energy partials only; a.dmax set = list mode (the drift
     // measures the displacement since the list build instead of binning)
     if (nblk <= 0) return hipSuccess;
     dim3 g(nblk), b(kKickThreads);
-    const bool list = a.dpart != nullptr;
+    const bool list = a.dmax != nullptr;
 #define FM_KICK(K2, D, G) \
     do { \
         if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true>), g, b, 0, s, a); \
