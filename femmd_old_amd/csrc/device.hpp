@@ -40,6 +40,9 @@ struct alignas(16) RepDesc {
     float rl2_uniform; // list radius^2, (rc*s + skin)^2 with a rounding margin
     int32_t ks0, nks;  // this replica's kick spans [ks0, ks0 + nks)
     float dslack;      // rounding allowance of the displacement test
+    float rl_max;      // list mode: farthest reach of a list entry,
+                       // rc*sigma_max + 2 skin (only a particle that close
+                       // to an edge has pairs needing the minimum image)
 };
 
 // Displacement maxima of a kick: each workgroup takes an atomic max into one

@@ -21,6 +21,7 @@
 // cell key, minimum image, cutoff predicate) is evaluated in its source
 // order; fused multiply-adds are written explicitly where wanted.
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 
@@ -737,52 +738,49 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int gi = __float_as_int(pi.z);
         const int gix = gi >= 0 ? gi : -2;
         const float si = pi.w;
-        const int a0 = L.off[col - 1], b0 = L.off[col + 2];
-        const int a1 = L.off[row + col - 1], b1 = L.off[row + col + 2];
-        const int a2 = L.off[2 * row + col - 1], b2 = L.off[2 * row + col + 2];
-        const int total = (b0 - a0) + (b1 - a1) + (b2 - a2);
-        int j = a0;
-        if (j == b0) j = a1;
-        if (j == b1) j = a2;
         const int p = sbase + k;  // sorted position of this centre particle
         int32_t *dst = a.nbr + p;
         int cnt = 0;
-        // the next candidate's LDS entry is read one iteration ahead
-        float4 pn = L.ent[total > 0 ? j : 0];
-        for (int t = 0; t < total; ++t) {
-            const float4 pj = pn;
-            int jn = j + 1;
-            if (jn == b0) jn = a1;
-            if (jn == b1) jn = a2;
-            pn = L.ent[(t + 1 < total) ? jn : j];
-            const int gj = __float_as_int(pj.z);
-            float dx = xi - pj.x;
-            float dy = yi - pj.y;
-            if (kWrap) {
-                // minimum image as two selects per axis (no branches)
-                float ox = (dx >= hLx) ? Lx : 0.0f;
-                ox = (dx < -hLx) ? -Lx : ox;
-                float oy = (dy >= hLy) ? Ly : 0.0f;
-                oy = (dy < -hLy) ? -Ly : oy;
-                dx = dx - ox;
-                dy = dy - oy;
-            }
-            const float dr2 = dx * dx + dy * dy;
-            float rl2;
-            if (kSigma) {
-                const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
-                rl2 = (rl * rl) * 1.00002f;
-            } else {
-                rl2 = R.rl2_uniform;
-            }
-            if ((dr2 < rl2) & (gj != gix) & (j != me)) {
-                if (cnt < cap) {
-                    if (kBuf) buf[cnt * bd + tid] = (uint16_t)j;
-                    else dst[(size_t)cnt * N] = L.sp[j];
+        uint16_t *bp = buf + tid;  // this lane's next LDS list slot
+        // the three staged rows, each a contiguous run of candidates
+        // (cells col-1 .. col+1); the centre particle is in row 1 only
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int j0 = L.off[r * row + col - 1], j1 = L.off[r * row + col + 2];
+            float4 pn = L.ent[j0 < j1 ? j0 : me];
+            for (int j = j0; j < j1; ++j) {
+                const float4 pj = pn;
+                pn = L.ent[j + 1 < j1 ? j + 1 : j];  // read one iteration ahead
+                float dx = xi - pj.x;
+                float dy = yi - pj.y;
+                if (kWrap) {
+                    // minimum image as two selects per axis (no branches)
+                    float ox = (dx >= hLx) ? Lx : 0.0f;
+                    ox = (dx < -hLx) ? -Lx : ox;
+                    float oy = (dy >= hLy) ? Ly : 0.0f;
+                    oy = (dy < -hLy) ? -Ly : oy;
+                    dx = dx - ox;
+                    dy = dy - oy;
                 }
-                ++cnt;
+                const float dr2 = dx * dx + dy * dy;
+                float rl2;
+                if (kSigma) {
+                    const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
+                    rl2 = (rl * rl) * 1.00002f;
+                } else {
+                    rl2 = R.rl2_uniform;
+                }
+                bool hit = (dr2 < rl2) & (__float_as_int(pj.z) != gix);
+                if (r == 1) hit &= (j != me);
+                if (hit) {
+                    if (cnt < cap) {
+                        if (kBuf) *bp = (uint16_t)j;
+                        else dst[(size_t)cnt * N] = L.sp[j];
+                    }
+                    bp += bd;
+                    ++cnt;
+                }
             }
-            j = jn;
         }
         const int m = min(cnt, cap);
         if (kBuf) {
@@ -859,14 +857,17 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
         const int32_t *lst = a.nbr + p;
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
-        auto term = [&](float2 pj, int q) {
+        auto term = [&](float2 pj, int q, auto wrap) {
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
-            // minimum image as two selects per axis (no branches)
-            float ox = (ex >= hLx) ? Lx : 0.0f;
-            ox = (ex < -hLx) ? -Lx : ox;
-            float oy = (ey >= hLy) ? Ly : 0.0f;
-            oy = (ey < -hLy) ? -Ly : oy;
+            float ox = 0.0f, oy = 0.0f;
+            if (decltype(wrap)::value) {
+                // minimum image as two selects per axis (no branches)
+                ox = (ex >= hLx) ? Lx : 0.0f;
+                ox = (ex < -hLx) ? -Lx : ox;
+                oy = (ey >= hLy) ? Ly : 0.0f;
+                oy = (ey < -hLy) ? -Ly : oy;
+            }
             const float dx = ex - ox;
             const float dy = ey - oy;
             const float dr2 = dx * dx + dy * dy;
@@ -882,7 +883,7 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             }
             if ((__float_as_uint(dr2) - 1u) < rc2m1) {
                 float dxa = dx, dya = dy, r2a = dr2;
-                if ((ox != 0.0f) | (oy != 0.0f)) {
+                if (decltype(wrap)::value && ((ox != 0.0f) | (oy != 0.0f))) {
                     // image-exact difference: shift the coordinate first
                     if (ox > 0.0f) dxa = xim - xj; else if (ox < 0.0f) dxa = xi - (xj - Lx);
                     if (oy > 0.0f) dya = yim - yj; else if (oy < 0.0f) dya = yi - (yj - Ly);
@@ -902,16 +903,27 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 ++nint;
             }
         };
-        for (int k = 0; k < n4; ++k) {
-            // four independent index loads, then four gathers in flight
-            const int32_t *l4 = lst + (size_t)(4 * k) * N;
-            const int q0 = l4[0], q1 = l4[N], q2 = l4[2 * N], q3 = l4[3 * N];
-            const float2 r0 = a.xys[q0], r1 = a.xys[q1], r2 = a.xys[q2], r3 = a.xys[q3];
-            term(r0, q0);
-            term(r1, q1);
-            term(r2, q2);
-            term(r3, q3);
-        }
+        auto walk = [&](auto wrap) {
+            for (int k = 0; k < n4; ++k) {
+                // four independent index loads, then four gathers in flight
+                const int32_t *l4 = lst + (size_t)(4 * k) * N;
+                const int q0 = l4[0], q1 = l4[N], q2 = l4[2 * N], q3 = l4[3 * N];
+                const float2 r0 = a.xys[q0], r1 = a.xys[q1], r2 = a.xys[q2], r3 = a.xys[q3];
+                term(r0, q0, wrap);
+                term(r1, q1, wrap);
+                term(r2, q2, wrap);
+                term(r3, q3, wrap);
+            }
+        };
+        // neighbours lie within rl of the particle (up to skin / 2 more for
+        // the motion since the build): away from the box edges no pair
+        // needs the minimum image; the choice is uniform per wavefront
+        const float m = R.rl_max;
+        const bool edge = (xi < m) | (xi > Lx - m) | (yi < m) | (yi > Ly - m);
+        if (__any(edge))
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
         fx *= pe.e48;
         fy *= pe.e48;
         const double A = accA, B = accB;
