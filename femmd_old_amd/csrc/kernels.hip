@@ -845,6 +845,7 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
     const size_t N = a.n;
+    const int cap = a.nbr_cap;
     double Ublk = 0.0, Wblk = 0.0;
     // lanes walk sorted positions p (the list build's order): list slots
     // nbr[k * n + p] are read coalesced
@@ -904,15 +905,24 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             }
         };
         auto walk = [&](auto wrap) {
+            // groups of four list slots; the next group's indices are loaded
+            // while this group's positions are gathered and evaluated
+            // (loads are unconditional so that the compiler can count them;
+            // the row is clamped inside the list array, and indices read past
+            // a particle's list are never used)
+            int q0 = lst[0], q1 = lst[N], q2 = lst[2 * N], q3 = lst[3 * N];
             for (int k = 0; k < n4; ++k) {
-                // four independent index loads, then four gathers in flight
-                const int32_t *l4 = lst + (size_t)(4 * k) * N;
-                const int q0 = l4[0], q1 = l4[N], q2 = l4[2 * N], q3 = l4[3 * N];
                 const float2 r0 = a.xys[q0], r1 = a.xys[q1], r2 = a.xys[q2], r3 = a.xys[q3];
+                const int32_t *ln = lst + (size_t)min(4 * k + 4, cap - 4) * N;
+                const int n0 = ln[0], n1 = ln[N], n2 = ln[2 * N], n3 = ln[3 * N];
                 term(r0, q0, wrap);
                 term(r1, q1, wrap);
                 term(r2, q2, wrap);
                 term(r3, q3, wrap);
+                q0 = n0;
+                q1 = n1;
+                q2 = n2;
+                q3 = n3;
             }
         };
         // neighbours lie within rl of the particle (up to skin / 2 more for
