@@ -227,7 +227,8 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         return 0;
     };
     // ---- forward, checkpoint at every segment start and at the end; the
-    // last segment records its positions as it goes ----
+    // last segment records its positions as it goes (no energies needed) ----
+    energy_ = false;
     for (int s = 0; s < S; ++s) {
         if (list_ && S > 1) {
             // a fresh neighbour list at every checkpoint, as the recomputation
@@ -247,6 +248,7 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         }
         if (launch_step_body(true, false)) return check(hipGetLastError(), "kick", err);
     }
+    energy_ = true;
     steps_ += nsteps;
     float *final_state = ckpt_.p + (size_t)S * 4 * N;
     if (save_state(final_state)) return check(hipGetLastError(), "ckpt", err);

@@ -318,6 +318,9 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     }
     E->N_ = p0;
     E->ncells_ = cell0;
+    // the list pass addresses the lists with 32-bit byte offsets: larger
+    // systems walk the cells every step (cells of side >= rc*sigma + skin)
+    if (E->list_ && (double)E->nbr_cap_ * p0 * 4.0 >= 2147483648.0) E->list_ = false;
     E->ntri_ = (int)ht0.size();
     E->nbond_ = (int)hbi.size();
     E->gather_ = (E->ntri_ + E->nbond_) > 0;
@@ -704,7 +707,8 @@ int Engine::launch_list_build(bool gated) {
 
 int Engine::launch_pair_pass() {
     const ForceArgs f = force_args();
-    const hipError_t e = launch_forces(sigma_, types_, false, true, block_, f, stream_);
+    const hipError_t e = launch_forces(sigma_, types_, false, true, energy_, block_, f, stream_);
+    ene_valid_ = energy_;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
@@ -715,7 +719,9 @@ int Engine::launch_forces_only(bool debug) {
     if (launch_sort()) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[6], stream_);  // no list build
     const ForceArgs f = force_args();
-    const hipError_t e = launch_forces(sigma_, types_, debug, false, block_, f, stream_);
+    const hipError_t e =
+        launch_forces(sigma_, types_, debug, false, energy_ || debug, block_, f, stream_);
+    ene_valid_ = energy_ || debug;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
@@ -794,13 +800,15 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     }
     // the first half-kick of this batch of steps (velocities stay a
     // consistent KDK state: a final half-kick closes the loop below)
-    if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
+    energy_ = false;  // as in the timed steps of a run
+    if (launch_step_body(false, true)) { energy_ = true; return check(hipGetLastError(), "step", err); }
     for (int s = 0; s < nsteps; ++s) {
         prof_ev_ = &ev[K * s];
         int bad = launch_step_body(true, true);
         prof_ev_ = nullptr;
-        if (bad) return check(hipGetLastError(), "step", err);
+        if (bad) { energy_ = true; return check(hipGetLastError(), "step", err); }
     }
+    energy_ = true;
     if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps + 1;
     rc = sync(err);
@@ -844,8 +852,13 @@ int Engine::step(int nsteps, std::string &err) {
     rc = ensure_primed(err);
     if (rc) return rc;
     // femmd2D/src/integrator.cpp:8-29 per step; the second half-kick of
-    // step s is fused with the first half-kick of step s+1
-    if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
+    // step s is fused with the first half-kick of step s+1.  Pair energies
+    // and virials are accumulated only in the last step (the one whose
+    // energies can be queried).
+    energy_ = nsteps == 1;
+    int bad0 = launch_step_body(false, true);
+    energy_ = false;
+    if (bad0) { energy_ = true; return check(hipGetLastError(), "step", err); }
     int rem = nsteps - 1;
     if (use_graph_ && rem >= kGraphSteps + 1) {
         if (!graph_exec_) {
@@ -857,21 +870,26 @@ int Engine::step(int nsteps, std::string &err) {
             hipError_t ce = hipStreamEndCapture(stream_, &g);
             if (bad || ce != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
+                energy_ = true;
                 return check(ce != hipSuccess ? ce : hipErrorLaunchFailure, "capture step", err);
             }
             graph_ = g;
             rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate", err);
-            if (rc) { drop_graph(); return rc; }
+            if (rc) { drop_graph(); energy_ = true; return rc; }
             graph_steps_ = kGraphSteps;
         }
-        while (rem >= graph_steps_) {
+        // the last body stays outside the graph (it accumulates energies)
+        while (rem > graph_steps_) {
             rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);
-            if (rc) return rc;
+            if (rc) { energy_ = true; return rc; }
             rem -= graph_steps_;
         }
     }
-    while (rem-- > 0)
-        if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+    for (; rem > 0; --rem) {
+        energy_ = rem == 1;
+        if (launch_step_body(true, true)) { energy_ = true; return check(hipGetLastError(), "step", err); }
+    }
+    energy_ = true;
     if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps;
     return sync(err);
@@ -924,6 +942,9 @@ int Engine::energies(int r, fm_energies *out, std::string &err) {
     if (rc) return rc;
     if (!primed_) {
         rc = compute_forces(err);
+        if (rc) return rc;
+    } else if (!ene_valid_) {
+        rc = energy_pass(err);
         if (rc) return rc;
     }
     if (launch_energy(R_, rng_.p, ke_part_.p, lj_part_.p, fea_part_.p, bond_part_.p, ene_.p,
@@ -1107,16 +1128,21 @@ int Engine::time_forces(int reps, double *ms, std::string &err) {
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, stream_);
+    energy_ = false;  // the production force pass
     for (int k = 0; k < reps; ++k) {
-        if (launch_bin()) return check(hipGetLastError(), "bin", err);
-        if (list_) {
+        int bad = launch_bin();
+        if (!bad && list_) {
             // a rebuild (sort + list build) and the list pair pass
-            if (launch_sort() || launch_list_build(false) || launch_pair_pass())
-                return check(hipGetLastError(), "forces", err);
-        } else if (launch_forces_only(false)) {
+            bad = launch_sort() || launch_list_build(false) || launch_pair_pass();
+        } else if (!bad) {
+            bad = launch_forces_only(false);
+        }
+        if (bad) {
+            energy_ = true;
             return check(hipGetLastError(), "forces", err);
         }
     }
+    energy_ = true;
     (void)hipEventRecord(b, stream_);
     rc = check(hipEventSynchronize(b), "time", err);
     float t = 0;
@@ -1126,6 +1152,15 @@ int Engine::time_forces(int reps, double *ms, std::string &err) {
     *ms = t / std::max(reps, 1);
     if (rc) return rc;
     return sync(err);
+}
+
+// Pair energies and virials at the current positions (the forces are
+// recomputed with the same bits: same lists, or the same deterministic sort).
+int Engine::energy_pass(std::string &err) {
+    energy_ = true;
+    const int bad = list_ ? launch_pair_pass() : (launch_bin() || launch_forces_only(false));
+    if (bad) return check(hipGetLastError(), "energy pass", err);
+    return FM_OK;
 }
 
 int Engine::list_stats(int r, double *mean, int32_t *maxlen, int64_t *nbuild, std::string &err) {

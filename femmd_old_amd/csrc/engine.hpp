@@ -150,6 +150,11 @@ private:
     float skin_ = 0.0f;
     int nbr_cap_ = 0;
     bool primed_ = false;
+    // pair energies and virials are accumulated only by force passes with
+    // energy_ set (the last step of a run and every explicit query);
+    // ene_valid_: the partials belong to the current positions
+    bool energy_ = true, ene_valid_ = false;
+    int energy_pass(std::string &err);
     std::vector<char> lj_on_;
     int64_t steps_ = 0;
     int model_ = 0;
@@ -213,7 +218,7 @@ private:
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s);
 hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s);
-hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int block,
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s);
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
 size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered);

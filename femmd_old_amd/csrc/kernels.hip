@@ -468,7 +468,7 @@ struct PairLds {
 // LJ over one tile (femmd2D/src/interactionsCuda.cu:43-100 pair maths,
 // full-list accumulation of femmd2D/src/interactions.cpp:347-389: each
 // ordered pair adds to f_i only, U and W take 1/2).
-template <bool kWrap, bool kSigma, bool kTypes, bool kDebug>
+template <bool kWrap, bool kSigma, bool kTypes, bool kDebug, bool kEnergy>
 __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
                                int tile, PairLds &L) {
     const int row = W + 2;
@@ -570,17 +570,21 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
                     const float fc = e48 * tv * inv;
                     fx = __builtin_fmaf(fc, dxa, fx);
                     fy = __builtin_fmaf(fc, dya, fy);
-                    accA = __builtin_fmaf(e4, s12 - sr6, accA);  // energy
-                    accB = __builtin_fmaf(e48, tv, accB);        // virial
-                    ush += shift;
+                    if (kEnergy) {
+                        accA = __builtin_fmaf(e4, s12 - sr6, accA);  // energy
+                        accB = __builtin_fmaf(e48, tv, accB);        // virial
+                        ush += shift;
+                    }
                 } else {
                     const float fc = tv * inv;
                     fx = __builtin_fmaf(fc, dxa, fx);
                     fy = __builtin_fmaf(fc, dya, fy);
-                    accA += s12;
-                    accB += sr6;
+                    if (kEnergy) {
+                        accA += s12;
+                        accB += sr6;
+                    }
                 }
-                ++nint;
+                if (kEnergy || kDebug) ++nint;
                 if (kDebug) {
                     const long long v = (long long)a.uid[a.perm[L.sp[j]]] + 1;
                     s1 += v;
@@ -604,8 +608,10 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
         const int id = a.perm[sbase + k];
         a.fx[id] = fx;
         a.fy[id] = fy;
-        Ublk += 0.5 * Ui;
-        Wblk += 0.5 * Wi;
+        if (kEnergy) {
+            Ublk += 0.5 * Ui;
+            Wblk += 0.5 * Wi;
+        }
         if (kDebug) {
             const int u = a.uid[id];
             a.dbg_cnt[u] = nint;
@@ -613,10 +619,12 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             a.dbg_s2[u] = s2;
         }
     }
-    block_sum2(Ublk, Wblk, L.red);
-    if (threadIdx.x == 0) {
-        a.lj_part[2 * tile] = Ublk;
-        a.lj_part[2 * tile + 1] = Wblk;
+    if (kEnergy) {
+        block_sum2(Ublk, Wblk, L.red);
+        if (threadIdx.x == 0) {
+            a.lj_part[2 * tile] = Ublk;
+            a.lj_part[2 * tile + 1] = Wblk;
+        }
     }
 }
 
@@ -685,7 +693,7 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
     return M;
 }
 
-template <bool kSigma, bool kTypes, bool kDebug>
+template <bool kSigma, bool kTypes, bool kDebug, bool kEnergy>
 __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
     const Tile T = a.tiles[tile];
     const RepDesc R = a.reps[T.rep];
@@ -700,9 +708,9 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
         return;
     }
     if (wrapT)
-        pair_tile_body<true, kSigma, kTypes, kDebug>(a, R, W, tile, L);
+        pair_tile_body<true, kSigma, kTypes, kDebug, kEnergy>(a, R, W, tile, L);
     else
-        pair_tile_body<false, kSigma, kTypes, kDebug>(a, R, W, tile, L);
+        pair_tile_body<false, kSigma, kTypes, kDebug, kEnergy>(a, R, W, tile, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -837,7 +845,7 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // pair maths and accumulation of pair_tile_body (FP32 cutoff predicate on
 // the current positions, so the interacting set is the same as the cell
 // walk's whenever the list is complete).
-template <bool kSigma>
+template <bool kSigma, bool kEnergy>
 __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     const Span sp = a.lspans[b];
     const RepDesc R = a.reps[sp.rep];
@@ -847,6 +855,8 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     const size_t N = a.n;
     const int cap = a.nbr_cap;
     double Ublk = 0.0, Wblk = 0.0;
+    (void)Ublk;
+    (void)Wblk;
     // lanes walk sorted positions p (the list build's order): list slots
     // nbr[k * n + p] are read coalesced
     for (int p = sp.begin + (int)threadIdx.x; p < sp.end; p += blockDim.x) {
@@ -855,7 +865,17 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? a.sigs[p] : 0.0f;
         const int n4 = (a.nnum[p] + 3) >> 2;  // padded with p itself
-        const int32_t *lst = a.nbr + p;
+        // 32-bit element offsets from the array bases (the engine keeps the
+        // lists below 2^31 bytes), so loads take a scalar base register
+        const char *xysb = reinterpret_cast<const char *>(a.xys);
+        const char *nbrb = reinterpret_cast<const char *>(a.nbr);
+        auto pos = [&](int q) {
+            return *reinterpret_cast<const float2 *>(xysb + (uint32_t)q * 8u);
+        };
+        auto slot = [&](uint32_t e) {
+            return *reinterpret_cast<const int32_t *>(nbrb + e * 4u);
+        };
+        const uint32_t Nu = (uint32_t)N;
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
         auto term = [&](float2 pj, int q, auto wrap) {
@@ -899,9 +919,11 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 const float fc = tv * inv;
                 fx = __builtin_fmaf(fc, dxa, fx);
                 fy = __builtin_fmaf(fc, dya, fy);
-                accA += s12;
-                accB += sr6;
-                ++nint;
+                if (kEnergy) {
+                    accA += s12;
+                    accB += sr6;
+                    ++nint;
+                }
             }
         };
         auto walk = [&](auto wrap) {
@@ -910,11 +932,13 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             // (loads are unconditional so that the compiler can count them;
             // the row is clamped inside the list array, and indices read past
             // a particle's list are never used)
-            int q0 = lst[0], q1 = lst[N], q2 = lst[2 * N], q3 = lst[3 * N];
+            const uint32_t e0 = (uint32_t)p;
+            int q0 = slot(e0), q1 = slot(e0 + Nu), q2 = slot(e0 + 2 * Nu), q3 = slot(e0 + 3 * Nu);
             for (int k = 0; k < n4; ++k) {
-                const float2 r0 = a.xys[q0], r1 = a.xys[q1], r2 = a.xys[q2], r3 = a.xys[q3];
-                const int32_t *ln = lst + (size_t)min(4 * k + 4, cap - 4) * N;
-                const int n0 = ln[0], n1 = ln[N], n2 = ln[2 * N], n3 = ln[3 * N];
+                const float2 r0 = pos(q0), r1 = pos(q1), r2 = pos(q2), r3 = pos(q3);
+                const uint32_t en = e0 + (uint32_t)min(4 * k + 4, cap - 4) * Nu;
+                const int n0 = slot(en), n1 = slot(en + Nu), n2 = slot(en + 2 * Nu),
+                          n3 = slot(en + 3 * Nu);
                 term(r0, q0, wrap);
                 term(r1, q1, wrap);
                 term(r2, q2, wrap);
@@ -936,17 +960,21 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             walk(std::false_type{});
         fx *= pe.e48;
         fy *= pe.e48;
-        const double A = accA, B = accB;
-        const double Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
-        const double Wi = (double)pe.e48 * (A - 0.5 * B);
         a.fs[p] = make_float2(fx, fy);
-        Ublk += 0.5 * Ui;
-        Wblk += 0.5 * Wi;
+        if (kEnergy) {
+            const double A = accA, B = accB;
+            const double Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
+            const double Wi = (double)pe.e48 * (A - 0.5 * B);
+            Ublk += 0.5 * Ui;
+            Wblk += 0.5 * Wi;
+        }
     }
-    block_sum2(Ublk, Wblk, red);
-    if (threadIdx.x == 0) {
-        a.lj_part[2 * b] = Ublk;
-        a.lj_part[2 * b + 1] = Wblk;
+    if (kEnergy) {
+        block_sum2(Ublk, Wblk, red);
+        if (threadIdx.x == 0) {
+            a.lj_part[2 * b] = Ublk;
+            a.lj_part[2 * b + 1] = Wblk;
+        }
     }
 }
 
@@ -1081,7 +1109,7 @@ __device__ void bond_block(const ForceArgs &a, int b, double *red) {
     }
 }
 
-template <bool kSigma, bool kTypes, bool kDebug, bool kList>
+template <bool kSigma, bool kTypes, bool kDebug, bool kList, bool kEnergy>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
@@ -1098,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     }
     if (kList) {
         if (b < a.nlspans) {
-            list_pair_block<kSigma>(a, b, s_red);
+            list_pair_block<kSigma, kEnergy>(a, b, s_red);
             return;
         }
         b -= a.nlspans;
@@ -1112,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         L.off = s_off;
         L.pt = s_pt;
         L.red = s_red;
-        pair_tile<kSigma, kTypes, kDebug>(a, xcd_remap(b, a.ntiles), L);
+        pair_tile<kSigma, kTypes, kDebug, kEnergy>(a, xcd_remap(b, a.ntiles), L);
         return;
     } else {
         b -= a.ntiles;
@@ -1659,29 +1687,33 @@ size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
            (debug ? lds_cap * sizeof(int) : 0);
 }
 
-hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, int block,
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s) {
     if (list) {
         // neighbour lists: single type, no dump (the caller checks)
         const int nblk = a.nlspans + a.nespans + a.nbspans;
         if (nblk <= 0) return hipSuccess;
-        if (sigma)
-            hipLaunchKernelGGL((k_forces<true, false, false, true>), dim3(nblk), dim3(block), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_forces<false, false, false, true>), dim3(nblk), dim3(block), 0, s, a);
+        dim3 g(nblk), b(block);
+#define FM_LIST(S, E) hipLaunchKernelGGL((k_forces<S, false, false, true, E>), g, b, 0, s, a)
+        if (sigma) { if (energy) FM_LIST(true, true); else FM_LIST(true, false); }
+        else { if (energy) FM_LIST(false, true); else FM_LIST(false, false); }
+#undef FM_LIST
         return hipGetLastError();
     }
     const int nblk = a.ntiles + a.nespans + a.nbspans;
     if (nblk <= 0) return hipSuccess;
     const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
     dim3 g(nblk), b(block);
-#define FM_LAUNCH(S, T, D) hipLaunchKernelGGL((k_forces<S, T, D, false>), g, b, lds, s, a)
-    if (debug) {
-        if (sigma) { if (types) FM_LAUNCH(true, true, true); else FM_LAUNCH(true, false, true); }
-        else { if (types) FM_LAUNCH(false, true, true); else FM_LAUNCH(false, false, true); }
+#define FM_LAUNCH(S, T, D, E) hipLaunchKernelGGL((k_forces<S, T, D, false, E>), g, b, lds, s, a)
+    if (debug) {  // the dump pass always accumulates energies
+        if (sigma) { if (types) FM_LAUNCH(true, true, true, true); else FM_LAUNCH(true, false, true, true); }
+        else { if (types) FM_LAUNCH(false, true, true, true); else FM_LAUNCH(false, false, true, true); }
+    } else if (energy) {
+        if (sigma) { if (types) FM_LAUNCH(true, true, false, true); else FM_LAUNCH(true, false, false, true); }
+        else { if (types) FM_LAUNCH(false, true, false, true); else FM_LAUNCH(false, false, false, true); }
     } else {
-        if (sigma) { if (types) FM_LAUNCH(true, true, false); else FM_LAUNCH(true, false, false); }
-        else { if (types) FM_LAUNCH(false, true, false); else FM_LAUNCH(false, false, false); }
+        if (sigma) { if (types) FM_LAUNCH(true, true, false, false); else FM_LAUNCH(true, false, false, false); }
+        else { if (types) FM_LAUNCH(false, true, false, false); else FM_LAUNCH(false, false, false, false); }
     }
 #undef FM_LAUNCH
     return hipGetLastError();
