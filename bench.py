@@ -7,8 +7,9 @@ energy sums, dt = 0.005, synthetic seeded inputs (femmd_old_amd.configs.c2).
 
 A step is one pass of the hot path over the whole system: half-kick + drift
 + wrap, the neighbour-list update (binning, counting sort and list build
-on the steps where the displacement bound crosses skin / 2, the
-reference's rule with its skin 0.4), LJ / CST forces, and the second
+on the steps where some particle has moved more than skin / 2 since the
+last build, the reference's rule; skin 0.6, the engine default), LJ / CST
+forces, and the second
 half-kick with its kinetic-energy reduction (fused into the next step's
 first kernel).  --skin 0 walks the cells every step instead.  Inputs are
 resident in HBM when the timed region starts.
@@ -221,7 +222,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--tune-cache", default=None)
-    ap.add_argument("--skin", type=float, default=0.4,
+    ap.add_argument("--skin", type=float, default=0.6,
                     help="neighbour-list skin (0: walk the cells every step)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",

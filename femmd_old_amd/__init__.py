@@ -320,8 +320,9 @@ class System:
                          "forces", "step"), list(ms)))
 
     def set_skin(self, skin: float):
-        """Neighbour-list skin (reference default 0.4, femmd2D/src/nn.h:63);
-        0 walks the cells every step.  Before allocate_cuda."""
+        """Neighbour-list skin (default 0.6; the reference's CPU list uses 0.4,
+        femmd2D/src/nn.h:63); 0 walks the cells every step.  Before
+        allocate_cuda."""
         self._check(self._lib.fm_set_skin(self._h, float(skin)))
 
     def list_stats(self) -> dict:

@@ -494,7 +494,7 @@ def c5(seed: int = 5, scale: float = 1.0, nx: int = 128, ny: int = 256) -> Workl
 def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None):
     """A femmd_old_amd.System holding workload w (caller order: solvent, then
     each elastomer's nodes, as global_arrays() lists them).  skin: the
-    neighbour-list skin (None keeps the engine default 0.4; 0 walks the
+    neighbour-list skin (None keeps the engine default 0.6; 0 walks the
     cells every step)."""
     from . import System
     s = System(w.Lx, w.Ly, dt=w.dt, device=device)

@@ -126,7 +126,8 @@ int fm_set_lj(fm_system *h, int32_t on);
  * femmd2D/src/particles.cpp:289-293); -1 = none.  Before fm_commit. */
 int fm_set_exclusion(fm_system *h, int32_t i, int32_t group);
 
-/* Neighbour-list skin (femmd2D/src/nn.h:63, default 0.4), before fm_commit.
+/* Neighbour-list skin (default 0.6; the reference's CPU list uses 0.4,
+ * femmd2D/src/nn.h:63), before fm_commit.
  * skin > 0: pair forces run over Verlet lists of radius rc*sigma + skin,
  * rebuilt (binning, counting sort, list build) only when the running sum of
  * per-step maximum displacements exceeds skin / 2 (femmd2D/src/nn.cpp:

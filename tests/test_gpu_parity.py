@@ -13,8 +13,8 @@ Bars:
   * run-twice, graph/no-graph, tile-shape and batched-vs-looped runs are
     bitwise identical.
 
-Both pair paths run every case: the neighbour lists (skin 0.4, the engine
-default) and the per-step cell walk (skin 0).  Forces checked after some
+Both pair paths run every case: the neighbour lists (the engine default
+skin 0.6, and the reference's 0.4) and the per-step cell walk (skin 0).  Forces checked after some
 steps in list mode come from lists built steps earlier, so a list that
 missed an interacting pair fails the force bar.
 """
@@ -29,7 +29,7 @@ pytestmark = pytest.mark.gpu
 FORCE_TOL = 1e-5
 
 
-SKINS = [0.4, 0.0]
+SKINS = [0.6, 0.4, 0.0]
 
 
 def engine_and_oracle(w, skin=None):
