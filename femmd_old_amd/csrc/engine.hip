@@ -516,9 +516,10 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
         while (bb < (int)bspans_h_.size() && bspans_h_[bb].rep == r) ++bb;
         g[7] = bb;
     }
-    // LDS capacity: the close-packed count of the staged cells, rounded up
-    // to a multiple of 128
-    int cap = ((int)std::ceil(worst) + 127) / 128 * 128;
+    // LDS capacity: 1.3 x the close-packed count of the staged cells (an
+    // elastomer under compression can exceed close packing a little), rounded
+    // up to a multiple of 128
+    int cap = ((int)std::ceil(1.3 * worst) + 127) / 128 * 128;
     const int hard = (types_ ? 2560 : 3072);  // dynamic LDS stays <= 64 KB
     lds_cap_ = std::max(256, std::min(cap, hard));
     hipError_t he;

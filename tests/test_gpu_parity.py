@@ -263,3 +263,13 @@ def test_list_and_cell_paths_agree():
     d = np.hypot((xa - xb + w.Lx / 2) % w.Lx - w.Lx / 2, (ya - yb + w.Ly / 2) % w.Ly - w.Ly / 2)
     assert d.max() < 1e-3
     assert a.list_stats()["rebuilds"] >= 1 and b.list_stats()["rebuilds"] == 0
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_full_sheet_tiles_fit(skin):
+    """The 128 x 256-node sheet (hex lattice at close packing) in solvent:
+    staged tiles must fit the LDS capacity the engine sizes."""
+    w = configs.c5()
+    s = configs.build_system(w, skin=skin)
+    s.run(40)
+    assert s.device_errors() == 0
