@@ -491,7 +491,8 @@ def c5(seed: int = 5, scale: float = 1.0, nx: int = 128, ny: int = 256) -> Workl
 # ---------------------------------------------------------------------------
 # engine construction
 # ---------------------------------------------------------------------------
-def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None):
+def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None,
+                 list_wide: bool = False):
     """A femmd_old_amd.System holding workload w (caller order: solvent, then
     each elastomer's nodes, as global_arrays() lists them).  skin: the
     neighbour-list skin (None keeps the engine default 0.6; 0 walks the
@@ -500,6 +501,8 @@ def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None):
     s = System(w.Lx, w.Ly, dt=w.dt, device=device)
     if skin is not None:
         s.set_skin(skin)
+    if list_wide:
+        s.set_list_wide(True)
     s.set_pair_potential([w.eps], [w.rc], [w.ushift])
     s.add_particles(w.x, w.y, w.vx, w.vy, w.mass, w.sigma)
     for el in w.elastomers:

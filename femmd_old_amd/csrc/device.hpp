@@ -160,9 +160,14 @@ struct ForceArgs {
     // neighbour-list mode: pair role over kick spans, one lane per particle
     const Span *lspans;
     int32_t nlspans;
-    int32_t nbr_cap;           // list capacity per particle
-    int32_t *nbr;              // nbr[k * n + p]: k-th neighbour (internal id)
-                               // of the particle at sorted position p
+    int32_t nbr_cap;           // list capacity per particle (multiple of 8)
+    void *nbr;                 // neighbours (sorted positions) of the particle
+                               // at sorted position p in groups of four, group
+                               // g at element g * n + p: four int16 offsets
+                               // q - p (uint64 per group; wrapped within the
+                               // replica when q - p does not fit) or, with
+                               // lwide, four int32 positions (int4 per group)
+    int32_t lwide;
     int32_t *nnum;             // list length per sorted position
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set

@@ -321,6 +321,12 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     // the list pass addresses the lists with 32-bit byte offsets: larger
     // systems walk the cells every step (cells of side >= rc*sigma + skin)
     if (E->list_ && (double)E->nbr_cap_ * p0 * 4.0 >= 2147483648.0) E->list_ = false;
+    // 16-bit list offsets: a neighbour's sorted position is at most two cell
+    // rows and three cells (of at most cap_ particles) away from the centre
+    // particle's (across the periodic boundary: within the replica's range)
+    E->lwide_ = reps[0]->list_wide != 0;
+    for (int r = 0; r < R; ++r)
+        if ((double)(2 * rd[r].ncx + 3) * E->cap_ > 32000.0) E->lwide_ = true;
     E->ntri_ = (int)ht0.size();
     E->nbond_ = (int)hbi.size();
     E->gather_ = (E->ntri_ + E->nbond_) > 0;
@@ -410,7 +416,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(ene_, 8 * R);
     FM_AL(err_, 1);
     if (E->list_) {
-        FM_AL(nbr_, (size_t)E->nbr_cap_ * N);
+        FM_AL(nbr_, (size_t)E->nbr_cap_ * N / (E->lwide_ ? 1 : 2));
         FM_AL(nnum_, N);
         FM_AL(ctl_, R);
         FM_AL(any_, 1);
@@ -765,6 +771,7 @@ ForceArgs Engine::force_args() {
     f.nlspans = (int)lspans_h_.size();
     f.nbr_cap = nbr_cap_;
     f.nbr = nbr_.p;
+    f.lwide = lwide_ ? 1 : 0;
     f.nnum = nnum_.p;
     // stage list entries in LDS when the tile and the buffer fit in 64 KB
     // (static LDS of the kernel: ~2 KB)

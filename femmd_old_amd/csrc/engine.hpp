@@ -44,6 +44,7 @@ struct HostSystem {
     // slightly longer list and measured faster here.  Any skin gives the same
     // interacting sets.
     float skin = 0.6f;
+    int32_t list_wide = 0;  // 1: 32-bit list entries even where 16 bits fit
     // launch tuning
     int32_t tile_w = 0, block = 256, use_graph = 1;
 };
@@ -152,6 +153,7 @@ private:
     bool list_ = false;  // neighbour-list mode
     float skin_ = 0.0f;
     int nbr_cap_ = 0;
+    bool lwide_ = false;  // 32-bit list entries (offsets may not fit 16 bits)
     bool primed_ = false;
     // pair energies and virials are accumulated only by force passes with
     // energy_ set (the last step of a run and every explicit query);

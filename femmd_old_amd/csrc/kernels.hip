@@ -717,9 +717,41 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
 // Neighbour-list mode (femmd2D/src/nn.cpp:22-62, re-designed): a tile of
 // cells of side >= rc*sigma + skin is staged as in the pair pass; every
 // centre particle records the partners within rc*sigma + skin (exclusion
-// rule applied here, once) in column-major order, nbr[k * n + i], so that a
-// wavefront of consecutive particles reads one list slot coalesced.
+// rule applied here, once) as sorted positions, four per group, groups
+// interleaved over particles (group g of position p at g * n + p) so that a
+// wavefront of consecutive positions reads one group in one coalesced load.
 // ---------------------------------------------------------------------------
+
+// 16-bit list offset of neighbour q from p: q - p when it fits, else the
+// offset wrapped within the replica's position range [p0, p0 + nr) (a pair
+// across the periodic boundary of a large replica).
+__device__ __forceinline__ uint32_t list_offset(int q, int p, int nr) {
+    int d = q - p;
+    if (d > 32767 || d < -32768) d += (d > 0) ? -nr : nr;
+    return (uint32_t)(uint16_t)(int16_t)d;
+}
+
+// Stores group g (four neighbours q[0..3]) of sorted position p.
+__device__ __forceinline__ void list_store_group(const ForceArgs &a, int g, int p, int nr,
+                                                 const int *q) {
+    const size_t e = (size_t)g * a.n + p;
+    if (a.lwide) {
+        reinterpret_cast<int4 *>(a.nbr)[e] = make_int4(q[0], q[1], q[2], q[3]);
+    } else {
+        const uint64_t v = (uint64_t)list_offset(q[0], p, nr) |
+                           ((uint64_t)list_offset(q[1], p, nr) << 16) |
+                           ((uint64_t)list_offset(q[2], p, nr) << 32) |
+                           ((uint64_t)list_offset(q[3], p, nr) << 48);
+        reinterpret_cast<uint64_t *>(a.nbr)[e] = v;
+    }
+}
+
+// Stores slot k alone (list build without the LDS staging buffer).
+__device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p, int nr, int q) {
+    const size_t e = ((size_t)(k >> 2) * a.n + p) * 4 + (k & 3);
+    if (a.lwide) reinterpret_cast<int32_t *>(a.nbr)[e] = q;
+    else reinterpret_cast<uint16_t *>(a.nbr)[e] = (uint16_t)list_offset(q, p, nr);
+}
 template <bool kWrap, bool kSigma, bool kBuf>
 __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
                                uint16_t *buf) {
@@ -731,7 +763,6 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
     const PairEntry pe = L.pt[0];
     const int bd = blockDim.x;  // held in a register across the loops
     const int tid = threadIdx.x;
-    const size_t N = a.n;
     const int cap = a.nbr_cap;
     for (int k = tid; k < nc; k += bd) {
         const int me = c_lo + k;
@@ -747,7 +778,7 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int gix = gi >= 0 ? gi : -2;
         const float si = pi.w;
         const int p = sbase + k;  // sorted position of this centre particle
-        int32_t *dst = a.nbr + p;
+        const int nr = R.p1 - R.p0;
         int cnt = 0;
         uint16_t *bp = buf + tid;  // this lane's next LDS list slot
         // the three staged rows, each a contiguous run of candidates
@@ -783,7 +814,7 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 if (hit) {
                     if (cnt < cap) {
                         if (kBuf) *bp = (uint16_t)j;
-                        else dst[(size_t)cnt * N] = L.sp[j];
+                        else list_store_slot(a, cnt, p, nr, L.sp[j]);
                     }
                     bp += bd;
                     ++cnt;
@@ -791,14 +822,23 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
             }
         }
         const int m = min(cnt, cap);
-        if (kBuf) {
-            // column by column: the lanes of a wavefront hold consecutive
-            // sorted positions, so each store is contiguous
-            for (int q = 0; q < m; ++q) dst[(size_t)q * N] = L.sp[buf[q * bd + tid]];
-        }
-        // pad to a multiple of 4 with the particle itself (distance 0: the
+        // groups of four, padded with the particle itself (distance 0: the
         // pair pass's cutoff test rejects it); cap is a multiple of 8
-        for (int q = m; q < ((m + 3) & ~3); ++q) dst[(size_t)q * N] = p;
+        if (kBuf) {
+            // the lanes of a wavefront hold consecutive sorted positions, so
+            // each group store is contiguous
+            for (int g = 0; g < ((m + 3) >> 2); ++g) {
+                int q[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int kk = 4 * g + t;
+                    q[t] = kk < m ? L.sp[buf[kk * bd + tid]] : p;
+                }
+                list_store_group(a, g, p, nr, q);
+            }
+        } else {
+            for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, nr, p);
+        }
         a.nnum[p] = m;
         if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
@@ -857,8 +897,8 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
     double Ublk = 0.0, Wblk = 0.0;
     (void)Ublk;
     (void)Wblk;
-    // lanes walk sorted positions p (the list build's order): list slots
-    // nbr[k * n + p] are read coalesced
+    // lanes walk sorted positions p (the list build's order): list groups
+    // g * n + p are read coalesced
     for (int p = sp.begin + (int)threadIdx.x; p < sp.end; p += blockDim.x) {
         const float2 pi = a.xys[p];
         const float xi = pi.x, yi = pi.y;
@@ -872,10 +912,8 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
         auto pos = [&](int q) {
             return *reinterpret_cast<const float2 *>(xysb + (uint32_t)q * 8u);
         };
-        auto slot = [&](uint32_t e) {
-            return *reinterpret_cast<const int32_t *>(nbrb + e * 4u);
-        };
         const uint32_t Nu = (uint32_t)N;
+        const int capg = cap >> 2;
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
         auto term = [&](float2 pj, int q, auto wrap) {
@@ -926,27 +964,40 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 }
             }
         };
-        auto walk = [&](auto wrap) {
-            // groups of four list slots; the next group's indices are loaded
-            // while this group's positions are gathered and evaluated
-            // (loads are unconditional so that the compiler can count them;
-            // the row is clamped inside the list array, and indices read past
-            // a particle's list are never used)
-            const uint32_t e0 = (uint32_t)p;
-            int q0 = slot(e0), q1 = slot(e0 + Nu), q2 = slot(e0 + 2 * Nu), q3 = slot(e0 + 3 * Nu);
+        // groups of four neighbours; the next group is loaded while this
+        // group's positions are gathered and evaluated (loads are
+        // unconditional so the compiler can count them: the group index is
+        // clamped inside the array, and groups past the list are not used)
+        const int p0 = R.p0, nr = R.p1 - R.p0;
+        auto walk = [&](auto wrap, auto wide) {
+            auto load = [&](int g) {
+                const uint32_t e = (uint32_t)min(g, capg - 1) * Nu + (uint32_t)p;
+                if (decltype(wide)::value)
+                    return *reinterpret_cast<const int4 *>(nbrb + e * 16u);
+                const uint64_t v = *reinterpret_cast<const uint64_t *>(nbrb + e * 8u);
+                return make_int4((int)(int16_t)(v & 0xffffu), (int)(int16_t)((v >> 16) & 0xffffu),
+                                 (int)(int16_t)((v >> 32) & 0xffffu), (int)(int16_t)(v >> 48));
+            };
+            auto decode = [&](int d) {
+                if (decltype(wide)::value) return d;  // absolute position
+                int q = p + d;
+                if (decltype(wrap)::value) {
+                    // an offset wrapped within the replica (see list_offset)
+                    q += (q < p0) ? nr : 0;
+                    q -= (q >= p0 + nr) ? nr : 0;
+                }
+                return q;
+            };
+            int4 gv = load(0);
             for (int k = 0; k < n4; ++k) {
+                const int q0 = decode(gv.x), q1 = decode(gv.y), q2 = decode(gv.z),
+                          q3 = decode(gv.w);
                 const float2 r0 = pos(q0), r1 = pos(q1), r2 = pos(q2), r3 = pos(q3);
-                const uint32_t en = e0 + (uint32_t)min(4 * k + 4, cap - 4) * Nu;
-                const int n0 = slot(en), n1 = slot(en + Nu), n2 = slot(en + 2 * Nu),
-                          n3 = slot(en + 3 * Nu);
+                gv = load(k + 1);
                 term(r0, q0, wrap);
                 term(r1, q1, wrap);
                 term(r2, q2, wrap);
                 term(r3, q3, wrap);
-                q0 = n0;
-                q1 = n1;
-                q2 = n2;
-                q3 = n3;
             }
         };
         // neighbours lie within rl of the particle (up to skin / 2 more for
@@ -954,10 +1005,13 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
         // needs the minimum image; the choice is uniform per wavefront
         const float m = R.rl_max;
         const bool edge = (xi < m) | (xi > Lx - m) | (yi < m) | (yi > Ly - m);
-        if (__any(edge))
-            walk(std::true_type{});
-        else
-            walk(std::false_type{});
+        if (a.lwide) {
+            if (__any(edge)) walk(std::true_type{}, std::true_type{});
+            else walk(std::false_type{}, std::true_type{});
+        } else {
+            if (__any(edge)) walk(std::true_type{}, std::false_type{});
+            else walk(std::false_type{}, std::false_type{});
+        }
         fx *= pe.e48;
         fy *= pe.e48;
         a.fs[p] = make_float2(fx, fy);
