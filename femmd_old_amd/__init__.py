@@ -325,11 +325,6 @@ class System:
         allocate_cuda."""
         self._check(self._lib.fm_set_skin(self._h, float(skin)))
 
-    def set_list_wide(self, wide: bool = True):
-        """32-bit neighbour-list entries even where 16-bit offsets fit (the
-        results do not depend on it).  Before allocate_cuda."""
-        self._check(self._lib.fm_set_list_wide(self._h, int(bool(wide))))
-
     def list_stats(self) -> dict:
         """{"mean", "max", "rebuilds"} of the neighbour lists (zeros when
         the cells are walked every step)."""

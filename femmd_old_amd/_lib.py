@@ -83,7 +83,6 @@ PROTOTYPES = {
     "fm_set_lj": (C.c_int, [_h, C.c_int32]),
     "fm_set_exclusion": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "fm_set_skin": (C.c_int, [_h, C.c_float]),
-    "fm_set_list_wide": (C.c_int, [_h, C.c_int32]),
     "fm_list_stats": (C.c_int, [_h, _d, _i, _l]),
     "fm_set_mass": (C.c_int, [_h, C.c_int32, _i, _f]),
     "fm_commit": (C.c_int, [_h]),

@@ -491,8 +491,7 @@ def c5(seed: int = 5, scale: float = 1.0, nx: int = 128, ny: int = 256) -> Workl
 # ---------------------------------------------------------------------------
 # engine construction
 # ---------------------------------------------------------------------------
-def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None,
-                 list_wide: bool = False):
+def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None):
     """A femmd_old_amd.System holding workload w (caller order: solvent, then
     each elastomer's nodes, as global_arrays() lists them).  skin: the
     neighbour-list skin (None keeps the engine default 0.6; 0 walks the
@@ -501,8 +500,6 @@ def build_system(w: Workload, device: int = 0, commit: bool = True, skin=None,
     s = System(w.Lx, w.Ly, dt=w.dt, device=device)
     if skin is not None:
         s.set_skin(skin)
-    if list_wide:
-        s.set_list_wide(True)
     s.set_pair_potential([w.eps], [w.rc], [w.ushift])
     s.add_particles(w.x, w.y, w.vx, w.vy, w.mass, w.sigma)
     for el in w.elastomers:

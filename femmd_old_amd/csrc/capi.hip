@@ -243,14 +243,6 @@ int fm_set_skin(fm_system *h, float skin) {
     FM_END(h)
 }
 
-int fm_set_list_wide(fm_system *h, int32_t wide) {
-    FM_GUARD(h)
-    if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
-    h->hs.list_wide = wide != 0;
-    return FM_OK;
-    FM_END(h)
-}
-
 int fm_set_exclusion(fm_system *h, int32_t i, int32_t group) {
     FM_GUARD(h)
     if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");

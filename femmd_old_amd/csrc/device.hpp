@@ -157,17 +157,12 @@ struct ForceArgs {
     const float *bk, *br0;
     int32_t cf_bond0;          // first corner slot of the bonds
     double *bond_part;         // 2 per bond workgroup: U, W
-    // neighbour-list mode: pair role over kick spans, one lane per particle
-    const Span *lspans;
-    int32_t nlspans;
+    // neighbour-list mode (pair role over the tiles of the list build)
     int32_t nbr_cap;           // list capacity per particle (multiple of 8)
-    void *nbr;                 // neighbours (sorted positions) of the particle
-                               // at sorted position p in groups of four, group
-                               // g at element g * n + p: four int16 offsets
-                               // q - p (uint64 per group; wrapped within the
-                               // replica when q - p does not fit) or, with
-                               // lwide, four int32 positions (int4 per group)
-    int32_t lwide;
+    void *nbr;                 // neighbours of the particle at sorted position
+                               // p as uint16 indices into its tile's LDS
+                               // staging, four per uint64 group, group g at
+                               // element g * n + p
     int32_t *nnum;             // list length per sorted position
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set

@@ -207,7 +207,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     std::vector<float2> halm;
     std::vector<float> heE;
     std::vector<float> hbk, hbr0;
-    std::vector<Span> ks, es, bs, ls;
+    std::vector<Span> ks, es, bs;
     std::vector<int32_t> erange(2 * R), brange(2 * R);
     E->iid_h_.resize(R);
     E->lj_on_.resize(R);
@@ -258,10 +258,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             huid.push_back(u);
         }
         D.ks0 = (int32_t)ks.size();
-        for (int b = p0; b < p0 + n; b += kKickThreads) {
+        for (int b = p0; b < p0 + n; b += kKickThreads)
             ks.push_back(Span{r, b, std::min(b + kKickThreads, p0 + n), 0});
-            if (S.lj_on) ls.push_back(ks.back());
-        }
         D.nks = (int32_t)ks.size() - D.ks0;
         // elements, ordered by their lowest internal node for locality
         const int nt = (int)S.tri.size() / 3;
@@ -318,15 +316,10 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     }
     E->N_ = p0;
     E->ncells_ = cell0;
-    // the list pass addresses the lists with 32-bit byte offsets: larger
+    // the list pass indexes list groups with 32-bit element indices: larger
     // systems walk the cells every step (cells of side >= rc*sigma + skin)
-    if (E->list_ && (double)E->nbr_cap_ * p0 * 4.0 >= 2147483648.0) E->list_ = false;
-    // 16-bit list offsets: a neighbour's sorted position is at most two cell
-    // rows and three cells (of at most cap_ particles) away from the centre
-    // particle's (across the periodic boundary: within the replica's range)
-    E->lwide_ = reps[0]->list_wide != 0;
-    for (int r = 0; r < R; ++r)
-        if ((double)(2 * rd[r].ncx + 3) * E->cap_ > 32000.0) E->lwide_ = true;
+    if (E->list_ && (double)E->nbr_cap_ / 4 * p0 >= 4294967295.0) E->list_ = false;
+
     E->ntri_ = (int)ht0.size();
     E->nbond_ = (int)hbi.size();
     E->gather_ = (E->ntri_ + E->nbond_) > 0;
@@ -335,7 +328,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->kspans_h_ = ks;
     E->espans_h_ = es;
     E->bspans_h_ = bs;
-    E->lspans_h_ = ls;
 
     // gather CSR: per particle, its corner slots in ascending slot order
     std::vector<int32_t> goff, gent;
@@ -380,7 +372,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_UP(kspans_, ks);
     FM_UP(espans_, es);
     FM_UP(bspans_, bs);
-    FM_UP(lspans_, ls);
     FM_UP(t0_, ht0);
     FM_UP(t1_, ht1);
     FM_UP(t2_, ht2);
@@ -416,7 +407,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(ene_, 8 * R);
     FM_AL(err_, 1);
     if (E->list_) {
-        FM_AL(nbr_, (size_t)E->nbr_cap_ * N / (E->lwide_ ? 1 : 2));
+        FM_AL(nbr_, (size_t)E->nbr_cap_ * N / 2);  // uint16 entries
         FM_AL(nnum_, N);
         FM_AL(ctl_, R);
         FM_AL(any_, 1);
@@ -503,18 +494,8 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
         g[0] = kb;
         while (kb < (int)kspans_h_.size() && kspans_h_[kb].rep == r) ++kb;
         g[1] = kb;
-        if (list_) {
-            // list mode: the pair partials are per list span of the replica
-            int lb = 0;
-            while (lb < (int)lspans_h_.size() && lspans_h_[lb].rep < r) ++lb;
-            int le = lb;
-            while (le < (int)lspans_h_.size() && lspans_h_[le].rep == r) ++le;
-            g[2] = lb;
-            g[3] = le;
-        } else {
-            g[2] = t_begin;
-            g[3] = (int)tiles_h_.size();
-        }
+        g[2] = t_begin;
+        g[3] = (int)tiles_h_.size();
         g[4] = eb;
         while (eb < (int)espans_h_.size() && espans_h_[eb].rep == r) ++eb;
         g[5] = eb;
@@ -531,8 +512,7 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
     hipError_t he;
     if ((he = upload(tiles_, tiles_h_, stream_)) != hipSuccess) return check(he, "tiles", err);
     if ((he = upload(rng_, rng_h_, stream_)) != hipSuccess) return check(he, "ranges", err);
-    if ((he = lj_part_.alloc(2 * std::max<size_t>(std::max(tiles_h_.size(), lspans_h_.size()), 1))) !=
-        hipSuccess)
+    if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
         return check(he, "lj_part", err);
     if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess)
         return check(he, "memset", err);
@@ -767,11 +747,8 @@ ForceArgs Engine::force_args() {
     f.br0 = br0_.p;
     f.cf_bond0 = 3 * ntri_;
     f.bond_part = bond_part_.p;
-    f.lspans = lspans_.p;
-    f.nlspans = (int)lspans_h_.size();
     f.nbr_cap = nbr_cap_;
     f.nbr = nbr_.p;
-    f.lwide = lwide_ ? 1 : 0;
     f.nnum = nnum_.p;
     // stage list entries in LDS when the tile and the buffer fit in 64 KB
     // (static LDS of the kernel: ~2 KB)

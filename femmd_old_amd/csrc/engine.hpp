@@ -44,7 +44,6 @@ struct HostSystem {
     // slightly longer list and measured faster here.  Any skin gives the same
     // interacting sets.
     float skin = 0.6f;
-    int32_t list_wide = 0;  // 1: 32-bit list entries even where 16 bits fit
     // launch tuning
     int32_t tile_w = 0, block = 256, use_graph = 1;
 };
@@ -153,7 +152,6 @@ private:
     bool list_ = false;  // neighbour-list mode
     float skin_ = 0.0f;
     int nbr_cap_ = 0;
-    bool lwide_ = false;  // 32-bit list entries (offsets may not fit 16 bits)
     bool primed_ = false;
     // pair energies and virials are accumulated only by force passes with
     // energy_ set (the last step of a run and every explicit query);
@@ -169,7 +167,7 @@ private:
     std::vector<int32_t> uid_h_;               // internal -> caller id (per replica)
     std::vector<std::vector<int32_t>> iid_h_;  // per replica: caller -> internal
     std::vector<int> tile_w_;                  // per replica tile width in use
-    std::vector<Span> kspans_h_, espans_h_, bspans_h_, lspans_h_;
+    std::vector<Span> kspans_h_, espans_h_, bspans_h_;
     std::vector<Tile> tiles_h_;
     std::vector<int32_t> rng_h_;  // per replica 8 ints (energy ranges)
     std::vector<double> ene_h_;
@@ -180,7 +178,7 @@ private:
     DBuf<int32_t> grp_, typ_, uid_;
     DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_, key_;
     DBuf<float4> sorted_;
-    DBuf<Span> kspans_, espans_, bspans_, lspans_;
+    DBuf<Span> kspans_, espans_, bspans_;
     DBuf<int32_t> nbr_, nnum_, any_;
     DBuf<Ctl> ctl_;
     DBuf<float> xb_, yb_, sigs_;

@@ -633,9 +633,13 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
 // 26-37; sorted positions are contiguous per cell.  With wsrc set (the
 // adjoint pass) the per-particle vector w is staged beside each entry.
 // Returns the number of staged particles, or -1 if they exceed the LDS.
-template <bool kTypes, bool kDebug, bool kPid = false>
-__device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
-                          PairLds &L, const float2 *wsrc, float2 *lw) {
+// Layout of a tile's staging: the 3 x (W+2) cells of rows cy-1, cy, cy+1 and
+// columns cx0-1 .. cx0+W with the periodic wrap of femmd2D/src/nnCuda.cu:
+// 26-37.  Fills L.cstart (first sorted position per slot), L.off (first LDS
+// entry per slot, S + 1) and the pair table; returns the number of staged
+// particles, or -1 if they exceed the LDS capacity.
+__device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
+                            PairLds &L) {
     const int row = W + 2, S = 3 * row;
     const int tid = threadIdx.x;
     const int base = R.p0 - a.start[R.cell0];  // see k_scatter
@@ -675,20 +679,48 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
     }
     __syncthreads();
     const int M = L.off[S];
-    if (M > a.lds_cap) return -1;
-    for (int e = tid; e < M; e += blockDim.x) {
-        int lo = 0, hi = S;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (L.off[mid] <= e) lo = mid; else hi = mid;
+    return M > a.lds_cap ? -1 : M;
+}
+
+// Calls f(e, p) for every staged entry e (LDS index) and its sorted position
+// p.  The cells of one staged row are consecutive in sorted order except
+// where the column index wraps to 0 (at most twice), so each row is copied
+// as up to three contiguous runs, lanes striding over the row's entries.
+template <class F>
+__device__ void stage_copy(const Tile &T, const RepDesc &R, int W, const PairLds &L, F &&f) {
+    const int row = W + 2;
+    // slots (within a row) where the column wraps to 0
+    int b1 = (T.cx0 == 0) ? 1 : (R.ncx - T.cx0 + 1);
+    int b2 = b1 + R.ncx;
+    if (b1 >= row) b1 = row;
+    if (b2 >= row) b2 = row;
+    for (int r = 0; r < 3; ++r) {
+        const int s0 = r * row;
+        const int e0 = L.off[s0], e1 = L.off[s0 + row];
+        const int o1 = L.off[s0 + b1], o2 = L.off[s0 + b2];
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            const int ps = (e >= o2) ? b2 : ((e >= o1) ? b1 : 0);
+            f(e, L.cstart[s0 + ps] + (e - L.off[s0 + ps]));
         }
-        const int p = L.cstart[lo] + (e - L.off[lo]);
+    }
+}
+
+// Stages a tile's particles from the sorted arrays ({x, y, grp bits,
+// sigma} and types; with wsrc set, the adjoint's per-particle vector beside
+// them).  Returns the number of staged particles, or -1 if they exceed the
+// LDS capacity.
+template <bool kTypes, bool kDebug, bool kPid = false>
+__device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
+                          PairLds &L, const float2 *wsrc, float2 *lw) {
+    const int M = stage_layout(a, T, R, W, L);
+    if (M < 0) return -1;
+    stage_copy(T, R, W, L, [&](int e, int p) {
         L.ent[e] = a.sorted[p];
         if (kTypes) L.typ[e] = a.sorted_typ[p];
         if (kDebug) L.sp[e] = p;
-        if (kPid) L.sp[e] = a.perm[p];  // internal id (list build)
+        if (kPid) L.sp[e] = a.perm[p];  // internal id
         if (wsrc) lw[e] = wsrc[a.perm[p]];
-    }
+    });
     __syncthreads();
     return M;
 }
@@ -717,41 +749,28 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
 // Neighbour-list mode (femmd2D/src/nn.cpp:22-62, re-designed): a tile of
 // cells of side >= rc*sigma + skin is staged as in the pair pass; every
 // centre particle records the partners within rc*sigma + skin (exclusion
-// rule applied here, once) as sorted positions, four per group, groups
-// interleaved over particles (group g of position p at g * n + p) so that a
-// wavefront of consecutive positions reads one group in one coalesced load.
+// rule applied here, once) by their index in the tile's LDS staging, 16 bits
+// each, four per group, groups interleaved over particles (group g of sorted
+// position p at g * n + p) so that a wavefront of consecutive positions reads
+// one group in one coalesced load.  Until the next rebuild the sorted order
+// and the tiles stay fixed, so the list pass restages each tile the same way
+// from the current positions and gathers the neighbours from LDS.
 // ---------------------------------------------------------------------------
 
-// 16-bit list offset of neighbour q from p: q - p when it fits, else the
-// offset wrapped within the replica's position range [p0, p0 + nr) (a pair
-// across the periodic boundary of a large replica).
-__device__ __forceinline__ uint32_t list_offset(int q, int p, int nr) {
-    int d = q - p;
-    if (d > 32767 || d < -32768) d += (d > 0) ? -nr : nr;
-    return (uint32_t)(uint16_t)(int16_t)d;
-}
-
-// Stores group g (four neighbours q[0..3]) of sorted position p.
-__device__ __forceinline__ void list_store_group(const ForceArgs &a, int g, int p, int nr,
-                                                 const int *q) {
-    const size_t e = (size_t)g * a.n + p;
-    if (a.lwide) {
-        reinterpret_cast<int4 *>(a.nbr)[e] = make_int4(q[0], q[1], q[2], q[3]);
-    } else {
-        const uint64_t v = (uint64_t)list_offset(q[0], p, nr) |
-                           ((uint64_t)list_offset(q[1], p, nr) << 16) |
-                           ((uint64_t)list_offset(q[2], p, nr) << 32) |
-                           ((uint64_t)list_offset(q[3], p, nr) << 48);
-        reinterpret_cast<uint64_t *>(a.nbr)[e] = v;
-    }
+// Stores group g (four LDS indices j[0..3]) of sorted position p.
+__device__ __forceinline__ void list_store_group(const ForceArgs &a, int g, int p,
+                                                 const int *j) {
+    const uint64_t v = (uint64_t)(uint16_t)j[0] | ((uint64_t)(uint16_t)j[1] << 16) |
+                       ((uint64_t)(uint16_t)j[2] << 32) | ((uint64_t)(uint16_t)j[3] << 48);
+    reinterpret_cast<uint64_t *>(a.nbr)[(size_t)g * a.n + p] = v;
 }
 
 // Stores slot k alone (list build without the LDS staging buffer).
-__device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p, int nr, int q) {
-    const size_t e = ((size_t)(k >> 2) * a.n + p) * 4 + (k & 3);
-    if (a.lwide) reinterpret_cast<int32_t *>(a.nbr)[e] = q;
-    else reinterpret_cast<uint16_t *>(a.nbr)[e] = (uint16_t)list_offset(q, p, nr);
+__device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p, int j) {
+    reinterpret_cast<uint16_t *>(a.nbr)[((size_t)(k >> 2) * a.n + p) * 4 + (k & 3)] =
+        (uint16_t)j;
 }
+
 template <bool kWrap, bool kSigma, bool kBuf>
 __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
                                uint16_t *buf) {
@@ -778,7 +797,6 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int gix = gi >= 0 ? gi : -2;
         const float si = pi.w;
         const int p = sbase + k;  // sorted position of this centre particle
-        const int nr = R.p1 - R.p0;
         int cnt = 0;
         uint16_t *bp = buf + tid;  // this lane's next LDS list slot
         // the three staged rows, each a contiguous run of candidates
@@ -814,7 +832,7 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 if (hit) {
                     if (cnt < cap) {
                         if (kBuf) *bp = (uint16_t)j;
-                        else list_store_slot(a, cnt, p, nr, L.sp[j]);
+                        else list_store_slot(a, cnt, p, j);
                     }
                     bp += bd;
                     ++cnt;
@@ -828,16 +846,16 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
             // the lanes of a wavefront hold consecutive sorted positions, so
             // each group store is contiguous
             for (int g = 0; g < ((m + 3) >> 2); ++g) {
-                int q[4];
+                int jj[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int kk = 4 * g + t;
-                    q[t] = kk < m ? L.sp[buf[kk * bd + tid]] : p;
+                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : me;
                 }
-                list_store_group(a, g, p, nr, q);
+                list_store_group(a, g, p, jj);
             }
         } else {
-            for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, nr, p);
+            for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, me);
         }
         a.nnum[p] = m;
         if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
@@ -853,18 +871,18 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     PairLds L;
     L.ent = dyn;
     L.typ = nullptr;
-    L.sp = reinterpret_cast<int *>(dyn + a.lds_cap);  // staged sorted positions
+    L.sp = nullptr;
     L.cstart = s_cstart;
     L.off = s_off;
     L.pt = s_pt;
     L.red = nullptr;
-    uint16_t *buf = reinterpret_cast<uint16_t *>(L.sp + a.lds_cap);
+    uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap);
     for (int b = blockIdx.x; b < a.ntiles; b += gridDim.x) {
         const Tile T = a.tiles[xcd_remap(b, a.ntiles)];
         if (a.ctl && !a.ctl[T.rep].flag) continue;  // uniform per workgroup
         const RepDesc R = a.reps[T.rep];
         const int W = T.w_flags & 0xffff;
-        if (stage_tile<false, true>(a, T, R, W, L, nullptr, nullptr) < 0) {
+        if (stage_tile<false, false>(a, T, R, W, L, nullptr, nullptr) < 0) {
             if (threadIdx.x == 0) atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
             __syncthreads();
             continue;
@@ -885,42 +903,41 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // pair maths and accumulation of pair_tile_body (FP32 cutoff predicate on
 // the current positions, so the interacting set is the same as the cell
 // walk's whenever the list is complete).
-template <bool kSigma, bool kEnergy>
-__device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
-    const Span sp = a.lspans[b];
-    const RepDesc R = a.reps[sp.rep];
-    const PairEntry pe = a.ptab[R.pt];
+// LJ over the neighbour lists, one workgroup per tile of the list build: the
+// tile's current positions (and sigmas) are staged in LDS in the build's
+// layout, each lane takes a centre particle and gathers its listed
+// neighbours from LDS (the pair maths and accumulation of pair_tile_body;
+// the FP32 cutoff predicate on the current positions, so the interacting
+// set is the cell walk's whenever the list is complete).
+template <bool kWrap, bool kSigma, bool kEnergy>
+__device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W, int tile,
+                                    const PairLds &L, const float2 *lp, const float *ls) {
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const PairEntry pe = L.pt[0];
     const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
-    const size_t N = a.n;
-    const int cap = a.nbr_cap;
+    const uint32_t Nu = (uint32_t)a.n;
+    const int capg = a.nbr_cap >> 2;
+    const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
     double Ublk = 0.0, Wblk = 0.0;
-    (void)Ublk;
-    (void)Wblk;
-    // lanes walk sorted positions p (the list build's order): list groups
-    // g * n + p are read coalesced
-    for (int p = sp.begin + (int)threadIdx.x; p < sp.end; p += blockDim.x) {
-        const float2 pi = a.xys[p];
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int me = c_lo + k, p = sbase + k;
+        const float2 pi = lp[me];
         const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
-        const float si = kSigma ? a.sigs[p] : 0.0f;
-        const int n4 = (a.nnum[p] + 3) >> 2;  // padded with p itself
-        // 32-bit element offsets from the array bases (the engine keeps the
-        // lists below 2^31 bytes), so loads take a scalar base register
-        const char *xysb = reinterpret_cast<const char *>(a.xys);
-        const char *nbrb = reinterpret_cast<const char *>(a.nbr);
-        auto pos = [&](int q) {
-            return *reinterpret_cast<const float2 *>(xysb + (uint32_t)q * 8u);
-        };
-        const uint32_t Nu = (uint32_t)N;
-        const int capg = cap >> 2;
+        const float si = kSigma ? ls[me] : 0.0f;
+        const int n4 = (a.nnum[p] + 3) >> 2;  // padded with the particle itself
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
-        auto term = [&](float2 pj, int q, auto wrap) {
+        auto term = [&](int j) {
+            const float2 pj = lp[j];
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
-            if (decltype(wrap)::value) {
+            if (kWrap) {
                 // minimum image as two selects per axis (no branches)
                 ox = (ex >= hLx) ? Lx : 0.0f;
                 ox = (ex < -hLx) ? -Lx : ox;
@@ -933,7 +950,7 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             float s2ij;
             unsigned rc2m1;
             if (kSigma) {
-                const float sij = (si + a.sigs[q]) * 0.5f;
+                const float sij = (si + ls[j]) * 0.5f;
                 rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
                 s2ij = sij * sij;
             } else {
@@ -942,7 +959,7 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
             }
             if ((__float_as_uint(dr2) - 1u) < rc2m1) {
                 float dxa = dx, dya = dy, r2a = dr2;
-                if (decltype(wrap)::value && ((ox != 0.0f) | (oy != 0.0f))) {
+                if (kWrap && ((ox != 0.0f) | (oy != 0.0f))) {
                     // image-exact difference: shift the coordinate first
                     if (ox > 0.0f) dxa = xim - xj; else if (ox < 0.0f) dxa = xi - (xj - Lx);
                     if (oy > 0.0f) dya = yim - yj; else if (oy < 0.0f) dya = yi - (yj - Ly);
@@ -964,72 +981,60 @@ __device__ void list_pair_block(const ForceArgs &a, int b, double *red) {
                 }
             }
         };
-        // groups of four neighbours; the next group is loaded while this
-        // group's positions are gathered and evaluated (loads are
-        // unconditional so the compiler can count them: the group index is
-        // clamped inside the array, and groups past the list are not used)
-        const int p0 = R.p0, nr = R.p1 - R.p0;
-        auto walk = [&](auto wrap, auto wide) {
-            auto load = [&](int g) {
-                const uint32_t e = (uint32_t)min(g, capg - 1) * Nu + (uint32_t)p;
-                if (decltype(wide)::value)
-                    return *reinterpret_cast<const int4 *>(nbrb + e * 16u);
-                const uint64_t v = *reinterpret_cast<const uint64_t *>(nbrb + e * 8u);
-                return make_int4((int)(int16_t)(v & 0xffffu), (int)(int16_t)((v >> 16) & 0xffffu),
-                                 (int)(int16_t)((v >> 32) & 0xffffu), (int)(int16_t)(v >> 48));
-            };
-            auto decode = [&](int d) {
-                if (decltype(wide)::value) return d;  // absolute position
-                int q = p + d;
-                if (decltype(wrap)::value) {
-                    // an offset wrapped within the replica (see list_offset)
-                    q += (q < p0) ? nr : 0;
-                    q -= (q >= p0 + nr) ? nr : 0;
-                }
-                return q;
-            };
-            int4 gv = load(0);
-            for (int k = 0; k < n4; ++k) {
-                const int q0 = decode(gv.x), q1 = decode(gv.y), q2 = decode(gv.z),
-                          q3 = decode(gv.w);
-                const float2 r0 = pos(q0), r1 = pos(q1), r2 = pos(q2), r3 = pos(q3);
-                gv = load(k + 1);
-                term(r0, q0, wrap);
-                term(r1, q1, wrap);
-                term(r2, q2, wrap);
-                term(r3, q3, wrap);
-            }
-        };
-        // neighbours lie within rl of the particle (up to skin / 2 more for
-        // the motion since the build): away from the box edges no pair
-        // needs the minimum image; the choice is uniform per wavefront
-        const float m = R.rl_max;
-        const bool edge = (xi < m) | (xi > Lx - m) | (yi < m) | (yi > Ly - m);
-        if (a.lwide) {
-            if (__any(edge)) walk(std::true_type{}, std::true_type{});
-            else walk(std::false_type{}, std::true_type{});
-        } else {
-            if (__any(edge)) walk(std::true_type{}, std::false_type{});
-            else walk(std::false_type{}, std::false_type{});
+        // the next group is loaded while this one is evaluated (the group
+        // index is clamped inside the array; groups past the list are unused)
+        uint64_t gv = nb[(uint32_t)p];
+        for (int g = 0; g < n4; ++g) {
+            const uint64_t cur = gv;
+            gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)p];
+            term((int)(cur & 0xffffu));
+            term((int)((cur >> 16) & 0xffffu));
+            term((int)((cur >> 32) & 0xffffu));
+            term((int)(cur >> 48));
         }
         fx *= pe.e48;
         fy *= pe.e48;
         a.fs[p] = make_float2(fx, fy);
         if (kEnergy) {
             const double A = accA, B = accB;
-            const double Ui = (double)pe.e4 * (A - B) + (double)nint * pe.shift;
-            const double Wi = (double)pe.e48 * (A - 0.5 * B);
-            Ublk += 0.5 * Ui;
-            Wblk += 0.5 * Wi;
+            Ublk += 0.5 * ((double)pe.e4 * (A - B) + (double)nint * pe.shift);
+            Wblk += 0.5 * ((double)pe.e48 * (A - 0.5 * B));
         }
     }
     if (kEnergy) {
-        block_sum2(Ublk, Wblk, red);
+        block_sum2(Ublk, Wblk, L.red);
         if (threadIdx.x == 0) {
-            a.lj_part[2 * b] = Ublk;
-            a.lj_part[2 * b + 1] = Wblk;
+            a.lj_part[2 * tile] = Ublk;
+            a.lj_part[2 * tile + 1] = Wblk;
         }
     }
+}
+
+template <bool kSigma, bool kEnergy>
+__device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 *lp,
+                               float *ls) {
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    if (stage_layout(a, T, R, W, L) < 0) {
+        if (threadIdx.x == 0) {
+            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+            if (kEnergy) {
+                a.lj_part[2 * tile] = 0.0;
+                a.lj_part[2 * tile + 1] = 0.0;
+            }
+        }
+        return;
+    }
+    stage_copy(T, R, W, L, [&](int e, int p) {
+        lp[e] = a.xys[p];
+        if (kSigma) ls[e] = a.sigs[p];
+    });
+    __syncthreads();
+    if ((T.w_flags >> 16) & 1)
+        list_tile_pass_body<true, kSigma, kEnergy>(a, R, W, tile, L, lp, ls);
+    else
+        list_tile_pass_body<false, kSigma, kEnergy>(a, R, W, tile, L, lp, ls);
 }
 
 // CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
@@ -1179,11 +1184,21 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             a.dmax[(size_t)k * kDispStride] = 0u;
     }
     if (kList) {
-        if (b < a.nlspans) {
-            list_pair_block<kSigma, kEnergy>(a, b, s_red);
+        if (b < a.ntiles) {
+            PairLds L;
+            L.ent = nullptr;
+            L.typ = nullptr;
+            L.sp = nullptr;
+            L.cstart = s_cstart;
+            L.off = s_off;
+            L.pt = s_pt;
+            L.red = s_red;
+            float2 *lp = reinterpret_cast<float2 *>(dyn);
+            float *ls = reinterpret_cast<float *>(lp + a.lds_cap);
+            list_tile_pass<kSigma, kEnergy>(a, xcd_remap(b, a.ntiles), L, lp, ls);
             return;
         }
-        b -= a.nlspans;
+        b -= a.ntiles;
     } else if (b < a.ntiles) {
         PairLds L;
         L.ent = dyn;
@@ -1745,10 +1760,11 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
                          const ForceArgs &a, hipStream_t s) {
     if (list) {
         // neighbour lists: single type, no dump (the caller checks)
-        const int nblk = a.nlspans + a.nespans + a.nbspans;
+        const int nblk = a.ntiles + a.nespans + a.nbspans;
         if (nblk <= 0) return hipSuccess;
         dim3 g(nblk), b(block);
-#define FM_LIST(S, E) hipLaunchKernelGGL((k_forces<S, false, false, true, E>), g, b, 0, s, a)
+        const size_t lds = (size_t)a.lds_cap * (sizeof(float2) + (sigma ? sizeof(float) : 0));
+#define FM_LIST(S, E) hipLaunchKernelGGL((k_forces<S, false, false, true, E>), g, b, lds, s, a)
         if (sigma) { if (energy) FM_LIST(true, true); else FM_LIST(true, false); }
         else { if (energy) FM_LIST(false, true); else FM_LIST(false, false); }
 #undef FM_LIST
@@ -1774,7 +1790,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
 }
 
 size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered) {
-    return (size_t)lds_cap * (sizeof(float4) + sizeof(int)) +
+    return (size_t)lds_cap * sizeof(float4) +
            (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) : 0);
 }
 

@@ -134,10 +134,6 @@ int fm_set_exclusion(fm_system *h, int32_t i, int32_t group);
  * 114-134), so most steps skip the sort.  skin = 0: the cells are walked
  * every step.  Typed pair tables (ntypes > 1) always walk the cells. */
 int fm_set_skin(fm_system *h, float skin);
-/* List entry format, before fm_commit: 0 (default) stores 16-bit offsets
- * wherever the cell grid guarantees they fit (32-bit otherwise), 1 always
- * stores 32-bit positions.  Results do not depend on it. */
-int fm_set_list_wide(fm_system *h, int32_t wide);
 
 /* Masses of the given particles (Particles::mass, femmd2D/src/particles.h:56).
  * A zero, negative or infinite mass pins the particle (it never moves).

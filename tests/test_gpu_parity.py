@@ -275,27 +275,16 @@ def test_full_sheet_tiles_fit(skin):
     assert s.device_errors() == 0
 
 
-@pytest.mark.parametrize("wide", [False, True])
-def test_list_formats_and_wrapped_offsets(wide):
-    """A system of ~114k particles: pairs across the periodic boundary are
-    more than 32767 sorted positions apart, so the 16-bit list wraps their
-    offsets within the replica; the 32-bit format stores positions.  Both
-    meet the force bar and give the same forces bit for bit (same lists,
-    same order)."""
+def test_large_system_lists():
+    """~114k particles (tiles with and without the periodic wrap, many list
+    rebuilds): forces from lists built steps earlier meet the bar."""
     w = configs.c2(scale=0.1)
-    s = configs.build_system(w, skin=0.6, list_wide=wide)
+    s = configs.build_system(w, skin=0.6)
     osys = O.from_workload(w, *s.get_state()[:2])
-    for k in range(2):
+    for k in range(3):
         x, y, _, _ = s.get_state()
         err, _ = force_err(s, osys.with_positions(x, y))
-        print(f"[c2m wide={wide} step {k * 25}] max rel force err {err.max():.3e}")
+        print(f"[c2m step {k * 25}] max rel force err {err.max():.3e}")
         assert err.max() <= FORCE_TOL
         s.run(25)
-    assert s.device_errors() == 0
-    if wide:
-        n = configs.build_system(w, skin=0.6, list_wide=False)
-        n.run(25)
-        m = configs.build_system(w, skin=0.6, list_wide=True)
-        m.run(25)
-        for a_, b_ in zip(n.get_state(), m.get_state()):
-            np.testing.assert_array_equal(a_, b_)
+    assert s.device_errors() == 0 and s.list_stats()["rebuilds"] >= 2
