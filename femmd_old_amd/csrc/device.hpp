@@ -4,7 +4,8 @@
 // internal order (per replica: particles sorted by their cell at commit time,
 // which keeps every cell's members close in memory).  A batch of replicas is
 // stored back to back: replica r owns particles [p0, p1), cells
-// [cell0, cell0 + ncx*ncy), elements and bonds of its own ranges.
+// [cell0, cell0 + ncx*ncy] (the last one always empty: it closes the
+// replica's cell starts), elements and bonds of its own ranges.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -103,6 +104,7 @@ struct KickArgs {
 
 struct ScanArgs {
     const RepDesc *reps;
+    int32_t nrep;
     int32_t ncells;
     int32_t cap;
     int32_t n;                 // particles

@@ -312,7 +312,10 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         for (int b = b_begin; b < (int)hbi.size(); b += kElemThreads)
             bs.push_back(Span{r, b, std::min(b + kElemThreads, (int)hbi.size()), 0});
         p0 += n;
-        cell0 += D.ncx * D.ncy;
+        // one extra (always empty) cell closes the replica's range of cell
+        // starts, so a replica's counts never depend on its neighbour's
+        // starts (list-mode rebuilds rescan only the rebuilding replicas)
+        cell0 += D.ncx * D.ncy + 1;
     }
     E->N_ = p0;
     E->ncells_ = cell0;
@@ -667,6 +670,7 @@ int Engine::launch_sort(bool gated) {
     s.reps = reps_.p;
     s.spans = kspans_.p;
     s.nspans = (int)kspans_h_.size();
+    s.nrep = R_;
     if (list_) {
         s.spos = spos_.p;
         s.xys = xys_.p;

@@ -393,11 +393,27 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
     int wbase = 0;
     for (int k = 0; k < w; ++k) wbase += wsum[k];
     int run = s_pre + wbase + incl - tsum;
+    // list mode: only the rebuilding replicas' cell starts change (the
+    // others' lists index staging laid out by their own last starts)
+    int rep = 0;
+    if (a.ctl && base < a.ncells) {
+        int lo = 0, hi = a.nrep;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.reps[mid].cell0 <= base) lo = mid; else hi = mid;
+        }
+        rep = lo;
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int c = base + q;
         if (c >= a.ncells) break;
-        a.start[c] = run;
+        if (a.ctl) {
+            while (rep + 1 < a.nrep && a.reps[rep + 1].cell0 <= c) ++rep;
+            if (a.ctl[rep].flag) a.start[c] = run;
+        } else {
+            a.start[c] = run;
+        }
         a.count[c] = 0;
         run += n[q];
         if (c == a.ncells - 1) a.start[a.ncells] = run;
