@@ -55,7 +55,7 @@ constexpr int kDispStride = 32;
 // Per-replica control words of the neighbour-list mode.
 struct alignas(16) Ctl {
     int32_t flag;    // rebuild the list in this step
-    float disp;      // largest displacement since the last build (this step)
+    float disp;      // displacement bound since the last build
     uint32_t nbuild; // builds so far
     int32_t pad;
 };
@@ -92,10 +92,9 @@ struct KickArgs {
     uint32_t *err;
     Ctl *ctl;                  // list mode (gated binning): rebuild decision
     int32_t *any;              // list mode: set when some replica rebuilds
-    float *xb, *yb;            // list mode: positions at the last list build
     uint32_t *dmax;            // list mode: per replica kDispSlots slots of the
-                               // step's max |x - xb|^2 (float bits), kDispStride
-                               // words apart
+                               // step's max squared displacement (float bits),
+                               // kDispStride words apart
     const float2 *fs;          // list mode: pair forces in sorted order
     const int32_t *spos;       // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order (out)
@@ -174,6 +173,7 @@ struct ForceArgs {
     const Ctl *ctl;            // list build gating (null: always build)
     int32_t *any;              // reset by the force pass after the rebuild
     uint32_t *dmax;            // displacement slots, cleared by the force pass
+    Ctl *ctl_w;                // displacement bounds, updated by the force pass
     int32_t nrep;
     int32_t n;                 // particles (stride of nbr)
     // diagnostics (pair-set dump); null in production

@@ -414,8 +414,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         FM_AL(nnum_, N);
         FM_AL(ctl_, R);
         FM_AL(any_, 1);
-        FM_AL(xb_, N);
-        FM_AL(yb_, N);
         FM_AL(dmax_, (size_t)R * kDispSlots * kDispStride);
         FM_AL(spos_, N);
         FM_AL(xys_, N);
@@ -582,10 +580,6 @@ int Engine::launch_bin(bool gated) {
         a.any = any_.p;
         a.dmax = dmax_.p;
     }
-    if (list_) {
-        a.xb = xb_.p;
-        a.yb = yb_.p;
-    }
     return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
@@ -637,8 +631,6 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.ke_part = ke_part_.p;
     k.err = err_.p;
     if (list_) {
-        k.xb = xb_.p;
-        k.yb = yb_.p;
         k.dmax = dmax_.p;
         k.fs = fs_.p;
         k.spos = spos_.p;
@@ -762,6 +754,7 @@ ForceArgs Engine::force_args() {
     f.sigs = sigs_.p;
     f.any = list_ ? any_.p : nullptr;
     f.dmax = dmax_.p;
+    f.ctl_w = ctl_.p;
     f.nrep = R_;
     f.n = N_;
     f.uid = uid_.p;

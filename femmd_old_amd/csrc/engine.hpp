@@ -181,7 +181,7 @@ private:
     DBuf<Span> kspans_, espans_, bspans_;
     DBuf<int32_t> nbr_, nnum_, any_;
     DBuf<Ctl> ctl_;
-    DBuf<float> xb_, yb_, sigs_;
+    DBuf<float> sigs_;
     DBuf<uint32_t> dmax_;
     DBuf<int32_t> spos_;
     DBuf<float2> xys_, fs_;

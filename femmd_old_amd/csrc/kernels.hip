@@ -192,7 +192,10 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             vx = vx + R.hdt * ax;
             vy = vy + R.hdt * ay;
             const float vv = vx * vx + vy * vy;
-            ke = (double)(a.mass[p] * vv);
+            // m from 1/m (one rounding from the stored mass; a pinned
+            // particle has inv_m = 0 and zero velocity)
+            const float m = im > 0.0f ? 1.0f / im : 0.0f;
+            ke = (double)(m * vv);
             vvmax = vv;
         }
         if (kDrift) {
@@ -213,11 +216,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             if (!kList) cell = R.cell0 + cell_key(x, y, R);
             if (kList) {
                 a.xys[spi] = make_float2(x, y);
-                // displacement since the last list build (minimum image)
-                float ex = x - a.xb[p], ey = y - a.yb[p];
-                if (ex >= R.hLx) ex -= R.Lx; else if (ex < -R.hLx) ex += R.Lx;
-                if (ey >= R.hLy) ey -= R.Ly; else if (ey < -R.hLy) ey += R.Ly;
-                d2 = ex * ex + ey * ey;
+                d2 = dxp * dxp + dyp * dyp;  // this step's displacement
             }
         }
         a.vx[p] = vx;
@@ -277,23 +276,31 @@ __global__ __launch_bounds__(kKickThreads) void k_ke(KickArgs a) {
 
 // Binning of the current positions without moving them (used after
 // fm_set_state / fm_commit and by fm_compute_forces).
-// List mode rebuild decision for replica R, made by every wavefront that
-// bins one of its spans (all read the same kDispSlots maxima); the
-// workgroup holding the replica's first span records it.  The largest
-// displacement since the last build against skin / 2 is the reference's
-// criterion (femmd2D/src/nn.cpp:114-134) with the exact displacement in
-// place of its running bound sum sqrt(max v^2) dt.
-__device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R) {
+// List mode displacement bound of replica r after this step: the running
+// sum of per-step maximum displacements since the last build (the
+// reference's dispHi, femmd2D/src/md3dsystem.cpp:177, here summed from the
+// drift actually applied, before the decision) plus a rounding allowance
+// for the stored positions.  Called by one wavefront; every caller reads
+// the same slots and bound, so all get the same value.
+__device__ __forceinline__ float list_disp(const uint32_t *dmax, const Ctl *ctl, int r,
+                                           const RepDesc &R) {
     const int lane = threadIdx.x & 63;
-    const uint32_t v =
-        lane < kDispSlots ? a.dmax[((size_t)sp.rep * kDispSlots + lane) * kDispStride] : 0u;
+    const uint32_t v = lane < kDispSlots ? dmax[((size_t)r * kDispSlots + lane) * kDispStride] : 0u;
     const float mm = wave_max(__uint_as_float(v));
-    const float d = sqrtf(mm) * 1.0001f + R.dslack;
+    return ctl[r].disp + sqrtf(mm) * 1.0001f + R.dslack;
+}
+
+// List mode rebuild decision for replica R (femmd2D/src/nn.cpp:114-134:
+// rebuild when the bound exceeds skin / 2), made by every wavefront that
+// bins one of its spans; the workgroup holding the replica's first span
+// records it.  The force pass of the step then folds this step into the
+// bound (or resets it after a rebuild).
+__device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R) {
+    const float d = list_disp(a.dmax, a.ctl, sp.rep, R);
     const int flag = d > 0.5f * R.skin;
     if (threadIdx.x == 0 && sp.begin == R.p0) {
         Ctl *c = a.ctl + sp.rep;
         c->flag = flag;
-        c->disp = d;
         if (flag) {
             c->nbuild += 1u;
             *a.any = 1;  // every writer stores the same value
@@ -325,10 +332,6 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
             y = 0.0f;
         }
         c = R.cell0 + cell_key(x, y, R);
-        if (a.xb) {  // list mode: the list about to be built starts here
-            a.xb[p] = x;
-            a.yb[p] = y;
-        }
     }
     bin_wave(a, active, p, c);
   }
@@ -1192,12 +1195,20 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
     __shared__ double s_red[2 * 4];
     int b = blockIdx.x;
-    // the rebuild of this step (if any) is complete: clear its summary and
-    // the displacement slots for the next kick
-    if (kList && b == 0) {
-        if (a.any && threadIdx.x == 0) *a.any = 0;
-        for (int k = threadIdx.x; k < a.nrep * kDispSlots; k += blockDim.x)
-            a.dmax[(size_t)k * kDispStride] = 0u;
+    // the rebuild of this step (if any) is complete: clear its summary,
+    // fold this step into each replica's displacement bound (zero after a
+    // rebuild) and clear the displacement slots for the next kick
+    if (kList) {
+        if (b == 0 && a.any && threadIdx.x == 0) *a.any = 0;
+        // replica r by workgroup r (the first wavefront): bound, then slots
+        if (threadIdx.x < 64) {
+            for (int r = blockIdx.x; r < a.nrep; r += gridDim.x) {
+                const float d = list_disp(a.dmax, a.ctl_w, r, a.reps[r]);
+                if (threadIdx.x == 0) a.ctl_w[r].disp = a.ctl_w[r].flag ? 0.0f : d;
+                if (threadIdx.x < kDispSlots)
+                    a.dmax[((size_t)r * kDispSlots + threadIdx.x) * kDispStride] = 0u;
+            }
+        }
     }
     if (kList) {
         if (b < a.ntiles) {
