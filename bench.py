@@ -39,7 +39,7 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     read once, each output written once; FP32 values, int32 indices).
 
     Per-step cell walk (list_len None):
-    kick     r x,y,vx,vy,fx,fy,inv_m,m,goff; w x,y,vx,vy,slot,key: 60/particle;
+    kick     r x,y,vx,vy,fx,fy,inv_m,goff; w x,y,vx,vy,slot,key: 56/particle;
              counter read-modify-write 8/cell; gather 12/corner
     reduce   r count: 4/cell
     apply    r count, w start, w count (zeroed): 12/cell
@@ -48,19 +48,22 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     forces   r sorted (16), perm; w fx,fy: 28/particle; r start: 4/cell;
              CST: r 3 ids, Dm, Bm, A0*lambda, A0*mu, 3 node x,y; w 3 corners:
              100/element; bond: r ids, k, r0, 2 node x,y; w 2 ends: 48/bond
-    Neighbour lists (list_len = mean list length L):
-    kick     as above without slot and key: 52/particle
+    Neighbour lists (list_len = mean list length L, padded to groups of 4:
+    about L + 1.5 slots of 2 bytes):
+    kick     r x,y,vx,vy,inv_m,spos,goff, pair force (8); w x,y,vx,vy and the
+             sorted-order copy of x,y (8): 60/particle; gather 12/corner
     bin      r x,y; w slot, key: 16/particle; counters 8/cell  (rebuild steps)
-    list     r sorted (16), perm; w L entries, length: 24 + 4L/particle;
+    scatter  as above plus spos and the sorted x,y: 52/particle (rebuild steps)
+    list     r sorted (16); w slots, length: 20 + 2(L+1.5)/particle;
              r start: 4/cell                                   (rebuild steps)
-    forces   r x,y, length, L entries; w fx,fy: 20 + 4L/particle; elements
-             and bonds as above
+    forces   r sorted x,y (8), length, slots; w pair force (8):
+             20 + 2(L+1.5)/particle; r start: 4/cell; elements and bonds as above
     """
     corners = 36 * ntri + 24 * nbond
     elems = 100 * ntri + 48 * nbond
     if list_len is None:
         b = {
-            "kick": 60 * n + 8 * ncells + corners,
+            "kick": 56 * n + 8 * ncells + corners,
             "bin": 0,
             "scan_reduce": 4 * ncells,
             "scan_apply": 12 * ncells,
@@ -69,15 +72,15 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
             "forces": 28 * n + 4 * ncells + elems,
         }
     else:
-        L = float(list_len)
+        slots = 2.0 * (float(list_len) + 1.5)
         b = {
-            "kick": 52 * n + corners,
+            "kick": 60 * n + corners,
             "bin": 16 * n + 8 * ncells,
             "scan_reduce": 4 * ncells,
             "scan_apply": 12 * ncells,
-            "scatter": 40 * n + 8 * ncells,
-            "list_build": int((24 + 4 * L) * n) + 4 * ncells,
-            "forces": int((20 + 4 * L) * n) + elems,
+            "scatter": 52 * n + 8 * ncells,
+            "list_build": int((20 + slots) * n) + 4 * ncells,
+            "forces": int((20 + slots) * n) + 4 * ncells + elems,
         }
     return b
 
