@@ -39,12 +39,10 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     read once, each output written once; FP32 values, int32 indices).
 
     Per-step cell walk (list_len None):
-    kick     r x,y,vx,vy,fx,fy,inv_m,goff; w x,y,vx,vy,slot,key: 56/particle;
+    kick     r x,y,vx,vy,fx,fy,inv_m,goff; w x,y,vx,vy,slot: 52/particle;
              counter read-modify-write 8/cell; gather 12/corner
-    reduce   r count: 4/cell
-    apply    r count, w start, w count (zeroed): 12/cell
-    scatter  r key, own slot entry, x,y,group; w perm, sorted (16): 40/particle;
-             r start (2 entries): 8/cell
+    sort     r count, w count (zeroed), w start: 12/cell; r own slot entry,
+             x,y,group; w perm, sorted (16): 36/particle
     forces   r sorted (16), perm; w fx,fy: 28/particle; r start: 4/cell;
              CST: r 3 ids, Dm, Bm, A0*lambda, A0*mu, 3 node x,y; w 3 corners:
              100/element; bond: r ids, k, r0, 2 node x,y; w 2 ends: 48/bond
@@ -52,8 +50,8 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     about L + 1.5 slots of 2 bytes):
     kick     r x,y,vx,vy,inv_m,spos,goff, pair force (8); w x,y,vx,vy and the
              sorted-order copy of x,y (8): 60/particle; gather 12/corner
-    bin      r x,y; w slot, key: 16/particle; counters 8/cell  (rebuild steps)
-    scatter  as above plus spos and the sorted x,y: 52/particle (rebuild steps)
+    bin      r x,y; w slot: 12/particle; counters 8/cell     (rebuild steps)
+    sort     as above plus spos and the sorted x,y: 48/particle (rebuild steps)
     list     r sorted (16); w slots, length: 20 + 2(L+1.5)/particle;
              r start: 4/cell                                   (rebuild steps)
     forces   r sorted x,y (8), length, slots; w pair force (8):
@@ -63,11 +61,9 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     elems = 100 * ntri + 48 * nbond
     if list_len is None:
         b = {
-            "kick": 56 * n + 8 * ncells + corners,
+            "kick": 52 * n + 8 * ncells + corners,
             "bin": 0,
-            "scan_reduce": 4 * ncells,
-            "scan_apply": 12 * ncells,
-            "scatter": 40 * n + 8 * ncells,
+            "sort": 36 * n + 12 * ncells,
             "list_build": 0,
             "forces": 28 * n + 4 * ncells + elems,
         }
@@ -75,17 +71,15 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
         slots = 2.0 * (float(list_len) + 1.5)
         b = {
             "kick": 60 * n + corners,
-            "bin": 16 * n + 8 * ncells,
-            "scan_reduce": 4 * ncells,
-            "scan_apply": 12 * ncells,
-            "scatter": 52 * n + 8 * ncells,
+            "bin": 12 * n + 8 * ncells,
+            "sort": 48 * n + 12 * ncells,
             "list_build": int((20 + slots) * n) + 4 * ncells,
             "forces": int((20 + slots) * n) + 4 * ncells + elems,
         }
     return b
 
 
-REBUILD_KERNELS = ("bin", "scan_reduce", "scan_apply", "scatter", "list_build")
+REBUILD_KERNELS = ("bin", "sort", "list_build")
 
 
 def measured_traffic(kernel: str, mode: str):

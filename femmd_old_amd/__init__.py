@@ -314,10 +314,9 @@ class System:
 
     def profile(self, nsteps: int = 10) -> dict:
         """Per-step device ms of each kernel (HIP events, engine stream)."""
-        ms = (C.c_double * 8)()
+        ms = (C.c_double * 6)()
         self._check(self._lib.fm_profile_step(self._h, int(nsteps), ms))
-        return dict(zip(("kick", "bin", "scan_reduce", "scan_apply", "scatter", "list_build",
-                         "forces", "step"), list(ms)))
+        return dict(zip(("kick", "bin", "sort", "list_build", "forces", "step"), list(ms)))
 
     def set_skin(self, skin: float):
         """Neighbour-list skin (default 0.6; the reference's CPU list uses 0.4,

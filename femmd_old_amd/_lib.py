@@ -24,6 +24,7 @@ FM_DEV_TILE_OVERFLOW = 2
 FM_DEV_INVERTED = 4
 FM_DEV_NONFINITE = 8
 FM_DEV_LIST_OVERFLOW = 16
+FM_DEV_SORT_STALL = 32
 
 FM_NEO_HOOKEAN = 0
 FM_SVK = 1

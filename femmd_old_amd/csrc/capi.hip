@@ -410,7 +410,7 @@ int fm_time_forces(fm_system *h, int32_t reps, double *ms) {
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms) {
     FM_GUARD(h)
     FM_NEED_ENGINE(h);
-    if (nsteps <= 0 || !ms) return fail(h, FM_ERR_ARG, "nsteps > 0 and ms[8] required");
+    if (nsteps <= 0 || !ms) return fail(h, FM_ERR_ARG, "nsteps > 0 and ms[6] required");
     return h->eng->profile(nsteps, ms, h->err);
     FM_END(h)
 }

@@ -16,9 +16,8 @@ namespace fm {
 // Largest tile row (cells) and its slot table: 3 rows of (W + 2) cells.
 constexpr int kMaxTileW = 64;
 constexpr int kMaxSlots = 3 * (kMaxTileW + 2);
-// Cells scanned per workgroup of the counting-sort scan (256 threads x 4).
-constexpr int kScanCells = 1024;
-constexpr int kScanThreads = 256;
+// Cells per workgroup of the single-pass counting sort (one per thread).
+constexpr int kSortCells = 256;
 constexpr int kKickThreads = 256;
 constexpr int kElemThreads = 256;
 constexpr int kMaxPairTypes = 4;  // ntypes <= 4 on the typed path
@@ -86,7 +85,6 @@ struct KickArgs {
     const float *mass, *inv_m;
     int32_t *count;            // per-cell counters (zero on entry)
     int32_t *tmp;              // cell * cap + slot -> particle
-    int32_t *key;              // particle -> global cell (-1 = dropped)
     int32_t cap;
     double *ke_part;           // 2 per workgroup: sum m v^2, max v^2
     uint32_t *err;
@@ -101,21 +99,22 @@ struct KickArgs {
     int32_t nspans;            // spans (grid-stride binning)
 };
 
+// Single-pass counting sort: workgroup b scans cells [b * kSortCells, ...),
+// takes its carry-in from its predecessors' status words (decoupled
+// look-back) and scatters the particles of its own cells.
 struct ScanArgs {
     const RepDesc *reps;
     int32_t nrep;
     int32_t ncells;
     int32_t cap;
-    int32_t n;                 // particles
-    int32_t *count;            // per-cell counters, zeroed once scanned
-    const Span *spans;         // kick spans (one scatter workgroup each)
-    int32_t nspans;
+    int32_t nsort;             // sort workgroups per launch
+    int32_t *count;            // per-cell counters, zeroed once read
+    const int32_t *tmp;        // cell * cap + slot -> particle
+    unsigned long long *status;  // nsort status words (value, state, launch)
+    unsigned long long *ticket;  // workgroup tickets, never reset
     const Ctl *ctl;            // list mode gating (null: always run)
     const int32_t *any;
-    const int32_t *key;
-    const int32_t *tmp;
-    int32_t *bsum;             // per scan workgroup totals
-    int32_t *start;            // ncells + 1
+    int32_t *start;            // per cell: first sorted position (replica based)
     int32_t *perm;             // sorted position -> particle
     const float *x, *y;
     const int32_t *grp;        // exclusion group per particle
@@ -126,6 +125,7 @@ struct ScanArgs {
     int32_t *spos;             // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order
     float *sigs;               // list mode: sigma in sorted order (may be null)
+    uint32_t *err;
 };
 
 struct ForceArgs {
@@ -167,6 +167,7 @@ struct ForceArgs {
     int32_t *nnum;             // list length per sorted position
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set
+    int32_t gated_grid;        // list build grid when gated (resident workgroups)
     const float2 *xys;         // list mode: positions in sorted order
     float2 *fs;                // list mode: pair forces in sorted order
     const float *sigs;         // list mode: sigma in sorted order (sigma path)

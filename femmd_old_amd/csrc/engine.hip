@@ -60,6 +60,7 @@ std::string dev_error_text(uint32_t e) {
     if (e & FM_DEV_INVERTED) s += " inverted element (det F <= 0);";
     if (e & FM_DEV_NONFINITE) s += " non-finite coordinate;";
     if (e & FM_DEV_LIST_OVERFLOW) s += " neighbour list capacity exceeded;";
+    if (e & FM_DEV_SORT_STALL) s += " counting sort look-back timed out;";
     return s;
 }
 
@@ -395,10 +396,10 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(outx_, N);
     FM_AL(outy_, N);
     FM_AL(count_, (size_t)E->ncells_);
-    FM_AL(key_, N);
     FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
-    const int nscan = (E->ncells_ + kScanCells - 1) / kScanCells;
-    FM_AL(bsum_, std::max(nscan, 1));
+    E->nsort_ = (E->ncells_ + kSortCells - 1) / kSortCells;
+    FM_AL(sort_status_, std::max(E->nsort_, 1));
+    FM_AL(sort_ticket_, 1);
     FM_AL(start_, E->ncells_ + 1);
     FM_AL(perm_, N);
     FM_AL(sorted_, N);
@@ -422,6 +423,11 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     }
 #undef FM_AL
     if ((he = hipMemsetAsync(E->count_.p, 0, sizeof(int32_t) * E->ncells_, st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    // status words of launch 0 must not read as valid: state 0
+    if ((he = hipMemsetAsync(E->sort_status_.p, 0, E->sort_status_.n * sizeof(unsigned long long),
+                             st)) != hipSuccess ||
+        (he = hipMemsetAsync(E->sort_ticket_.p, 0, sizeof(unsigned long long), st)) != hipSuccess)
         return E->check(he, "memset", err);
     // every sorted slot holds a valid particle index even where a cell
     // overflow left it unwritten (the list pass reads perm over whole ranges)
@@ -511,6 +517,7 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
     const int hard = (types_ ? 2560 : 3072);  // dynamic LDS stays <= 64 KB
     lds_cap_ = std::max(256, std::min(cap, hard));
     hipError_t he;
+    if (list_) lb_grid_ = list_build_resident(sigma_, block_, force_args(), device_);
     if ((he = upload(tiles_, tiles_h_, stream_)) != hipSuccess) return check(he, "tiles", err);
     if ((he = upload(rng_, rng_h_, stream_)) != hipSuccess) return check(he, "ranges", err);
     if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
@@ -570,7 +577,6 @@ int Engine::launch_bin(bool gated) {
     a.x = xcur();
     a.y = ycur();
     a.count = count_.p;
-    a.key = key_.p;
     a.tmp = tmp_.p;
     a.cap = cap_;
     a.err = err_.p;
@@ -581,30 +587,6 @@ int Engine::launch_bin(bool gated) {
         a.dmax = dmax_.p;
     }
     return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
-}
-
-static ScanArgs make_scan(int n, int ncells, int cap, int32_t *count, const int32_t *key,
-                          int32_t *tmp, int32_t *bsum,
-                          int32_t *start, int32_t *perm, float *x, float *y, int32_t *grp,
-                          float *sig, int32_t *typ, float4 *sorted, int32_t *sorted_typ) {
-    ScanArgs s{};
-    s.ncells = ncells;
-    s.n = n;
-    s.cap = cap;
-    s.count = count;
-    s.key = key;
-    s.tmp = tmp;
-    s.bsum = bsum;
-    s.start = start;
-    s.perm = perm;
-    s.x = x;
-    s.y = y;
-    s.grp = grp;
-    s.sig = sig;
-    s.typ = typ;
-    s.sorted = sorted;
-    s.sorted_typ = sorted_typ;
-    return s;
 }
 
 // kick2 / drift mode of one step body followed by binning, sort and forces
@@ -625,7 +607,6 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.mass = mass_.p;
     k.inv_m = inv_m_.p;
     k.count = count_.p;
-    k.key = key_.p;
     k.tmp = tmp_.p;
     k.cap = cap_;
     k.ke_part = ke_part_.p;
@@ -652,17 +633,28 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     return launch_pair_pass();
 }
 
-// Counting sort of the last binning (scan reduce, scan apply, scatter).
+// Counting sort of the last binning (single pass: scan and scatter).
 int Engine::launch_sort(bool gated) {
-    const int nscan = (ncells_ + kScanCells - 1) / kScanCells;
-    ScanArgs s = make_scan(N_, ncells_, cap_, count_.p, key_.p, tmp_.p, bsum_.p, start_.p,
-                           perm_.p, xcur(), ycur(), grp_.p, sigma_ ? sig_.p : nullptr,
-                           types_ ? typ_.p : nullptr, sorted_.p,
-                           types_ ? sorted_typ_.p : nullptr);
+    ScanArgs s{};
     s.reps = reps_.p;
-    s.spans = kspans_.p;
-    s.nspans = (int)kspans_h_.size();
     s.nrep = R_;
+    s.ncells = ncells_;
+    s.cap = cap_;
+    s.nsort = nsort_;
+    s.count = count_.p;
+    s.tmp = tmp_.p;
+    s.status = sort_status_.p;
+    s.ticket = sort_ticket_.p;
+    s.start = start_.p;
+    s.perm = perm_.p;
+    s.x = xcur();
+    s.y = ycur();
+    s.grp = grp_.p;
+    s.sig = sigma_ ? sig_.p : nullptr;
+    s.typ = types_ ? typ_.p : nullptr;
+    s.sorted = sorted_.p;
+    s.sorted_typ = types_ ? sorted_typ_.p : nullptr;
+    s.err = err_.p;
     if (list_) {
         s.spos = spos_.p;
         s.xys = xys_.p;
@@ -672,10 +664,8 @@ int Engine::launch_sort(bool gated) {
         s.ctl = ctl_.p;
         s.any = any_.p;
     }
-    for (int part = 0; part < 3; ++part) {
-        if (launch_scan_part(part, nscan, s, stream_) != hipSuccess) return -1;
-        if (prof_ev_) (void)hipEventRecord(prof_ev_[3 + part], stream_);
-    }
+    if (::fm::launch_sort(s, stream_) != hipSuccess) return -1;
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[3], stream_);
     return 0;
 }
 
@@ -684,7 +674,7 @@ int Engine::launch_list_build(bool gated) {
     ForceArgs f = force_args();
     if (gated) f.ctl = ctl_.p;
     const hipError_t e = ::fm::launch_list_build(sigma_, block_, f, stream_);
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[6], stream_);
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);
     return e == hipSuccess ? 0 : -1;
 }
 
@@ -700,7 +690,7 @@ int Engine::launch_pair_pass() {
 int Engine::launch_forces_only(bool debug) {
     if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);  // binning is in the kick
     if (launch_sort()) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[6], stream_);  // no list build
+    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);  // no list build
     const ForceArgs f = force_args();
     const hipError_t e =
         launch_forces(sigma_, types_, debug, false, energy_ || debug, block_, f, stream_);
@@ -749,6 +739,7 @@ ForceArgs Engine::force_args() {
     // stage list entries in LDS when the tile and the buffer fit in 64 KB
     // (static LDS of the kernel: ~2 KB)
     f.lbuf = list_build_lds_bytes(lds_cap_, nbr_cap_, block_, true) <= 62 * 1024;
+    f.gated_grid = lb_grid_;
     f.xys = xys_.p;
     f.fs = fs_.p;
     f.sigs = sigs_.p;
@@ -766,10 +757,10 @@ ForceArgs Engine::force_args() {
 }
 
 // Per-kernel device time of nsteps step bodies (no graph): HIP events on
-// the engine's stream around each launch.  out[0..7] = kick (+ bin in cell
-// mode), bin, scan_reduce, scan_apply, scatter, list build, forces, whole
-// body, in ms per step (list mode: the rebuild kernels are gated, so their
-// time is the average over rebuild and skip steps).
+// the engine's stream around each launch.  out[0..5] = kick (+ bin in cell
+// mode), bin, sort, list build, forces, whole body, in ms per step (list
+// mode: the rebuild kernels are gated, so their time is the average over
+// rebuild and skip steps).
 int Engine::profile(int nsteps, double *out, std::string &err) {
     if (nsteps <= 0) { err = "nsteps must be > 0"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);

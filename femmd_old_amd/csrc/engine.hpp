@@ -176,7 +176,10 @@ private:
     DBuf<PairEntry> ptab_;
     DBuf<float> x_, y_, vx_, vy_, fx_, fy_, mass_, inv_m_, sig_;
     DBuf<int32_t> grp_, typ_, uid_;
-    DBuf<int32_t> count_, tmp_, bsum_, start_, perm_, sorted_typ_, key_;
+    DBuf<int32_t> count_, tmp_, start_, perm_, sorted_typ_;
+    DBuf<unsigned long long> sort_status_, sort_ticket_;
+    int nsort_ = 0;
+    int lb_grid_ = 0;  // gated list-build grid (resident workgroups)
     DBuf<float4> sorted_;
     DBuf<Span> kspans_, espans_, bspans_;
     DBuf<int32_t> nbr_, nnum_, any_;
@@ -197,9 +200,9 @@ private:
     DBuf<int32_t> dbg_cnt_;
     DBuf<long long> dbg_s1_, dbg_s2_;
 
-    // profiling: events before the kick and after kick, bin, scan reduce,
-    // scan apply, scatter, list build and forces (set only while profiling)
-    static constexpr int kProfEvents = 8;
+    // profiling: events before the kick and after kick, bin, sort, list
+    // build and forces (set only while profiling)
+    static constexpr int kProfEvents = 6;
     hipEvent_t *prof_ev_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
@@ -219,12 +222,12 @@ private:
 
 // launchers (kernels.hip)
 hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a, hipStream_t s);
-hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s);
-hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s);
+hipError_t launch_sort(const ScanArgs &a, hipStream_t s);
 hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s);
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
 size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered);
+int list_build_resident(bool sigma, int block, const ForceArgs &a, int device);
 hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,

@@ -137,13 +137,8 @@ __device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, 
     base = __shfl(base, h);
     if (active) {
         const int slot = base + (lane - h);
-        if (slot < a.cap) {
-            a.tmp[(size_t)c * a.cap + slot] = p;
-            a.key[p] = c;
-        } else {
-            atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW
-            a.key[p] = -1;
-        }
+        if (slot < a.cap) a.tmp[(size_t)c * a.cap + slot] = p;
+        else atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW (the particle is dropped)
     }
 }
 
@@ -339,125 +334,158 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
 
 
 // ---------------------------------------------------------------------------
-// Counting sort over all cells of all replicas: per-workgroup cell totals,
-// exclusive scan into cell_start, then a particle-parallel scatter.
+// Counting sort (femmd2D/src/nn.cpp:8-20 cell fill order, re-designed as a
+// single pass).  Workgroup b (taken in launch order from a ticket counter)
+// owns cells [b * kSortCells, (b + 1) * kSortCells): it reads and clears
+// their capped counts, scans them, and publishes a status word before it
+// looks back: an inclusive prefix at once when the block holds the first
+// cell of a replica (every replica's sorted positions start at its own p0,
+// so the scan restarts there), else its aggregate, replaced by the inclusive
+// prefix once its look-back is done.  Look-back by the first wavefront
+// reads up to 64 predecessors at a time.  Each status word is one 64-bit
+// relaxed atomic (value, state, launch epoch), so no fences are needed;
+// the cell starts and the scattered particles are read by later kernels
+// only.  A workgroup waits only on lower tickets, whose workgroups are
+// already running and publish before they wait.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanArgs a) {
-    __shared__ int red[kScanThreads / 64];
-    if (a.any && !*a.any) return;  // list mode, no replica rebuilds
-    const int base = blockIdx.x * kScanCells + 4 * (int)threadIdx.x;
-    int s = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        int c = base + q;
-        if (c < a.ncells) s += min(a.count[c], a.cap);
-    }
-    s = wave_sum_int(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < kScanThreads / 64; ++k) t += red[k];
-        a.bsum[blockIdx.x] = t;
-    }
+constexpr unsigned long long kLbAggregate = 1ull, kLbInclusive = 2ull;
+constexpr int kLbSpinLimit = 1 << 22;
+
+__device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, unsigned long long state,
+                                                      int v) {
+    return ((unsigned long long)(epoch & 0x3fffffffu) << 34) | (state << 32) | (uint32_t)v;
 }
 
-// Exclusive scan of the (capped) cell counts -> cell_start; each counter is
-// zeroed once read (the scatter takes counts from cell_start), ready for
-// the next binning.
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArgs a) {
-    __shared__ int red[kScanThreads / 64];
-    __shared__ int wsum[kScanThreads / 64];
-    __shared__ int s_pre;
-    if (a.any && !*a.any) return;
+__device__ __forceinline__ int rep_of_cell(const ScanArgs &a, int c) {
+    int lo = 0, hi = a.nrep;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.reps[mid].cell0 <= c) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
+    __shared__ int s_ex[kSortCells + 1];  // local exclusive prefix of the counts
+    __shared__ int s_st[kSortCells];      // cell starts
+    __shared__ int s_w[kSortCells / 64];
+    __shared__ unsigned long long s_tk;
+    __shared__ int s_carry, s_head;
+    if (a.any && !*a.any) return;  // list mode, no replica rebuilds (uniform)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    int pre = 0;
-    for (int k = t; k < (int)blockIdx.x; k += kScanThreads) pre += a.bsum[k];
-    pre = wave_sum_int(pre);
-    if (lane == 0) red[w] = pre;
-    const int base = blockIdx.x * kScanCells + 4 * t;
-    int n[4];
-    int tsum = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        int c = base + q;
-        n[q] = (c < a.ncells) ? min(a.count[c], a.cap) : 0;
-        tsum += n[q];
-    }
-    int incl = wave_incl_scan(tsum);
-    if (lane == 63) wsum[w] = incl;
+    if (t == 0) s_tk = atomicAdd(a.ticket, 1ull);
     __syncthreads();
-    if (t == 0) {
-        int p = 0;
-        for (int k = 0; k < kScanThreads / 64; ++k) p += red[k];
-        s_pre = p;
+    const unsigned long long tk = s_tk;
+    const int b = (int)(tk % (unsigned long long)a.nsort);
+    const uint32_t epoch = (uint32_t)(tk / (unsigned long long)a.nsort);
+    const int c0 = b * kSortCells, c = c0 + t;
+    int n = 0;
+    if (c < a.ncells) {
+        n = min(a.count[c], a.cap);
+        a.count[c] = 0;  // ready for the next binning
+    }
+    const int r = rep_of_cell(a, min(c, a.ncells - 1));
+    const RepDesc &R = a.reps[r];
+    const int hl = R.cell0 - c0;  // replica's first cell, local (< 0: earlier block)
+    // block exclusive scan
+    const int incl = wave_incl_scan(n);
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    int wb = 0;
+    for (int k = 0; k < w; ++k) wb += s_w[k];
+    const int ex = wb + incl - n;
+    s_ex[t] = ex;
+    if (t == kSortCells - 1) s_ex[kSortCells] = ex + n;
+    __syncthreads();
+    const int tot = s_ex[kSortCells];
+    // publish (the last cell's replica decides: its first cell in this block
+    // means the block's end value is known without look-back)
+    if (t == kSortCells - 1) {
+        s_head = hl >= 0;
+        const unsigned long long v = (hl >= 0)
+            ? lb_word(epoch, kLbInclusive, R.p0 + tot - s_ex[hl])
+            : lb_word(epoch, kLbAggregate, tot);
+        __hip_atomic_store(a.status + b, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // look-back, needed when the block's first cell continues a replica
+    // begun in an earlier block (uniform: read from the first cell)
+    if (t == 0) s_carry = (hl >= 0) ? -1 : 0;
+    __syncthreads();
+    const bool need = s_carry == 0;
+    if (need && w == 0) {
+        int carry = 0;
+        int j = b - 1;
+        int spins = 0;
+        while (true) {
+            const int q = j - lane;
+            unsigned long long v = 0;
+            if (q >= 0) v = __hip_atomic_load(a.status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool cur = (uint32_t)(v >> 34) == (epoch & 0x3fffffffu);
+            const unsigned long long st = (v >> 32) & 3ull;
+            // q < 0 cannot be reached before an inclusive word (block 0
+            // always holds replica 0's first cell); treated as inclusive 0
+            const bool valid = (q < 0) || (cur && st != 0ull);
+            const bool inc = (q < 0) || (cur && st == kLbInclusive);
+            const unsigned long long im = __ballot(inc), vm = __ballot(valid);
+            const int k = im ? (__ffsll(im) - 1) : 63;
+            const unsigned long long want = (k == 63) ? ~0ull : ((2ull << k) - 1ull);
+            if ((vm & want) != want) {
+                if (++spins > kLbSpinLimit) {  // never expected: report, end the wait
+                    if (lane == 0) atomicOr(a.err, 32u);  // FM_DEV_SORT_STALL
+                    break;
+                }
+                continue;
+            }
+            carry += wave_sum_int((lane <= k && q >= 0) ? (int)(uint32_t)v : 0);
+            if (im) break;
+            j -= 64;
+        }
+        if (lane == 0) {
+            s_carry = carry;
+            if (!s_head)  // a block holding a replica's first cell published already
+                __hip_atomic_store(a.status + b, lb_word(epoch, kLbInclusive, carry + tot),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     __syncthreads();
-    int wbase = 0;
-    for (int k = 0; k < w; ++k) wbase += wsum[k];
-    int run = s_pre + wbase + incl - tsum;
-    // list mode: only the rebuilding replicas' cell starts change (the
-    // others' lists index staging laid out by their own last starts)
-    int rep = 0;
-    if (a.ctl && base < a.ncells) {
-        int lo = 0, hi = a.nrep;
+    const int st = (hl >= 0) ? R.p0 + ex - s_ex[hl] : s_carry + ex;
+    s_st[t] = st;
+    if (c < a.ncells) {
+        // list mode: only the rebuilding replicas' cell starts change (the
+        // others' lists index staging laid out by their own last starts)
+        if (!a.ctl || a.ctl[r].flag) a.start[c] = st;
+        if (c == a.ncells - 1) a.start[a.ncells] = st + n;
+    }
+    __syncthreads();
+    // scatter: the block's particles, one per lane; its sorted position is
+    // the cell start + its rank among the cell's members by particle index
+    // (deterministic in-cell order)
+    for (int q = t; q < tot; q += kSortCells) {
+        int lo = 0, hi = kSortCells;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (a.reps[mid].cell0 <= base) lo = mid; else hi = mid;
+            if (s_ex[mid] <= q) lo = mid; else hi = mid;
         }
-        rep = lo;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = base + q;
-        if (c >= a.ncells) break;
-        if (a.ctl) {
-            while (rep + 1 < a.nrep && a.reps[rep + 1].cell0 <= c) ++rep;
-            if (a.ctl[rep].flag) a.start[c] = run;
-        } else {
-            a.start[c] = run;
+        const int m = s_ex[lo + 1] - s_ex[lo];
+        const int *src = a.tmp + (size_t)(c0 + lo) * a.cap;
+        const int i = src[q - s_ex[lo]];
+        int rank = 0;
+        for (int k = 0; k < m; ++k) rank += (src[k] < i);
+        const int pos = s_st[lo] + rank;
+        a.perm[pos] = i;
+        float4 e;
+        e.x = a.x[i];
+        e.y = a.y[i];
+        e.z = __int_as_float(a.grp[i]);
+        e.w = a.sig ? a.sig[i] : 0.0f;
+        a.sorted[pos] = e;
+        if (a.typ) a.sorted_typ[pos] = a.typ[i];
+        if (a.spos) {  // list mode: the sorted-order state the list pass reads
+            a.spos[i] = pos;
+            a.xys[pos] = make_float2(e.x, e.y);
+            if (a.sigs) a.sigs[pos] = e.w;
         }
-        a.count[c] = 0;
-        run += n[q];
-        if (c == a.ncells - 1) a.start[a.ncells] = run;
     }
-}
-
-// One thread per particle: its sorted position is cell_start + its rank
-// among the cell's members by particle index (the in-cell order of
-// femmd2D/src/nn.cpp:8-20, deterministic); positions, exclusion group and
-// sigma are copied into sorted order for the force pass.
-__global__ __launch_bounds__(kScanThreads) void k_scatter(ScanArgs a) {
-  for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
-    const Span sp = a.spans[b];
-    if (a.ctl && !a.ctl[sp.rep].flag) continue;  // list mode, no rebuild
-    const int i = sp.begin + (int)threadIdx.x;
-    if (i >= sp.end) continue;
-    const int c = a.key[i];
-    if (c < 0) continue;  // dropped by a cell overflow (flagged)
-    const RepDesc &R = a.reps[sp.rep];
-    const int st = a.start[c];
-    const int m = a.start[c + 1] - st;
-    const int *src = a.tmp + (size_t)c * a.cap;
-    int rank = 0;
-    for (int k = 0; k < m; ++k) rank += (src[k] < i);
-    // sorted positions of a replica stay in its own range [p0, p1) even
-    // when other replicas were not binned (gated list rebuilds)
-    const int pos = (st - a.start[R.cell0]) + R.p0 + rank;
-    a.perm[pos] = i;
-    float4 e;
-    e.x = a.x[i];
-    e.y = a.y[i];
-    e.z = __int_as_float(a.grp[i]);
-    e.w = a.sig ? a.sig[i] : 0.0f;
-    a.sorted[pos] = e;
-    if (a.typ) a.sorted_typ[pos] = a.typ[i];
-    if (a.spos) {  // list mode: the sorted-order state the list pass reads
-        a.spos[i] = pos;
-        a.xys[pos] = make_float2(e.x, e.y);
-        if (a.sigs) a.sigs[pos] = e.w;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -661,7 +689,6 @@ __device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R,
                             PairLds &L) {
     const int row = W + 2, S = 3 * row;
     const int tid = threadIdx.x;
-    const int base = R.p0 - a.start[R.cell0];  // see k_scatter
     for (int s = tid; s < S; s += blockDim.x) {
         const int r = s / row, col = s - r * row;
         int cy = T.cy + r - 1;
@@ -670,7 +697,7 @@ __device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R,
         if (cx < 0) cx += R.ncx; else if (cx >= R.ncx) cx -= R.ncx;
         const int c = R.cell0 + cy * R.ncx + cx;
         const int st = a.start[c];
-        L.cstart[s] = st + base;
+        L.cstart[s] = st;
         L.off[s + 1] = a.start[c + 1] - st;  // count, scanned below
     }
     for (int k = tid; k < R.ntypes * R.ntypes; k += blockDim.x) L.pt[k] = a.ptab[R.pt + k];
@@ -1757,25 +1784,10 @@ hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
     return hipGetLastError();
 }
 
-hipError_t launch_scan_part(int part, int nblk, const ScanArgs &a, hipStream_t s) {
-    if (nblk <= 0) return hipSuccess;
-    if (part == 0) {
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(kScanThreads), 0, s, a);
-    } else if (part == 1) {
-        hipLaunchKernelGGL(k_scan_apply, dim3(nblk), dim3(kScanThreads), 0, s, a);
-    } else {
-        const int g = a.ctl ? std::min(a.nspans, kGatedGrid) : a.nspans;
-        if (g > 0) hipLaunchKernelGGL(k_scatter, dim3(g), dim3(kScanThreads), 0, s, a);
-    }
+hipError_t launch_sort(const ScanArgs &a, hipStream_t s) {
+    if (a.nsort <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort, dim3(a.nsort), dim3(kSortCells), 0, s, a);
     return hipGetLastError();
-}
-
-hipError_t launch_scan(int nblk, const ScanArgs &a, hipStream_t s) {
-    for (int part = 0; part < 3; ++part) {
-        hipError_t e = launch_scan_part(part, nblk, a, s);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
 }
 
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
@@ -1821,10 +1833,27 @@ size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered) 
            (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) : 0);
 }
 
+// Workgroups of the list build that fit on the device at once (the gated
+// build is launched with this grid: skipped steps cost one resident wave of
+// workgroups, and a rebuild loses nothing since the rest would only queue).
+int list_build_resident(bool sigma, int block, const ForceArgs &a, int device) {
+    const size_t lds = list_build_lds_bytes(a.lds_cap, a.nbr_cap, block, a.lbuf != 0);
+    int per_cu = 0, ncu = 0;
+    hipError_t e = sigma
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_list_build<true>, block, lds)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_list_build<false>, block, lds);
+    if (e != hipSuccess) per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        ncu = 0;
+    if (per_cu <= 0 || ncu <= 0) return kGatedGrid;
+    return per_cu * ncu;
+}
+
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
     const size_t lds = list_build_lds_bytes(a.lds_cap, a.nbr_cap, block, a.lbuf != 0);
-    const dim3 g(a.ctl ? std::min(a.ntiles, kGatedGrid) : a.ntiles);
+    const int gated = a.gated_grid > 0 ? a.gated_grid : kGatedGrid;
+    const dim3 g(a.ctl ? std::min(a.ntiles, gated) : a.ntiles);
     if (sigma)
         hipLaunchKernelGGL((k_list_build<true>), g, dim3(block), lds, s, a);
     else

@@ -49,6 +49,7 @@ typedef struct fm_batch fm_batch;   /* many independent replicas */
 #define FM_DEV_INVERTED 4u       /* an element with det F <= 0             */
 #define FM_DEV_NONFINITE 8u      /* a non-finite coordinate after a drift  */
 #define FM_DEV_LIST_OVERFLOW 16u /* more neighbours than the list capacity */
+#define FM_DEV_SORT_STALL 32u    /* the counting sort's look-back timed out */
 
 /* ---- material models (femmd2D/src/interactions.cpp:411-689) ---- */
 #define FM_NEO_HOOKEAN 0 /* calcFeaForcesCPU2, svk = 0 (reference CPU) */
@@ -173,9 +174,9 @@ int fm_pair_sets(fm_system *h, int32_t *cnt, int64_t *s1, int64_t *s2);
 /* Time one force pass on the device (ms per pass, averaged over reps). */
 int fm_time_forces(fm_system *h, int32_t reps, double *ms);
 /* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel with HIP
- * events on the engine's stream (no graph).  ms[8] = per step: kick (with
+ * events on the engine's stream (no graph).  ms[6] = per step: kick (with
  * the binning when cells are walked every step), binning (list mode),
- * scan reduce, scan apply, scatter, list build, force pass, whole step. */
+ * counting sort, list build, force pass, whole step. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
 /* Neighbour-list statistics: mean and max list length at the last build,
  * rebuilds so far (all zero when the cells are walked every step). */
