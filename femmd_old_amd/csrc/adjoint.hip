@@ -146,7 +146,7 @@ int Engine::adjoint_op(const float *wx, const float *wy, float *jx, float *jy, d
     if (rc) return rc;
     primed_ = false;  // the forward force buffers were not refreshed
     if (launch_total_force(N, jx_.p, jy_.p, cfa_.p, gather_ ? goff_.p : nullptr, gent_.p,
-                           nullptr, nullptr, outx_.p, outy_.p, stream_) != hipSuccess)
+                           outx_.p, outy_.p, stream_) != hipSuccess)
         return check(hipGetLastError(), "total", err);
     std::vector<float> bx(N), by(N);
     std::vector<double> de(dE_.n), pa(pacc_.n);

@@ -79,9 +79,9 @@ struct KickArgs {
     const Span *spans;
     float *x, *y, *vx, *vy;
     const float *fx, *fy;      // pair forces (internal order)
-    const float2 *cf;          // element / bond corner forces
+    const float2 *cf;          // element / bond corner forces, in gather order:
+                               // particle p's corners at [goff[p], goff[p + 1])
     const int32_t *goff;       // gather CSR offsets (n + 1), may be null
-    const int32_t *gent;       // gather CSR entries
     const float *mass, *inv_m;
     int32_t *count;            // per-cell counters (zero on entry)
     int32_t *tmp;              // cell * cap + slot -> particle
@@ -93,7 +93,6 @@ struct KickArgs {
     uint32_t *dmax;            // list mode: per replica kDispSlots slots of the
                                // step's max squared displacement (float bits),
                                // kDispStride words apart
-    const float2 *fs;          // list mode: pair forces in sorted order
     const int32_t *spos;       // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order (out)
     int32_t nspans;            // spans (grid-stride binning)
@@ -149,7 +148,9 @@ struct ForceArgs {
     const float2 *alm;         // A0 * lambda, A0 * mu
     int32_t model;             // 0 NH, 1 SVK (legacy is SVK with doubled moduli)
     const float *x, *y;        // current positions, internal order
-    float2 *cf;                // corner forces out
+    float2 *cf;                // corner forces out, in gather order
+    const int32_t *gpos;       // corner slot (3e + k, then bonds 2l + k) ->
+                               // its entry in the per-particle gather order
     double *fea_part;          // 2 per element workgroup: U, W
     // bond role
     const Span *bspans;
@@ -169,7 +170,6 @@ struct ForceArgs {
                                // column writes) when set
     int32_t gated_grid;        // list build grid when gated (resident workgroups)
     const float2 *xys;         // list mode: positions in sorted order
-    float2 *fs;                // list mode: pair forces in sorted order
     const float *sigs;         // list mode: sigma in sorted order (sigma path)
     const Ctl *ctl;            // list build gating (null: always build)
     int32_t *any;              // reset by the force pass after the rebuild

@@ -334,7 +334,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->bspans_h_ = bs;
 
     // gather CSR: per particle, its corner slots in ascending slot order
-    std::vector<int32_t> goff, gent;
+    std::vector<int32_t> goff, gent, gpos;
     if (E->gather_) {
         const int N = E->N_;
         std::vector<int32_t> cnt(N + 1, 0);
@@ -354,6 +354,9 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             gent[fill[hbi[l]]++] = b0 + 2 * l;
             gent[fill[hbj[l]]++] = b0 + 2 * l + 1;
         }
+        // the force pass writes each corner straight to its gather entry
+        gpos.assign(gent.size(), 0);
+        for (size_t e = 0; e < gent.size(); ++e) gpos[gent[e]] = (int32_t)e;
     }
 
     // ---- device allocations and uploads ----
@@ -388,6 +391,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_UP(br0_, hbr0);
     FM_UP(goff_, goff);
     FM_UP(gent_, gent);
+    FM_UP(gpos_, gpos);
 #undef FM_UP
 #define FM_AL(buf, cnt) \
     if ((he = E->buf.alloc(cnt)) != hipSuccess) return E->check(he, "alloc " #buf, err)
@@ -418,7 +422,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         FM_AL(dmax_, (size_t)R * kDispSlots * kDispStride);
         FM_AL(spos_, N);
         FM_AL(xys_, N);
-        FM_AL(fs_, N);
         if (E->sigma_) FM_AL(sigs_, N);
     }
 #undef FM_AL
@@ -438,7 +441,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             (he = hipMemsetAsync(E->ctl_.p, 0, sizeof(Ctl) * R, st)) != hipSuccess ||
             (he = hipMemsetAsync(E->any_.p, 0, sizeof(int32_t), st)) != hipSuccess ||
             (he = hipMemsetAsync(E->spos_.p, 0, sizeof(int32_t) * N, st)) != hipSuccess ||
-            (he = hipMemsetAsync(E->fs_.p, 0, sizeof(float2) * N, st)) != hipSuccess ||
             (he = hipMemsetAsync(E->dmax_.p, 0, E->dmax_.n * sizeof(uint32_t), st)) != hipSuccess)
             return E->check(he, "memset", err);
     }
@@ -586,7 +588,7 @@ int Engine::launch_bin(bool gated) {
         a.any = any_.p;
         a.dmax = dmax_.p;
     }
-    return launch_kick(0, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
+    return launch_kick(0, false, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
 // kick2 / drift mode of one step body followed by binning, sort and forces
@@ -603,7 +605,6 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.fy = fy_.p;
     k.cf = cf_.p;
     k.goff = gather_ ? goff_.p : nullptr;
-    k.gent = gather_ ? gent_.p : nullptr;
     k.mass = mass_.p;
     k.inv_m = inv_m_.p;
     k.count = count_.p;
@@ -613,13 +614,12 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.err = err_.p;
     if (list_) {
         k.dmax = dmax_.p;
-        k.fs = fs_.p;
         k.spos = spos_.p;
         k.xys = xys_.p;
     }
     const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
     if (prof_ev_) (void)hipEventRecord(prof_ev_[0], stream_);
-    e = launch_kick(mode, gather_, (int)kspans_h_.size(), k, stream_);
+    e = launch_kick(mode, gather_, energy_, (int)kspans_h_.size(), k, stream_);
     if (e != hipSuccess) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[1], stream_);
     if (!drift) return 0;
@@ -724,6 +724,7 @@ ForceArgs Engine::force_args() {
     f.x = xcur();
     f.y = ycur();
     f.cf = cf_.p;
+    f.gpos = gpos_.p;
     f.fea_part = fea_part_.p;
     f.bspans = bspans_.p;
     f.nbspans = (int)bspans_h_.size();
@@ -741,7 +742,6 @@ ForceArgs Engine::force_args() {
     f.lbuf = list_build_lds_bytes(lds_cap_, nbr_cap_, block_, true) <= 62 * 1024;
     f.gated_grid = lb_grid_;
     f.xys = xys_.p;
-    f.fs = fs_.p;
     f.sigs = sigs_.p;
     f.any = list_ ? any_.p : nullptr;
     f.dmax = dmax_.p;
@@ -904,7 +904,7 @@ int Engine::compute_forces(std::string &err) {
     k.vy = vy_.p;
     k.mass = mass_.p;
     k.ke_part = ke_part_.p;
-    if (launch_kick(4, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
+    if (launch_kick(4, false, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
         return check(hipGetLastError(), "ke", err);
     return sync(err);
 }
@@ -1000,9 +1000,9 @@ int Engine::get_forces(int r, float *fx, float *fy, std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
     if (rc) return rc;
-    if (launch_total_force(N_, fx_.p, fy_.p, cf_.p, gather_ ? goff_.p : nullptr, gent_.p,
-                           list_ ? spos_.p : nullptr, fs_.p, outx_.p, outy_.p,
-                           stream_) != hipSuccess)
+    // corner forces are held in gather order (no entry table)
+    if (launch_total_force(N_, fx_.p, fy_.p, cf_.p, gather_ ? goff_.p : nullptr, nullptr,
+                           outx_.p, outy_.p, stream_) != hipSuccess)
         return check(hipGetLastError(), "total force", err);
     const RepDesc &D = reps_h_[r];
     const int n = D.p1 - D.p0;

@@ -187,9 +187,9 @@ private:
     DBuf<float> sigs_;
     DBuf<uint32_t> dmax_;
     DBuf<int32_t> spos_;
-    DBuf<float2> xys_, fs_;
+    DBuf<float2> xys_;
     DBuf<Tile> tiles_;
-    DBuf<int32_t> t0_, t1_, t2_, bi_, bj_, goff_, gent_;
+    DBuf<int32_t> t0_, t1_, t2_, bi_, bj_, goff_, gent_, gpos_;
     DBuf<float4> dmbm_;
     DBuf<float2> alm_, cf_;
     DBuf<float> bk_, br0_;
@@ -221,7 +221,8 @@ private:
 };
 
 // launchers (kernels.hip)
-hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a, hipStream_t s);
+hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickArgs &a,
+                       hipStream_t s);
 hipError_t launch_sort(const ScanArgs &a, hipStream_t s);
 hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s);
@@ -231,8 +232,8 @@ int list_build_resident(bool sigma, int block, const ForceArgs &a, int device);
 hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
-                              const int32_t *goff, const int32_t *gent, const int32_t *spos,
-                              const float2 *fs, float *ox, float *oy, hipStream_t s);
+                              const int32_t *goff, const int32_t *gent, float *ox, float *oy,
+                              hipStream_t s);
 hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
                          const double *fea, const double *bond, double *out, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);

@@ -146,11 +146,15 @@ __device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, 
 // k_kick: integrator + binning (femmd2D/src/integratorCuda.cu:110-206,
 // femmd2D/src/nnCuda.cu:69-82 re-designed).  One workgroup covers one span
 // of one replica.
-//   kKick2: v += dt/2 a (second half of the previous step), KE partials
-//   kDrift: v += dt/2 a, x += dt v, wrap, key, bin
-// a = (f_pair + sum of gathered corner forces) * inv_m.
+//   kKick2: v += dt/2 a (second half of the previous step); with kEnergy the
+//           kinetic-energy and max v^2 partials of the workgroup
+//   kDrift: v += dt/2 a, x += dt v, wrap; bin (cell walk every step) or
+//           mirror x, y into sorted order and take the step's largest
+//           displacement (neighbour lists)
+// a = (f_pair + sum of the particle's corner forces) * inv_m; the corner
+// forces of particle p are the consecutive entries [goff[p], goff[p + 1]).
 // ---------------------------------------------------------------------------
-template <bool kKick2, bool kDrift, bool kGather, bool kList>
+template <bool kKick2, bool kDrift, bool kGather, bool kList, bool kEnergy>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     __shared__ double red[2 * (kKickThreads / 64)];
     __shared__ float redm[kKickThreads / 64];
@@ -160,22 +164,12 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     double ke = 0.0;
     float vvmax = 0.0f, d2 = 0.0f;
     int cell = 0;
-    int spi = 0;
     if (p < sp.end) {
-        float fx, fy;
-        if (kList) {
-            spi = a.spos[p];
-            const float2 f = a.fs[spi];
-            fx = f.x;
-            fy = f.y;
-        } else {
-            fx = a.fx[p];
-            fy = a.fy[p];
-        }
+        float fx = a.fx[p], fy = a.fy[p];
         if (kGather) {
             const int e0 = a.goff[p], e1 = a.goff[p + 1];
             for (int e = e0; e < e1; ++e) {
-                const float2 c = a.cf[a.gent[e]];
+                const float2 c = a.cf[e];
                 fx += c.x;
                 fy += c.y;
             }
@@ -186,12 +180,14 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         if (kKick2) {
             vx = vx + R.hdt * ax;
             vy = vy + R.hdt * ay;
-            const float vv = vx * vx + vy * vy;
-            // m from 1/m (one rounding from the stored mass; a pinned
-            // particle has inv_m = 0 and zero velocity)
-            const float m = im > 0.0f ? 1.0f / im : 0.0f;
-            ke = (double)(m * vv);
-            vvmax = vv;
+            if (kEnergy) {
+                const float vv = vx * vx + vy * vy;
+                // m from 1/m (one rounding from the stored mass; a pinned
+                // particle has inv_m = 0 and zero velocity)
+                const float m = im > 0.0f ? 1.0f / im : 0.0f;
+                ke = (double)(m * vv);
+                vvmax = vv;
+            }
         }
         if (kDrift) {
             vx = vx + R.hdt * ax;
@@ -210,7 +206,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             a.y[p] = y;
             if (!kList) cell = R.cell0 + cell_key(x, y, R);
             if (kList) {
-                a.xys[spi] = make_float2(x, y);
+                a.xys[a.spos[p]] = make_float2(x, y);
                 d2 = dxp * dxp + dyp * dyp;  // this step's displacement
             }
         }
@@ -219,16 +215,22 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     }
     if (kDrift && !kList) bin_wave(a, p < sp.end, p, cell);
     if (kDrift && kList) {
-        // one atomic max per wavefront into a slot of the replica (relaxed:
-        // the binning reads the slots after this kernel has completed)
+        // the workgroup's largest displacement, one atomic max into a slot of
+        // the replica (relaxed: the binning reads the slots after this
+        // kernel has completed)
+        __shared__ float redd[kKickThreads / 64];
         const float m = wave_max(d2);
-        if ((threadIdx.x & 63) == 0) {
-            const int slot = (blockIdx.x * (kKickThreads / 64) + (threadIdx.x >> 6)) % kDispSlots;
+        if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float mm = redd[0];
+            for (int k = 1; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redd[k]);
+            const int slot = blockIdx.x % kDispSlots;
             atomicMax(a.dmax + ((size_t)sp.rep * kDispSlots + slot) * kDispStride,
-                      __float_as_uint(m));
+                      __float_as_uint(mm));
         }
     }
-    if (kKick2) {
+    if (kKick2 && kEnergy) {
         double vm = 0.0;
         // max via a float wave reduction (exact, order independent)
         float m = wave_max(vvmax);
@@ -1038,9 +1040,9 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             term((int)((cur >> 32) & 0xffffu));
             term((int)(cur >> 48));
         }
-        fx *= pe.e48;
-        fy *= pe.e48;
-        a.fs[p] = make_float2(fx, fy);
+        const int id = a.perm[p];
+        a.fx[id] = fx * pe.e48;
+        a.fy[id] = fy * pe.e48;
         if (kEnergy) {
             const double A = accA, B = accB;
             Ublk += 0.5 * ((double)pe.e4 * (A - B) + (double)nint * pe.shift);
@@ -1086,7 +1088,8 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
 // CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
 // displacement form: with U = Ds - Dm and G = U Bm = F - I, every term of P
 // is O(|G|), so FP32 keeps relative accuracy at small strain.  Corner forces
-// go to cf[3e + k]; no atomics.
+// go to their entries of the per-particle gather order, cf[gpos[3e + k]]; no
+// atomics.
 __device__ void elem_block(const ForceArgs &a, int b, double *red) {
     const Span sp = a.espans[b];
     const RepDesc R = a.reps[sp.rep];
@@ -1160,9 +1163,9 @@ __device__ void elem_block(const ForceArgs &a, int b, double *red) {
         const float f0y = -(P10 * B.x + P11 * B.y);
         const float f1x = -(P00 * B.z + P01 * B.w);
         const float f1y = -(P10 * B.z + P11 * B.w);
-        a.cf[3 * e] = make_float2(f0x, f0y);
-        a.cf[3 * e + 1] = make_float2(f1x, f1y);
-        a.cf[3 * e + 2] = make_float2(-f0x - f1x, -f0y - f1y);
+        a.cf[a.gpos[3 * e]] = make_float2(f0x, f0y);
+        a.cf[a.gpos[3 * e + 1]] = make_float2(f1x, f1y);
+        a.cf[a.gpos[3 * e + 2]] = make_float2(-f0x - f1x, -f0y - f1y);
         U += (double)ue;
         Wv += (double)(f0x * Ds00 + f0y * Ds10) + (double)(f1x * Ds01 + f1y * Ds11);
     }
@@ -1202,8 +1205,8 @@ __device__ void bond_block(const ForceArgs &a, int b, double *red) {
         const float d = q / (r + r0);
         const float kdr = -k * d / r;
         const float fx = kdr * dx, fy = kdr * dy;
-        a.cf[a.cf_bond0 + 2 * l] = make_float2(fx, fy);
-        a.cf[a.cf_bond0 + 2 * l + 1] = make_float2(-fx, -fy);
+        a.cf[a.gpos[a.cf_bond0 + 2 * l]] = make_float2(fx, fy);
+        a.cf[a.gpos[a.cf_bond0 + 2 * l + 1]] = make_float2(-fx, -fy);
         U += 0.5 * (double)(k * d * d);
         Wv += (double)(kdr * r2);
     }
@@ -1282,23 +1285,14 @@ __global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
                                                      const float2 *cf,
                                                      const int32_t *goff,
                                                      const int32_t *gent,
-                                                     const int32_t *spos,
-                                                     const float2 *fs,
                                                      float *ox, float *oy) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    float x, y;
-    if (spos) {  // list mode: pair forces are held in sorted order
-        const float2 f = fs[spos[p]];
-        x = f.x;
-        y = f.y;
-    } else {
-        x = fx[p];
-        y = fy[p];
-    }
+    float x = fx[p], y = fy[p];
     if (goff) {
+        // gent null: cf is held in gather order
         for (int e = goff[p]; e < goff[p + 1]; ++e) {
-            const float2 c = cf[gent[e]];
+            const float2 c = cf[gent ? gent[e] : e];
             x += c.x;
             y += c.y;
         }
@@ -1749,18 +1743,23 @@ hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_loss, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
+hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
     // 4 = kinetic-energy partials only; a.dmax set = list mode (the drift
-    // measures the displacement since the list build instead of binning)
+    // measures the displacement since the list build instead of binning);
+    // energy: the final-kick part writes the kinetic-energy partials
     if (nblk <= 0) return hipSuccess;
     dim3 g(nblk), b(kKickThreads);
     const bool list = a.dmax != nullptr;
+#define FM_KICK1(K2, D, G, E) \
+    do { \
+        if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true, E>), g, b, 0, s, a); \
+        else hipLaunchKernelGGL((k_kick<K2, D, G, false, E>), g, b, 0, s, a); \
+    } while (0)
 #define FM_KICK(K2, D, G) \
     do { \
-        if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true>), g, b, 0, s, a); \
-        else hipLaunchKernelGGL((k_kick<K2, D, G, false>), g, b, 0, s, a); \
+        if (K2 && energy) FM_KICK1(K2, D, G, true); else FM_KICK1(K2, D, G, false); \
     } while (0)
     switch (mode) {
         case 0:
@@ -1771,7 +1770,7 @@ hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
             break;
         case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
         case 1:
-            if (gather) FM_KICK(false, true, true); else FM_KICK(false, true, false);
+            if (gather) FM_KICK1(false, true, true, false); else FM_KICK1(false, true, false, false);
             break;
         case 2:
             if (gather) FM_KICK(true, false, true); else FM_KICK(true, false, false);
@@ -1781,6 +1780,7 @@ hipError_t launch_kick(int mode, bool gather, int nblk, const KickArgs &a,
             break;
     }
 #undef FM_KICK
+#undef FM_KICK1
     return hipGetLastError();
 }
 
@@ -1863,12 +1863,11 @@ hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStrea
 
 hipError_t launch_total_force(int n, const float *fx, const float *fy,
                               const float2 *cf, const int32_t *goff,
-                              const int32_t *gent, const int32_t *spos,
-                              const float2 *fs, float *ox, float *oy,
+                              const int32_t *gent, float *ox, float *oy,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_total_force, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                       fx, fy, cf, goff, gent, spos, fs, ox, oy);
+                       fx, fy, cf, goff, gent, ox, oy);
     return hipGetLastError();
 }
 
