@@ -169,6 +169,7 @@ struct ForceArgs {
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set
     int32_t gated_grid;        // list build grid when gated (resident workgroups)
+    int32_t cell_cap;          // slots per cell (cell counts are capped at it)
     const float2 *xys;         // list mode: positions in sorted order
     const float *sigs;         // list mode: sigma in sorted order (sigma path)
     const Ctl *ctl;            // list build gating (null: always build)

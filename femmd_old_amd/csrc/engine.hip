@@ -739,7 +739,8 @@ ForceArgs Engine::force_args() {
     f.nnum = nnum_.p;
     // stage list entries in LDS when the tile and the buffer fit in 64 KB
     // (static LDS of the kernel: ~2 KB)
-    f.lbuf = list_build_lds_bytes(lds_cap_, nbr_cap_, block_, true) <= 62 * 1024;
+    f.lbuf = list_build_lds_bytes(lds_cap_, cap_, nbr_cap_, block_, true) <= 62 * 1024;
+    f.cell_cap = cap_;
     f.gated_grid = lb_grid_;
     f.xys = xys_.p;
     f.sigs = sigs_.p;

@@ -227,7 +227,7 @@ hipError_t launch_sort(const ScanArgs &a, hipStream_t s);
 hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s);
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
-size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered);
+size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered);
 int list_build_resident(bool sigma, int block, const ForceArgs &a, int device);
 hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);

@@ -730,25 +730,41 @@ __device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R,
     return M > a.lds_cap ? -1 : M;
 }
 
-// Calls f(e, p) for every staged entry e (LDS index) and its sorted position
-// p.  The cells of one staged row are consecutive in sorted order except
-// where the column index wraps to 0 (at most twice), so each row is copied
-// as up to three contiguous runs, lanes striding over the row's entries.
-template <class F>
-__device__ void stage_copy(const Tile &T, const RepDesc &R, int W, const PairLds &L, F &&f) {
+// Copies every staged entry e (LDS index) from its sorted position p:
+// st(e, ld(p)).  The cells of one staged row are consecutive in sorted order
+// except where the column index wraps to 0 (at most twice), so each row is
+// up to three contiguous runs; the rows themselves are consecutive in LDS.
+// Lanes stride over all M entries of the tile, kStageBatch per lane at a
+// time with every load issued before the first store.
+constexpr int kStageBatch = 4;
+template <class Ld, class St>
+__device__ void stage_copy(const Tile &T, const RepDesc &R, int W, const PairLds &L, Ld &&ld,
+                           St &&st) {
+    using V = decltype(ld(0));
     const int row = W + 2;
     // slots (within a row) where the column wraps to 0
     int b1 = (T.cx0 == 0) ? 1 : (R.ncx - T.cx0 + 1);
     int b2 = b1 + R.ncx;
     if (b1 >= row) b1 = row;
     if (b2 >= row) b2 = row;
-    for (int r = 0; r < 3; ++r) {
-        const int s0 = r * row;
-        const int e0 = L.off[s0], e1 = L.off[s0 + row];
-        const int o1 = L.off[s0 + b1], o2 = L.off[s0 + b2];
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-            const int ps = (e >= o2) ? b2 : ((e >= o1) ? b1 : 0);
-            f(e, L.cstart[s0 + ps] + (e - L.off[s0 + ps]));
+    const int r1 = L.off[row], r2 = L.off[2 * row], M = L.off[3 * row];
+    const int bd = blockDim.x;
+    for (int e0 = (int)threadIdx.x; e0 < M; e0 += kStageBatch * bd) {
+        V v[kStageBatch];
+#pragma unroll
+        for (int q = 0; q < kStageBatch; ++q) {
+            const int e = e0 + q * bd;
+            if (e < M) {
+                const int s0 = (e >= r2) ? 2 * row : ((e >= r1) ? row : 0);
+                const int o1 = L.off[s0 + b1], o2 = L.off[s0 + b2];
+                const int ps = s0 + ((e >= o2) ? b2 : ((e >= o1) ? b1 : 0));
+                v[q] = ld(L.cstart[ps] + (e - L.off[ps]));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kStageBatch; ++q) {
+            const int e = e0 + q * bd;
+            if (e < M) st(e, v[q]);
         }
     }
 }
@@ -762,13 +778,27 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
                           PairLds &L, const float2 *wsrc, float2 *lw) {
     const int M = stage_layout(a, T, R, W, L);
     if (M < 0) return -1;
-    stage_copy(T, R, W, L, [&](int e, int p) {
-        L.ent[e] = a.sorted[p];
-        if (kTypes) L.typ[e] = a.sorted_typ[p];
-        if (kDebug) L.sp[e] = p;
-        if (kPid) L.sp[e] = a.perm[p];  // internal id
-        if (wsrc) lw[e] = wsrc[a.perm[p]];
-    });
+    struct Ent {
+        float4 v;
+        int t, sp;
+        float2 w;
+    };
+    stage_copy(
+        T, R, W, L,
+        [&](int p) {
+            Ent u;
+            u.v = a.sorted[p];
+            u.t = kTypes ? a.sorted_typ[p] : 0;
+            u.sp = kDebug ? p : (kPid ? a.perm[p] : 0);  // sorted position / internal id
+            if (wsrc) u.w = wsrc[a.perm[p]];
+            return u;
+        },
+        [&](int e, const Ent &u) {
+            L.ent[e] = u.v;
+            if (kTypes) L.typ[e] = u.t;
+            if (kDebug || kPid) L.sp[e] = u.sp;
+            if (wsrc) lw[e] = u.w;
+        });
     __syncthreads();
     return M;
 }
@@ -819,19 +849,19 @@ __device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p
         (uint16_t)j;
 }
 
-template <bool kWrap, bool kSigma, bool kBuf>
-__device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
-                               uint16_t *buf) {
+// List build of one staged tile, entries written straight to the list (used
+// when the LDS holds no list buffer: one 2-byte store per entry).
+template <bool kWrap, bool kSigma>
+__device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int W, PairLds &L) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
-    const int bd = blockDim.x;  // held in a register across the loops
-    const int tid = threadIdx.x;
+    const int bd = blockDim.x;
     const int cap = a.nbr_cap;
-    for (int k = tid; k < nc; k += bd) {
+    for (int k = threadIdx.x; k < nc; k += bd) {
         const int me = c_lo + k;
         int lo = row + 1, hi = row + W + 1;
         while (hi - lo > 1) {
@@ -844,22 +874,16 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int gi = __float_as_int(pi.z);
         const int gix = gi >= 0 ? gi : -2;
         const float si = pi.w;
-        const int p = sbase + k;  // sorted position of this centre particle
+        const int p = sbase + k;
         int cnt = 0;
-        uint16_t *bp = buf + tid;  // this lane's next LDS list slot
-        // the three staged rows, each a contiguous run of candidates
-        // (cells col-1 .. col+1); the centre particle is in row 1 only
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int j0 = L.off[r * row + col - 1], j1 = L.off[r * row + col + 2];
-            float4 pn = L.ent[j0 < j1 ? j0 : me];
             for (int j = j0; j < j1; ++j) {
-                const float4 pj = pn;
-                pn = L.ent[j + 1 < j1 ? j + 1 : j];  // read one iteration ahead
+                const float4 pj = L.ent[j];
                 float dx = xi - pj.x;
                 float dy = yi - pj.y;
                 if (kWrap) {
-                    // minimum image as two selects per axis (no branches)
                     float ox = (dx >= hLx) ? Lx : 0.0f;
                     ox = (dx < -hLx) ? -Lx : ox;
                     float oy = (dy >= hLy) ? Ly : 0.0f;
@@ -878,11 +902,7 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 bool hit = (dr2 < rl2) & (__float_as_int(pj.z) != gix);
                 if (r == 1) hit &= (j != me);
                 if (hit) {
-                    if (cnt < cap) {
-                        if (kBuf) *bp = (uint16_t)j;
-                        else list_store_slot(a, cnt, p, j);
-                    }
-                    bp += bd;
+                    if (cnt < cap) list_store_slot(a, cnt, p, j);
                     ++cnt;
                 }
             }
@@ -890,9 +910,98 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
         const int m = min(cnt, cap);
         // groups of four, padded with the particle itself (distance 0: the
         // pair pass's cutoff test rejects it); cap is a multiple of 8
-        if (kBuf) {
-            // the lanes of a wavefront hold consecutive sorted positions, so
-            // each group store is contiguous
+        for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, me);
+        a.nnum[p] = m;
+        if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
+    }
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// List build of one staged tile through the LDS list buffer (slot k of lane
+// t at buf[k * blockDim + t]).  Each row's candidate loop runs a
+// wavefront-uniform trip count (the longest run among the lanes; a lane
+// past its own run is masked), and every candidate is written to the lane's
+// next buffer slot while only hits advance it, so the loop has no branches.
+// Reads past a lane's run stay inside the staging's headroom of 3 cells.
+template <bool kWrap, bool kSigma>
+__device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
+                                   uint16_t *buf) {
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const PairEntry pe = L.pt[0];
+    const int bd = blockDim.x;  // held in a register across the loops
+    const int tid = threadIdx.x;
+    const int cap = a.nbr_cap, capm1 = a.nbr_cap - 1;
+    // every lane of a wavefront takes part in each pass (uniform loop)
+    for (int k0 = 0; k0 < nc; k0 += bd) {
+        const int k = k0 + tid;
+        const bool act = k < nc;
+        const int me = c_lo + min(k, nc - 1);
+        int lo = row + 1, hi = row + W + 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= me) lo = mid; else hi = mid;
+        }
+        const int col = lo - row;
+        const float4 pi = L.ent[me];
+        const float xi = pi.x, yi = pi.y;
+        const int gi = __float_as_int(pi.z);
+        const int gix = gi >= 0 ? gi : -2;
+        const float si = pi.w;
+        int cnt = 0;
+        // the three staged rows, each a contiguous run of candidates
+        // (cells col-1 .. col+1); the centre particle is in row 1 only
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int j0 = L.off[r * row + col - 1];
+            const int len = act ? L.off[r * row + col + 2] - j0 : 0;
+            const int lmax = __builtin_amdgcn_readfirstlane(wave_max_int(len));
+#pragma unroll 2
+            for (int t = 0; t < lmax; ++t) {
+                const int j = j0 + t;
+                const float4 pj = L.ent[j];
+                float dx = xi - pj.x;
+                float dy = yi - pj.y;
+                if (kWrap) {
+                    // minimum image as two selects per axis
+                    float ox = (dx >= hLx) ? Lx : 0.0f;
+                    ox = (dx < -hLx) ? -Lx : ox;
+                    float oy = (dy >= hLy) ? Ly : 0.0f;
+                    oy = (dy < -hLy) ? -Ly : oy;
+                    dx = dx - ox;
+                    dy = dy - oy;
+                }
+                // the margins of the list radius cover the rounding of this
+                // fused form
+                const float dr2 = __builtin_fmaf(dx, dx, dy * dy);
+                float rl2;
+                if (kSigma) {
+                    const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
+                    rl2 = (rl * rl) * 1.00002f;
+                } else {
+                    rl2 = R.rl2_uniform;
+                }
+                bool hit = (t < len) & (dr2 < rl2) & (__float_as_int(pj.z) != gix);
+                if (r == 1) hit &= (j != me);
+                buf[min(cnt, capm1) * bd + tid] = (uint16_t)j;
+                cnt += hit ? 1 : 0;
+            }
+        }
+        if (act) {
+            const int p = sbase + k;  // sorted position of this centre particle
+            const int m = min(cnt, cap);
+            // groups of four, padded with the particle itself (distance 0:
+            // the pair pass's cutoff test rejects it); cap is a multiple of 8.
+            // The lanes of a wavefront hold consecutive sorted positions, so
+            // each group store is contiguous.
             for (int g = 0; g < ((m + 3) >> 2); ++g) {
                 int jj[4];
 #pragma unroll
@@ -902,11 +1011,9 @@ __device__ void list_tile_body(const ForceArgs &a, const RepDesc &R, int W, Pair
                 }
                 list_store_group(a, g, p, jj);
             }
-        } else {
-            for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, me);
+            a.nnum[p] = m;
+            if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
         }
-        a.nnum[p] = m;
-        if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
 }
 
@@ -924,7 +1031,9 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     L.off = s_off;
     L.pt = s_pt;
     L.red = nullptr;
-    uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap);
+    // list buffer after the staging and its read headroom of 3 cells
+    uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
+    if (a.ctl && a.any && !*a.any) return;  // gated, and no replica rebuilds
     for (int b = blockIdx.x; b < a.ntiles; b += gridDim.x) {
         const Tile T = a.tiles[xcd_remap(b, a.ntiles)];
         if (a.ctl && !a.ctl[T.rep].flag) continue;  // uniform per workgroup
@@ -937,11 +1046,11 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
         }
         const bool wrapT = (T.w_flags >> 16) & 1;
         if (a.lbuf) {
-            if (wrapT) list_tile_body<true, kSigma, true>(a, R, W, L, buf);
-            else list_tile_body<false, kSigma, true>(a, R, W, L, buf);
+            if (wrapT) list_tile_body_buf<true, kSigma>(a, R, W, L, buf);
+            else list_tile_body_buf<false, kSigma>(a, R, W, L, buf);
         } else {
-            if (wrapT) list_tile_body<true, kSigma, false>(a, R, W, L, buf);
-            else list_tile_body<false, kSigma, false>(a, R, W, L, buf);
+            if (wrapT) list_tile_body_direct<true, kSigma>(a, R, W, L);
+            else list_tile_body_direct<false, kSigma>(a, R, W, L);
         }
         __syncthreads();  // the next tile restages the LDS
     }
@@ -980,13 +1089,15 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
         const int n4 = (a.nnum[p] + 3) >> 2;  // padded with the particle itself
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
-        auto term = [&](int j) {
-            const float2 pj = lp[j];
+        // one list entry, without branches: the force of a non-interacting
+        // entry (or of the padding entry, the particle itself) is computed
+        // and discarded by a select
+        auto term = [&](float2 pj, float sj) {
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
             if (kWrap) {
-                // minimum image as two selects per axis (no branches)
+                // minimum image as two selects per axis
                 ox = (ex >= hLx) ? Lx : 0.0f;
                 ox = (ex < -hLx) ? -Lx : ox;
                 oy = (ey >= hLy) ? Ly : 0.0f;
@@ -998,47 +1109,61 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             float s2ij;
             unsigned rc2m1;
             if (kSigma) {
-                const float sij = (si + ls[j]) * 0.5f;
+                const float sij = (si + sj) * 0.5f;
                 rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
                 s2ij = sij * sij;
             } else {
                 rc2m1 = rc2m1_u;
                 s2ij = R.s2_uniform;
             }
-            if ((__float_as_uint(dr2) - 1u) < rc2m1) {
-                float dxa = dx, dya = dy, r2a = dr2;
-                if (kWrap && ((ox != 0.0f) | (oy != 0.0f))) {
-                    // image-exact difference: shift the coordinate first
-                    if (ox > 0.0f) dxa = xim - xj; else if (ox < 0.0f) dxa = xi - (xj - Lx);
-                    if (oy > 0.0f) dya = yim - yj; else if (oy < 0.0f) dya = yi - (yj - Ly);
-                    r2a = dxa * dxa + dya * dya;
-                }
-                const float inv = __builtin_amdgcn_rcpf(r2a);
-                const float sr2 = s2ij * inv;
-                const float sr4 = sr2 * sr2;
-                const float sr6 = sr4 * sr2;
-                const float s12 = sr6 * sr6;
-                const float tv = __builtin_fmaf(-0.5f, sr6, s12);
-                const float fc = tv * inv;
-                fx = __builtin_fmaf(fc, dxa, fx);
-                fy = __builtin_fmaf(fc, dya, fy);
-                if (kEnergy) {
-                    accA += s12;
-                    accB += sr6;
-                    ++nint;
-                }
+            const bool hit = (__float_as_uint(dr2) - 1u) < rc2m1;
+            float dxa = dx, dya = dy, r2a = dr2;
+            if (kWrap) {
+                // image-exact difference: the far coordinate is shifted by L
+                // before subtracting (same bits as dx where no shift applies)
+                dxa = (ox > 0.0f) ? (xim - xj) : ((ox < 0.0f) ? (xi - (xj - Lx)) : dx);
+                dya = (oy > 0.0f) ? (yim - yj) : ((oy < 0.0f) ? (yi - (yj - Ly)) : dy);
+                r2a = dxa * dxa + dya * dya;
+            }
+            const float inv = __builtin_amdgcn_rcpf(r2a);
+            const float sr2 = s2ij * inv;
+            const float sr4 = sr2 * sr2;
+            const float sr6 = sr4 * sr2;
+            const float s12 = sr6 * sr6;
+            const float tv = __builtin_fmaf(-0.5f, sr6, s12);
+            const float fc = hit ? tv * inv : 0.0f;
+            fx = __builtin_fmaf(fc, dxa, fx);
+            fy = __builtin_fmaf(fc, dya, fy);
+            if (kEnergy) {
+                accA += hit ? s12 : 0.0f;
+                accB += hit ? sr6 : 0.0f;
+                nint += hit ? 1 : 0;
             }
         };
-        // the next group is loaded while this one is evaluated (the group
-        // index is clamped inside the array; groups past the list are unused)
-        uint64_t gv = nb[(uint32_t)p];
-        for (int g = 0; g < n4; ++g) {
-            const uint64_t cur = gv;
-            gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)p];
-            term((int)(cur & 0xffffu));
-            term((int)((cur >> 16) & 0xffffu));
-            term((int)((cur >> 32) & 0xffffu));
-            term((int)(cur >> 48));
+        auto group = [&](uint64_t cur) {
+            const int j0 = (int)(cur & 0xffffu), j1 = (int)((cur >> 16) & 0xffffu);
+            const int j2 = (int)((cur >> 32) & 0xffffu), j3 = (int)(cur >> 48);
+            const float2 q0 = lp[j0], q1 = lp[j1], q2 = lp[j2], q3 = lp[j3];
+            float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+            if (kSigma) {
+                s0 = ls[j0];
+                s1 = ls[j1];
+                s2 = ls[j2];
+                s3 = ls[j3];
+            }
+            term(q0, s0);
+            term(q1, s1);
+            term(q2, s2);
+            term(q3, s3);
+        };
+        // two groups per iteration, both loaded before either is used (the
+        // second group index is clamped inside the array; a group past the
+        // list is loaded and not used)
+        for (int g = 0; g < n4; g += 2) {
+            const uint64_t ga = nb[(uint32_t)g * Nu + (uint32_t)p];
+            const uint64_t gb = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)p];
+            group(ga);
+            if (g + 1 < n4) group(gb);
         }
         const int id = a.perm[p];
         a.fx[id] = fx * pe.e48;
@@ -1074,10 +1199,16 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
         }
         return;
     }
-    stage_copy(T, R, W, L, [&](int e, int p) {
-        lp[e] = a.xys[p];
-        if (kSigma) ls[e] = a.sigs[p];
-    });
+    stage_copy(
+        T, R, W, L,
+        [&](int p) {
+            const float2 q = a.xys[p];
+            return make_float3(q.x, q.y, kSigma ? a.sigs[p] : 0.0f);
+        },
+        [&](int e, const float3 &u) {
+            lp[e] = make_float2(u.x, u.y);
+            if (kSigma) ls[e] = u.z;
+        });
     __syncthreads();
     if ((T.w_flags >> 16) & 1)
         list_tile_pass_body<true, kSigma, kEnergy>(a, R, W, tile, L, lp, ls);
@@ -1828,8 +1959,8 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
     return hipGetLastError();
 }
 
-size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered) {
-    return (size_t)lds_cap * sizeof(float4) +
+size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered) {
+    return (size_t)(lds_cap + 3 * cell_cap) * sizeof(float4) +
            (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) : 0);
 }
 
@@ -1837,7 +1968,7 @@ size_t list_build_lds_bytes(int lds_cap, int nbr_cap, int block, bool buffered) 
 // build is launched with this grid: skipped steps cost one resident wave of
 // workgroups, and a rebuild loses nothing since the rest would only queue).
 int list_build_resident(bool sigma, int block, const ForceArgs &a, int device) {
-    const size_t lds = list_build_lds_bytes(a.lds_cap, a.nbr_cap, block, a.lbuf != 0);
+    const size_t lds = list_build_lds_bytes(a.lds_cap, a.cell_cap, a.nbr_cap, block, a.lbuf != 0);
     int per_cu = 0, ncu = 0;
     hipError_t e = sigma
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_list_build<true>, block, lds)
@@ -1851,7 +1982,7 @@ int list_build_resident(bool sigma, int block, const ForceArgs &a, int device) {
 
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
-    const size_t lds = list_build_lds_bytes(a.lds_cap, a.nbr_cap, block, a.lbuf != 0);
+    const size_t lds = list_build_lds_bytes(a.lds_cap, a.cell_cap, a.nbr_cap, block, a.lbuf != 0);
     const int gated = a.gated_grid > 0 ? a.gated_grid : kGatedGrid;
     const dim3 g(a.ctl ? std::min(a.ntiles, gated) : a.ntiles);
     if (sigma)
