@@ -21,6 +21,8 @@ constexpr int kSortCells = 256;
 constexpr int kKickThreads = 256;
 constexpr int kElemThreads = 256;
 constexpr int kMaxPairTypes = 4;  // ntypes <= 4 on the typed path
+constexpr int kMaxListCap = 512;  // neighbour-list capacity per particle
+constexpr int kMaxListGroups = kMaxListCap / 4;
 
 // Per-replica geometry and integration constants.
 struct alignas(16) RepDesc {
@@ -162,10 +164,11 @@ struct ForceArgs {
     // neighbour-list mode (pair role over the tiles of the list build)
     int32_t nbr_cap;           // list capacity per particle (multiple of 8)
     void *nbr;                 // neighbours of the particle at sorted position
-                               // p as uint16 indices into its tile's LDS
-                               // staging, four per uint64 group, group g at
-                               // element g * n + p
-    int32_t *nnum;             // list length per sorted position
+                               // p as uint16 byte offsets of float2 slots of
+                               // its tile's LDS staging, four per uint64
+                               // group, group g at element g * n + p
+    int32_t *nnum;             // per list slot q: length | (k << 16), k the
+                               // tile index of the particle whose list it is
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set
     int32_t gated_grid;        // list build grid when gated (resident workgroups)

@@ -351,6 +351,8 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
 // already running and publish before they wait.
 // ---------------------------------------------------------------------------
 constexpr unsigned long long kLbAggregate = 1ull, kLbInclusive = 2ull;
+constexpr int kSortIds = 4096;  // slot entries a sort workgroup copies to LDS
+constexpr int kSortBatch = 4;   // particles per lane in flight in the scatter
 constexpr int kLbSpinLimit = 1 << 22;
 
 __device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, unsigned long long state,
@@ -373,6 +375,8 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ int s_w[kSortCells / 64];
     __shared__ unsigned long long s_tk;
     __shared__ int s_carry, s_head;
+    __shared__ int s_ids[kSortIds];       // the block's slot entries, cell order
+    __shared__ uint8_t s_cl[kSortIds];    // their cell (local index)
     if (a.any && !*a.any) return;  // list mode, no replica rebuilds (uniform)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_tk = atomicAdd(a.ticket, 1ull);
@@ -459,33 +463,91 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
         if (c == a.ncells - 1) a.start[a.ncells] = st + n;
     }
     __syncthreads();
-    // scatter: the block's particles, one per lane; its sorted position is
-    // the cell start + its rank among the cell's members by particle index
-    // (deterministic in-cell order)
-    for (int q = t; q < tot; q += kSortCells) {
+    // scatter: the block's particles in cell order; a particle's sorted
+    // position is its cell's start + its rank among the cell's members by
+    // particle index (deterministic in-cell order).  The block's slot
+    // entries are first gathered into LDS (loads batched per lane), so the
+    // ranks read LDS; a block with more particles than the LDS copy holds
+    // reads the slot lists from memory.
+    const bool in_lds = tot <= kSortIds;
+    auto cell_of = [&](int q) {
         int lo = 0, hi = kSortCells;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (s_ex[mid] <= q) lo = mid; else hi = mid;
         }
-        const int m = s_ex[lo + 1] - s_ex[lo];
-        const int *src = a.tmp + (size_t)(c0 + lo) * a.cap;
-        const int i = src[q - s_ex[lo]];
-        int rank = 0;
-        for (int k = 0; k < m; ++k) rank += (src[k] < i);
-        const int pos = s_st[lo] + rank;
-        a.perm[pos] = i;
-        float4 e;
-        e.x = a.x[i];
-        e.y = a.y[i];
-        e.z = __int_as_float(a.grp[i]);
-        e.w = a.sig ? a.sig[i] : 0.0f;
-        a.sorted[pos] = e;
-        if (a.typ) a.sorted_typ[pos] = a.typ[i];
-        if (a.spos) {  // list mode: the sorted-order state the list pass reads
-            a.spos[i] = pos;
-            a.xys[pos] = make_float2(e.x, e.y);
-            if (a.sigs) a.sigs[pos] = e.w;
+        return lo;
+    };
+    if (in_lds) {
+        for (int q0 = t; q0 < tot; q0 += kSortBatch * kSortCells) {
+            int v[kSortBatch], cl[kSortBatch];
+#pragma unroll
+            for (int u = 0; u < kSortBatch; ++u) {
+                const int q = q0 + u * kSortCells;
+                if (q < tot) {
+                    cl[u] = cell_of(q);
+                    v[u] = a.tmp[(size_t)(c0 + cl[u]) * a.cap + (q - s_ex[cl[u]])];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kSortBatch; ++u) {
+                const int q = q0 + u * kSortCells;
+                if (q < tot) {
+                    s_ids[q] = v[u];
+                    s_cl[q] = (uint8_t)cl[u];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int q0 = t; q0 < tot; q0 += kSortBatch * kSortCells) {
+        int ii[kSortBatch], pos[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            const int q = q0 + u * kSortCells;
+            ii[u] = -1;
+            if (q < tot) {
+                const int cl = in_lds ? (int)s_cl[q] : cell_of(q);
+                const int b = s_ex[cl], m = s_ex[cl + 1] - b;
+                int i, rank = 0;
+                if (in_lds) {
+                    i = s_ids[q];
+                    for (int k = 0; k < m; ++k) rank += (s_ids[b + k] < i);
+                } else {
+                    const int *src = a.tmp + (size_t)(c0 + cl) * a.cap;
+                    i = src[q - b];
+                    for (int k = 0; k < m; ++k) rank += (src[k] < i);
+                }
+                ii[u] = i;
+                pos[u] = s_st[cl] + rank;
+            }
+        }
+        float4 e[kSortBatch];
+        int ty[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            if (ii[u] >= 0) {
+                const int i = ii[u];
+                e[u].x = a.x[i];
+                e[u].y = a.y[i];
+                e[u].z = __int_as_float(a.grp[i]);
+                e[u].w = a.sig ? a.sig[i] : 0.0f;
+                ty[u] = a.typ ? a.typ[i] : 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            if (ii[u] >= 0) {
+                const int i = ii[u], ps = pos[u];
+                a.perm[ps] = i;
+                a.sorted[ps] = e[u];
+                if (a.typ) a.sorted_typ[ps] = ty[u];
+                if (a.spos) {  // list mode: the sorted-order state the list pass reads
+                    a.spos[i] = ps;
+                    a.xys[ps] = make_float2(e[u].x, e[u].y);
+                    if (a.sigs) a.sigs[ps] = e[u].w;
+                }
+            }
         }
     }
 }
@@ -827,8 +889,10 @@ __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
 // Neighbour-list mode (femmd2D/src/nn.cpp:22-62, re-designed): a tile of
 // cells of side >= rc*sigma + skin is staged as in the pair pass; every
 // centre particle records the partners within rc*sigma + skin (exclusion
-// rule applied here, once) by their index in the tile's LDS staging, 16 bits
-// each, four per group, groups interleaved over particles (group g of sorted
+// rule applied here, once) by the byte offset of their slot in the list
+// pass's LDS staging (8 bytes per slot, so a tile stages at most 8191
+// particles), 16 bits each, four per group, groups interleaved over
+// particles (group g of sorted
 // position p at g * n + p) so that a wavefront of consecutive positions reads
 // one group in one coalesced load.  Until the next rebuild the sorted order
 // and the tiles stay fixed, so the list pass restages each tile the same way
@@ -902,7 +966,7 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
                 bool hit = (dr2 < rl2) & (__float_as_int(pj.z) != gix);
                 if (r == 1) hit &= (j != me);
                 if (hit) {
-                    if (cnt < cap) list_store_slot(a, cnt, p, j);
+                    if (cnt < cap) list_store_slot(a, cnt, p, 8 * j);
                     ++cnt;
                 }
             }
@@ -910,8 +974,8 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
         const int m = min(cnt, cap);
         // groups of four, padded with the particle itself (distance 0: the
         // pair pass's cutoff test rejects it); cap is a multiple of 8
-        for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, me);
-        a.nnum[p] = m;
+        for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, 8 * me);
+        a.nnum[p] = m | (k << 16);  // list slot p holds particle k of the tile
         if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
 }
@@ -922,6 +986,44 @@ __device__ __forceinline__ int wave_max_int(int v) {
     return v;
 }
 
+// Rank of this lane's list among the active lanes of the workgroup, longest
+// first (by groups of four entries), ties by lane: a counting sort over
+// (length, wavefront) with the ties inside a wavefront ordered by ballots.
+// hist holds (capg + 1) * (blockDim / 64) ints.  Deterministic.
+__device__ int list_rank(int n4, bool act, int capg, int *hist) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nw = blockDim.x >> 6, nk = capg + 1, tot = nw * nk;
+    const int key = capg - min(n4, capg);  // 0 = longest
+    __syncthreads();  // earlier readers of hist are done
+    for (int i = tid; i < tot; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    if (act) atomicAdd(&hist[key * nw + w], 1);
+    __syncthreads();
+    if (w == 0) {  // exclusive scan in (length, wavefront) order
+        const int per = (tot + 63) >> 6, b0 = lane * per;
+        int sum = 0;
+        for (int i = 0; i < per; ++i)
+            if (b0 + i < tot) sum += hist[b0 + i];
+        int run = wave_incl_scan(sum) - sum;
+        for (int i = 0; i < per; ++i)
+            if (b0 + i < tot) {
+                const int v = hist[b0 + i];
+                hist[b0 + i] = run;
+                run += v;
+            }
+    }
+    __syncthreads();
+    unsigned long long eq = __ballot(act);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {  // key < 256
+        const bool kb = (key >> bit) & 1;
+        const unsigned long long m = __ballot(kb);
+        eq &= kb ? m : ~m;
+    }
+    const unsigned long long before = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    return act ? hist[key * nw + w] + __popcll(eq & before) : -1;
+}
+
 // List build of one staged tile through the LDS list buffer (slot k of lane
 // t at buf[k * blockDim + t]).  Each row's candidate loop runs a
 // wavefront-uniform trip count (the longest run among the lanes; a lane
@@ -930,7 +1032,7 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // Reads past a lane's run stay inside the staging's headroom of 3 cells.
 template <bool kWrap, bool kSigma>
 __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
-                                   uint16_t *buf) {
+                                   uint16_t *buf, int *hist) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -991,27 +1093,29 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
                 }
                 bool hit = (t < len) & (dr2 < rl2) & (__float_as_int(pj.z) != gix);
                 if (r == 1) hit &= (j != me);
-                buf[min(cnt, capm1) * bd + tid] = (uint16_t)j;
+                buf[min(cnt, capm1) * bd + tid] = (uint16_t)(8 * j);
                 cnt += hit ? 1 : 0;
             }
         }
+        const int m = min(cnt, cap);
+        // the chunk's lists are stored longest first (list slot q = sbase +
+        // k0 + rank), so that the pass's wavefronts take lists of similar
+        // length; nnum[q] = length | (tile index k << 16)
+        const int rank = list_rank((m + 3) >> 2, act, cap >> 2, hist);
         if (act) {
-            const int p = sbase + k;  // sorted position of this centre particle
-            const int m = min(cnt, cap);
+            const int p = sbase + k0 + rank;  // list slot
             // groups of four, padded with the particle itself (distance 0:
             // the pair pass's cutoff test rejects it); cap is a multiple of 8.
-            // The lanes of a wavefront hold consecutive sorted positions, so
-            // each group store is contiguous.
             for (int g = 0; g < ((m + 3) >> 2); ++g) {
                 int jj[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int kk = 4 * g + t;
-                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : me;
+                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : 8 * me;
                 }
                 list_store_group(a, g, p, jj);
             }
-            a.nnum[p] = m;
+            a.nnum[p] = m | (k << 16);
             if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
         }
     }
@@ -1023,6 +1127,7 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     __shared__ int s_cstart[kMaxSlots];
     __shared__ int s_off[kMaxSlots + 1];
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
+    __shared__ int s_hist[(kMaxListGroups + 1) * 4];
     PairLds L;
     L.ent = dyn;
     L.typ = nullptr;
@@ -1046,8 +1151,8 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
         }
         const bool wrapT = (T.w_flags >> 16) & 1;
         if (a.lbuf) {
-            if (wrapT) list_tile_body_buf<true, kSigma>(a, R, W, L, buf);
-            else list_tile_body_buf<false, kSigma>(a, R, W, L, buf);
+            if (wrapT) list_tile_body_buf<true, kSigma>(a, R, W, L, buf, s_hist);
+            else list_tile_body_buf<false, kSigma>(a, R, W, L, buf, s_hist);
         } else {
             if (wrapT) list_tile_body_direct<true, kSigma>(a, R, W, L);
             else list_tile_body_direct<false, kSigma>(a, R, W, L);
@@ -1080,13 +1185,18 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
     const int capg = a.nbr_cap >> 2;
     const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
     double Ublk = 0.0, Wblk = 0.0;
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    for (int l = threadIdx.x; l < nc; l += blockDim.x) {
+        // list slot q holds the list of the tile's particle k (slots of a
+        // chunk of blockDim particles are ordered longest list first)
+        const int q = sbase + l;
+        const int nq = a.nnum[q];
+        const int k = nq >> 16;
         const int me = c_lo + k, p = sbase + k;
         const float2 pi = lp[me];
         const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
-        const int n4 = (a.nnum[p] + 3) >> 2;  // padded with the particle itself
+        const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the particle itself
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
         // one list entry, without branches: the force of a non-interacting
@@ -1143,27 +1253,34 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
         auto group = [&](uint64_t cur) {
             const int j0 = (int)(cur & 0xffffu), j1 = (int)((cur >> 16) & 0xffffu);
             const int j2 = (int)((cur >> 32) & 0xffffu), j3 = (int)(cur >> 48);
-            const float2 q0 = lp[j0], q1 = lp[j1], q2 = lp[j2], q3 = lp[j3];
+            // entries are byte offsets of float2 staging slots
+            const char *lpb = reinterpret_cast<const char *>(lp);
+            const float2 q0 = *reinterpret_cast<const float2 *>(lpb + j0);
+            const float2 q1 = *reinterpret_cast<const float2 *>(lpb + j1);
+            const float2 q2 = *reinterpret_cast<const float2 *>(lpb + j2);
+            const float2 q3 = *reinterpret_cast<const float2 *>(lpb + j3);
             float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
             if (kSigma) {
-                s0 = ls[j0];
-                s1 = ls[j1];
-                s2 = ls[j2];
-                s3 = ls[j3];
+                s0 = ls[j0 >> 3];
+                s1 = ls[j1 >> 3];
+                s2 = ls[j2 >> 3];
+                s3 = ls[j3 >> 3];
             }
             term(q0, s0);
             term(q1, s1);
             term(q2, s2);
             term(q3, s3);
         };
-        // two groups per iteration, both loaded before either is used (the
-        // second group index is clamped inside the array; a group past the
-        // list is loaded and not used)
-        for (int g = 0; g < n4; g += 2) {
-            const uint64_t ga = nb[(uint32_t)g * Nu + (uint32_t)p];
-            const uint64_t gb = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)p];
-            group(ga);
-            if (g + 1 < n4) group(gb);
+        // the next group is loaded while this one is evaluated (the group
+        // index is clamped inside the array; a group past the list is loaded
+        // and not used); the empty asm keeps the compiler from moving the
+        // load down to its use in the next iteration
+        uint64_t gv = nb[(uint32_t)q];
+        for (int g = 0; g < n4; ++g) {
+            const uint64_t cur = gv;
+            gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)q];
+            __asm__ volatile("" ::: "memory");
+            group(cur);
         }
         const int id = a.perm[p];
         a.fx[id] = fx * pe.e48;
@@ -1371,6 +1488,19 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             }
         }
     }
+    // element and bond workgroups first (a whole number of rounds over the
+    // 8 XCDs, so each XCD's tiles still form one contiguous band), then the
+    // tiles: the kernel's tail is made of the uniform tile work
+    {
+        const int ne = a.nespans + a.nbspans;
+        const int nfirst = (ne + 7) & ~7;
+        if (b < nfirst) {
+            if (b < a.nespans) elem_block(a, b, s_red);
+            else if (b < ne) bond_block(a, b - a.nespans, s_red);
+            return;
+        }
+        b -= nfirst;
+    }
     if (kList) {
         if (b < a.ntiles) {
             PairLds L;
@@ -1384,9 +1514,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             float2 *lp = reinterpret_cast<float2 *>(dyn);
             float *ls = reinterpret_cast<float *>(lp + a.lds_cap);
             list_tile_pass<kSigma, kEnergy>(a, xcd_remap(b, a.ntiles), L, lp, ls);
-            return;
         }
-        b -= a.ntiles;
     } else if (b < a.ntiles) {
         PairLds L;
         L.ent = dyn;
@@ -1398,16 +1526,13 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         L.pt = s_pt;
         L.red = s_red;
         pair_tile<kSigma, kTypes, kDebug, kEnergy>(a, xcd_remap(b, a.ntiles), L);
-        return;
-    } else {
-        b -= a.ntiles;
     }
-    if (b < a.nespans) {
-        elem_block(a, b, s_red);
-        return;
-    }
-    b -= a.nespans;
-    if (b < a.nbspans) bond_block(a, b, s_red);
+}
+
+// Workgroups of one force pass: element and bond spans (rounded up to a
+// multiple of 8), then the tiles.
+__host__ __device__ inline int forces_grid(const ForceArgs &a) {
+    return ((a.nespans + a.nbspans + 7) & ~7) + a.ntiles;
 }
 
 // Total force per particle (pair + gathered corners), internal order.
@@ -1930,7 +2055,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
                          const ForceArgs &a, hipStream_t s) {
     if (list) {
         // neighbour lists: single type, no dump (the caller checks)
-        const int nblk = a.ntiles + a.nespans + a.nbspans;
+        const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
         if (nblk <= 0) return hipSuccess;
         dim3 g(nblk), b(block);
         const size_t lds = (size_t)a.lds_cap * (sizeof(float2) + (sigma ? sizeof(float) : 0));
@@ -1940,7 +2065,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
 #undef FM_LIST
         return hipGetLastError();
     }
-    const int nblk = a.ntiles + a.nespans + a.nbspans;
+    const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
     if (nblk <= 0) return hipSuccess;
     const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
     dim3 g(nblk), b(block);
