@@ -56,6 +56,43 @@ int Engine::adj_alloc(std::string &err) {
     return FM_OK;
 }
 
+int Engine::adj_pass_list(bool fresh, std::string &err) {
+    if (fresh) {
+        // lists for this configuration; later configurations of the sweep
+        // rebuild them when a particle has moved more than skin / 2 from
+        // the positions of this build
+        if (launch_bin() || launch_sort() || launch_list_build(false))
+            return check(hipGetLastError(), "adjoint lists", err);
+        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, stream_) != hipSuccess)
+            return check(hipGetLastError(), "ctl", err);
+    } else {
+        KickArgs k{};
+        k.reps = reps_.p;
+        k.spans = kspans_.p;
+        k.x = xcur();
+        k.y = ycur();
+        k.spos = spos_.p;
+        k.xys = xys_.p;
+        k.dmax = dmax_.p;
+        if (launch_adj_disp((int)kspans_h_.size(), k, stream_) != hipSuccess)
+            return check(hipGetLastError(), "adjoint displacement", err);
+        if (launch_bin(true) || launch_sort(true) || launch_list_build(true))
+            return check(hipGetLastError(), "adjoint lists", err);
+    }
+    AdjArgs A{};
+    A.f = force_args();
+    A.w = w_.p;
+    A.jx = jx_.p;
+    A.jy = jy_.p;
+    A.cfa = cfa_.p;
+    A.eE = eE_.p;
+    A.dE = dE_.p;
+    A.pacc = pacc_.p;
+    if (launch_adj_forces(sigma_, true, block_, A, stream_) != hipSuccess)
+        return check(hipGetLastError(), "adjoint forces", err);
+    return FM_OK;
+}
+
 int Engine::adj_pass(std::string &err) {
     if (launch_bin()) return check(hipGetLastError(), "bin", err);
     if (launch_sort()) return check(hipGetLastError(), "sort", err);
@@ -68,7 +105,7 @@ int Engine::adj_pass(std::string &err) {
     A.eE = eE_.p;
     A.dE = dE_.p;
     A.pacc = pacc_.p;
-    if (launch_adj_forces(sigma_, block_, A, stream_) != hipSuccess)
+    if (launch_adj_forces(sigma_, false, block_, A, stream_) != hipSuccess)
         return check(hipGetLastError(), "adjoint forces", err);
     return FM_OK;
 }
@@ -241,11 +278,15 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         const int len = std::min(ckpt, nsteps - s * ckpt);
         const bool last = s == S - 1;
         if (last && record(0)) return check(hipGetLastError(), "history", err);
-        if (launch_step_body(false, true)) return check(hipGetLastError(), "step", err);
-        for (int k = 1; k < len; ++k) {
-            if (last && record(k)) return check(hipGetLastError(), "history", err);
-            if (launch_step_body(true, true)) return check(hipGetLastError(), "step", err);
+        // the drift of body k records x_{k+1} into history slot k + 1
+        rec_ = (last && len > 1) ? hist_.p + (size_t)2 * N : nullptr;
+        int bad = launch_step_body(false, true);
+        for (int k = 1; k < len && !bad; ++k) {
+            rec_ = (last && k + 1 < len) ? hist_.p + (size_t)(k + 1) * 2 * N : nullptr;
+            bad = launch_step_body(true, true);
         }
+        rec_ = nullptr;
+        if (bad) return check(hipGetLastError(), "step", err);
         if (launch_step_body(true, false)) return check(hipGetLastError(), "kick", err);
     }
     energy_ = true;
@@ -284,10 +325,14 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
             if (load_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "restore", err);
             rc = ensure_primed(err);
             if (rc) return rc;
-            for (int k = 0; k < len; ++k) {
-                if (k > 0 && launch_step_body(k > 1, true)) return check(hipGetLastError(), "step", err);
-                if (record(k)) return check(hipGetLastError(), "history", err);
+            if (record(0)) return check(hipGetLastError(), "history", err);
+            int bad = 0;
+            for (int k = 1; k < len && !bad; ++k) {
+                rec_ = hist_.p + (size_t)k * 2 * N;  // the drift records x_k
+                bad = launch_step_body(k > 1, true);
             }
+            rec_ = nullptr;
+            if (bad) return check(hipGetLastError(), "step", err);
         }
         for (int k = len - 1; k >= 0; --k) {
             const int n = s * ckpt + k;
@@ -297,7 +342,9 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
             }
             xalt_ = hist_.p + (size_t)k * 2 * N;
             yalt_ = xalt_ + N;
-            rc = adj_pass(err);
+            // list mode: the segment's first configuration (the layout was
+            // last laid out by the forward pass) builds fresh lists
+            rc = list_ ? adj_pass_list(k == len - 1, err) : adj_pass(err);
             xalt_ = yalt_ = nullptr;
             if (rc) return rc;
             const bool more = n > 0;

@@ -97,6 +97,9 @@ struct KickArgs {
                                // kDispStride words apart
     const int32_t *spos;       // list mode: particle -> sorted position
     float2 *xys;               // list mode: positions in sorted order (out)
+    float *rec;                // if set, the drift also writes x to rec[p] and
+                               // y to rec[n + p] (trajectory recording)
+    int32_t n;                 // particles (stride of rec)
     int32_t nspans;            // spans (grid-stride binning)
 };
 

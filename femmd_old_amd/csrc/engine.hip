@@ -612,6 +612,8 @@ int Engine::launch_step_body(bool kick2, bool drift) {
     k.cap = cap_;
     k.ke_part = ke_part_.p;
     k.err = err_.p;
+    k.rec = drift ? rec_ : nullptr;
+    k.n = N_;
     if (list_) {
         k.dmax = dmax_.p;
         k.spos = spos_.p;
@@ -809,7 +811,7 @@ int Engine::ensure_primed(std::string &err) {
         // a fresh list epoch at the current positions
         if (launch_sort()) return check(hipGetLastError(), "sort", err);
         if (launch_list_build(false)) return check(hipGetLastError(), "list build", err);
-        if (launch_ctl_reset(ctl_.p, R_, stream_) != hipSuccess) return check(hipGetLastError(), "ctl", err);
+        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, stream_) != hipSuccess) return check(hipGetLastError(), "ctl", err);
         if (launch_pair_pass()) return check(hipGetLastError(), "forces", err);
     } else {
         if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);

@@ -136,6 +136,9 @@ private:
     float *ycur() const { return yalt_ ? yalt_ : y_.p; }
     int adj_alloc(std::string &err);
     int adj_pass(std::string &err);  // bin + sort + adjoint force kernel at xcur
+    // list mode backward sweep at xcur: lists rebuilt (fresh, or gated on the
+    // displacement from the last build) + adjoint kernel over the lists
+    int adj_pass_list(bool fresh, std::string &err);
     int adj_kick(bool accumulate, bool drift, bool make_w, float c);
     int save_state(float *dst);
     int load_state(const float *src);
@@ -210,6 +213,7 @@ private:
 
     // adjoint state
     float *xalt_ = nullptr, *yalt_ = nullptr;
+    float *rec_ = nullptr;  // drift of the next step body records x, y here
     std::vector<int32_t> ecaller_h_;  // internal element -> caller element index
     DBuf<float> eE_;                  // Young's modulus per element
     DBuf<float2> w_, cfa_;
@@ -229,7 +233,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
 size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered);
 int list_build_resident(bool sigma, int block, const ForceArgs &a, int device);
-hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s);
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
                               const int32_t *goff, const int32_t *gent, float *ox, float *oy,
@@ -237,7 +241,8 @@ hipError_t launch_total_force(int n, const float *fx, const float *fy, const flo
 hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
                          const double *fea, const double *bond, double *out, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
-hipError_t launch_adj_forces(bool sigma, int block, const AdjArgs &A, hipStream_t s);
+hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s);
+hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s);
 hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s);
 

@@ -204,6 +204,10 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
             }
             a.x[p] = x;
             a.y[p] = y;
+            if (a.rec) {
+                a.rec[p] = x;
+                a.rec[a.n + p] = y;
+            }
             if (!kList) cell = R.cell0 + cell_key(x, y, R);
             if (kList) {
                 a.xys[a.spos[p]] = make_float2(x, y);
@@ -1609,17 +1613,19 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
 }
 
 // Start a new list epoch (after a forced rebuild), which is counted.
-__global__ void k_ctl_reset(Ctl *ctl, int nrep) {
+__global__ void k_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax) {
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
         ctl[r].flag = 0;
         ctl[r].disp = 0.0f;
         ctl[r].nbuild += 1u;
     }
+    for (int k = threadIdx.x; k < nrep * kDispSlots; k += blockDim.x)
+        dmax[(size_t)k * kDispStride] = 0u;
 }
 
-hipError_t launch_ctl_reset(Ctl *ctl, int nrep, hipStream_t s) {
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, hipStream_t s) {
     if (nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, nrep);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, nrep, dmax);
     return hipGetLastError();
 }
 
@@ -1759,6 +1765,148 @@ __device__ void adj_pair_tile(const AdjArgs &A, int tile, PairLds &L, float2 *lw
         adj_pair_body<false, kSigma>(A, R, W, tile, L, lw);
 }
 
+// J w and the parameter terms over the neighbour lists (the backward sweep
+// in list mode): the tile's positions at the configuration being
+// differentiated and w are staged in the list build's layout (gathered
+// through perm), each lane takes a list slot, as the forward list pass.
+template <bool kWrap, bool kSigma>
+__device__ void adj_list_body(const AdjArgs &A, const RepDesc &R, int W, int tile,
+                              const PairLds &L, const float2 *lp, const float2 *lw,
+                              const float *ls) {
+    const ForceArgs &a = A.f;
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const int sbase = L.cstart[row + 1];
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const PairEntry pe = L.pt[0];
+    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    const uint32_t Nu = (uint32_t)a.n;
+    const int capg = a.nbr_cap >> 2;
+    const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
+    const char *lpb = reinterpret_cast<const char *>(lp);
+    const char *lwb = reinterpret_cast<const char *>(lw);
+    double peps = 0.0, psig = 0.0;
+    for (int l = threadIdx.x; l < nc; l += blockDim.x) {
+        const int q = sbase + l;
+        const int nq = a.nnum[q];
+        const int k = nq >> 16;
+        const int me = c_lo + k, p = sbase + k;
+        const float2 pi = lp[me];
+        const float2 wi = lw[me];
+        const float xi = pi.x, yi = pi.y, xim = xi - Lx, yim = yi - Ly;
+        const float si = kSigma ? ls[me] : 0.0f;
+        const int n4 = ((nq & 0xffff) + 3) >> 2;
+        float jx = 0.f, jy = 0.f, fx = 0.f, fy = 0.f, gx = 0.f, gy = 0.f;
+        auto term = [&](int off) {
+            const float2 pj = *reinterpret_cast<const float2 *>(lpb + off);
+            const float ex = xi - pj.x, ey = yi - pj.y;
+            float ox = 0.0f, oy = 0.0f;
+            if (kWrap) {
+                ox = (ex >= hLx) ? Lx : 0.0f;
+                ox = (ex < -hLx) ? -Lx : ox;
+                oy = (ey >= hLy) ? Ly : 0.0f;
+                oy = (ey < -hLy) ? -Ly : oy;
+            }
+            const float dx = ex - ox, dy = ey - oy;
+            const float dr2 = dx * dx + dy * dy;
+            float s2ij, sij;
+            unsigned rc2m1;
+            if (kSigma) {
+                sij = (si + ls[off >> 3]) * 0.5f;
+                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                s2ij = sij * sij;
+            } else {
+                sij = R.s_uniform;
+                rc2m1 = rc2m1_u;
+                s2ij = R.s2_uniform;
+            }
+            if ((__float_as_uint(dr2) - 1u) < rc2m1) {
+                float dxa = dx, dya = dy, r2a = dr2;
+                if (kWrap) {
+                    dxa = (ox > 0.0f) ? (xim - pj.x) : ((ox < 0.0f) ? (xi - (pj.x - Lx)) : dx);
+                    dya = (oy > 0.0f) ? (yim - pj.y) : ((oy < 0.0f) ? (yi - (pj.y - Ly)) : dy);
+                    r2a = dxa * dxa + dya * dya;
+                }
+                const float inv = __builtin_amdgcn_rcpf(r2a);
+                const float u = s2ij * inv;
+                const float s6 = u * u * u;
+                const float s12 = s6 * s6;
+                const float g = __builtin_fmaf(-0.5f, s6, s12) * inv;
+                const float gp = __builtin_fmaf(2.0f, s6, -7.0f * s12) * inv * inv;
+                const float gs = __builtin_fmaf(12.0f, s12, -3.0f * s6) * inv / sij;
+                const float2 wj = *reinterpret_cast<const float2 *>(lwb + off);
+                const float dwx = wi.x - wj.x, dwy = wi.y - wj.y;
+                const float ddw = __builtin_fmaf(dxa, dwx, dya * dwy);
+                const float g2 = 2.0f * gp * ddw;
+                jx = __builtin_fmaf(g, dwx, __builtin_fmaf(g2, dxa, jx));
+                jy = __builtin_fmaf(g, dwy, __builtin_fmaf(g2, dya, jy));
+                fx = __builtin_fmaf(g, dxa, fx);
+                fy = __builtin_fmaf(g, dya, fy);
+                gx = __builtin_fmaf(gs, dxa, gx);
+                gy = __builtin_fmaf(gs, dya, gy);
+            }
+        };
+        uint64_t gv = nb[(uint32_t)q];
+        for (int g = 0; g < n4; ++g) {
+            const uint64_t cur = gv;
+            gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)q];
+            __asm__ volatile("" ::: "memory");
+            term((int)(cur & 0xffffu));
+            term((int)((cur >> 16) & 0xffffu));
+            term((int)((cur >> 32) & 0xffffu));
+            term((int)(cur >> 48));
+        }
+        const int id = a.perm[p];
+        A.jx[id] = pe.e48 * jx;
+        A.jy[id] = pe.e48 * jy;
+        peps += 48.0 * ((double)wi.x * fx + (double)wi.y * fy);
+        psig += (double)pe.e48 * ((double)wi.x * gx + (double)wi.y * gy);
+    }
+    block_sum2(peps, psig, L.red);
+    if (threadIdx.x == 0) {
+        A.pacc[2 * tile] += peps;
+        A.pacc[2 * tile + 1] += psig;
+    }
+}
+
+template <bool kSigma>
+__device__ void adj_list_tile(const AdjArgs &A, int tile, PairLds &L, float2 *lp, float2 *lw,
+                              float *ls) {
+    const ForceArgs &a = A.f;
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    if (stage_layout(a, T, R, W, L) < 0) {
+        if (threadIdx.x == 0) atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+        return;
+    }
+    struct Ent {
+        float2 x, w;
+        float s;
+    };
+    stage_copy(
+        T, R, W, L,
+        [&](int p) {
+            const int i = a.perm[p];
+            Ent u;
+            u.x = make_float2(a.x[i], a.y[i]);
+            u.w = A.w[i];
+            u.s = kSigma ? a.sigs[p] : 0.0f;
+            return u;
+        },
+        [&](int e, const Ent &u) {
+            lp[e] = u.x;
+            lw[e] = u.w;
+            if (kSigma) ls[e] = u.s;
+        });
+    __syncthreads();
+    if ((T.w_flags >> 16) & 1)
+        adj_list_body<true, kSigma>(A, R, W, tile, L, lp, lw, ls);
+    else
+        adj_list_body<false, kSigma>(A, R, W, tile, L, lp, lw, ls);
+}
+
 // Element part of J w: dP (the directional derivative of the first
 // Piola-Kirchhoff stress along dF = dDs Bm) pushed through H = -A0 P Bm^T;
 // and dL/dE_e += w . f_e / E_e (P is linear in (lambda, mu), both
@@ -1890,7 +2038,7 @@ __device__ void adj_bond_block(const AdjArgs &A, int b) {
     }
 }
 
-template <bool kSigma>
+template <bool kSigma, bool kList>
 __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
@@ -1899,6 +2047,14 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
     __shared__ double s_red[2 * 4];
     const ForceArgs &a = A.f;
     int b = blockIdx.x;
+    if (kList) {
+        // the rebuild decision of this configuration is taken: clear its
+        // summary and the displacement slots (workgroup r: replica r)
+        if (b == 0 && threadIdx.x == 0) *a.any = 0;
+        if (threadIdx.x < kDispSlots)
+            for (int r = blockIdx.x; r < a.nrep; r += gridDim.x)
+                a.dmax[((size_t)r * kDispSlots + threadIdx.x) * kDispStride] = 0u;
+    }
     if (b < a.ntiles) {
         PairLds L;
         L.ent = dyn;
@@ -1908,8 +2064,15 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
         L.off = s_off;
         L.pt = s_pt;
         L.red = s_red;
-        float2 *lw = reinterpret_cast<float2 *>(dyn + a.lds_cap);
-        adj_pair_tile<kSigma>(A, xcd_remap(b, a.ntiles), L, lw);
+        if (kList) {
+            float2 *lp = reinterpret_cast<float2 *>(dyn);
+            float2 *lw = lp + a.lds_cap;
+            float *ls = reinterpret_cast<float *>(lw + a.lds_cap);
+            adj_list_tile<kSigma>(A, xcd_remap(b, a.ntiles), L, lp, lw, ls);
+        } else {
+            float2 *lw = reinterpret_cast<float2 *>(dyn + a.lds_cap);
+            adj_pair_tile<kSigma>(A, xcd_remap(b, a.ntiles), L, lw);
+        }
         return;
     }
     b -= a.ntiles;
@@ -1919,6 +2082,34 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
     }
     b -= a.nespans;
     if (b < a.nbspans) adj_bond_block(A, b);
+}
+
+// Backward sweep in list mode: the displacement of each particle from the
+// positions of the last list build (held in sorted order), as the largest
+// squared displacement per workgroup into the replica's slots (minimum image:
+// a particle may have crossed the box since).
+__global__ __launch_bounds__(kKickThreads) void k_adj_disp(KickArgs a) {
+    __shared__ float redd[kKickThreads / 64];
+    const Span sp = a.spans[blockIdx.x];
+    const RepDesc R = a.reps[sp.rep];
+    const int p = sp.begin + (int)threadIdx.x;
+    float d2 = 0.0f;
+    if (p < sp.end) {
+        const float2 r = a.xys[a.spos[p]];
+        float dx = a.x[p] - r.x, dy = a.y[p] - r.y;
+        dx = (dx >= R.hLx) ? dx - R.Lx : ((dx < -R.hLx) ? dx + R.Lx : dx);
+        dy = (dy >= R.hLy) ? dy - R.Ly : ((dy < -R.hLy) ? dy + R.Ly : dy);
+        d2 = dx * dx + dy * dy;
+    }
+    const float m = wave_max(d2);
+    if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mm = redd[0];
+        for (int k = 1; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redd[k]);
+        atomicMax(a.dmax + ((size_t)sp.rep * kDispSlots + blockIdx.x % kDispSlots) * kDispStride,
+                  __float_as_uint(mm));
+    }
 }
 
 __global__ __launch_bounds__(kKickThreads) void k_adj_kick(AdjKickArgs a) {
@@ -1976,15 +2167,29 @@ __global__ __launch_bounds__(256) void k_adj_loss(LossArgs a) {
 // ---------------------------------------------------------------------------
 // launchers (host side)
 // ---------------------------------------------------------------------------
-hipError_t launch_adj_forces(bool sigma, int block, const AdjArgs &A, hipStream_t s) {
+hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s) {
     const ForceArgs &a = A.f;
     const int nblk = a.ntiles + a.nespans + a.nbspans;
     if (nblk <= 0) return hipSuccess;
+    if (list) {
+        const size_t lds = (size_t)a.lds_cap * (2 * sizeof(float2) + (sigma ? sizeof(float) : 0));
+        if (sigma)
+            hipLaunchKernelGGL((k_adj_forces<true, true>), dim3(nblk), dim3(block), lds, s, A);
+        else
+            hipLaunchKernelGGL((k_adj_forces<false, true>), dim3(nblk), dim3(block), lds, s, A);
+        return hipGetLastError();
+    }
     const size_t lds = (size_t)a.lds_cap * (sizeof(float4) + sizeof(float2));
     if (sigma)
-        hipLaunchKernelGGL((k_adj_forces<true>), dim3(nblk), dim3(block), lds, s, A);
+        hipLaunchKernelGGL((k_adj_forces<true, false>), dim3(nblk), dim3(block), lds, s, A);
     else
-        hipLaunchKernelGGL((k_adj_forces<false>), dim3(nblk), dim3(block), lds, s, A);
+        hipLaunchKernelGGL((k_adj_forces<false, false>), dim3(nblk), dim3(block), lds, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_disp, dim3(nblk), dim3(kKickThreads), 0, s, a);
     return hipGetLastError();
 }
 
