@@ -261,6 +261,53 @@ class System:
     def info(self) -> dict:
         return {"N": self.N, "box": self.get_box(), "dt": self._dt}
 
+    # -- observables (femmd2D/src/feaElements.cpp:86-196,
+    #    femmd2D/src/md3dsystem.cpp:92-179) ---------------------------------
+    def cluster_count(self) -> int:
+        n = C.c_int32()
+        self._check(self._lib.fm_cluster_count(self._h, C.byref(n)))
+        return n.value
+
+    def cluster_props(self, unfolded: bool = False) -> dict:
+        """Per elastomer cluster: mass, centre of mass ``cm`` (of the unfolded
+        nodes), its velocity ``vcm`` and the circular-mean ``centroid``; with
+        ``unfolded`` also the unfolded positions ``ux``, ``uy`` (caller order;
+        particles outside clusters keep their wrapped positions)."""
+        nc = self.cluster_count()
+        out = np.zeros((max(nc, 1), 7), dtype=np.float64)
+        ux = uy = None
+        if unfolded:
+            ux = np.zeros(self.N, dtype=np.float32)
+            uy = np.zeros(self.N, dtype=np.float32)
+        self._check(self._lib.fm_cluster_props(self._h, _dp(out), _fp(ux), _fp(uy)))
+        out = out[:nc]
+        r = {"mass": out[:, 0].copy(), "cm": out[:, 1:3].copy(), "vcm": out[:, 3:5].copy(),
+             "centroid": out[:, 5:7].copy()}
+        if unfolded:
+            r["ux"], r["uy"] = ux, uy
+        return r
+
+    def calc_cluster_props(self) -> dict:
+        """System.calc_cluster_props (femmd2D/src/interface.cpp:333)."""
+        return self.cluster_props()
+
+    def cluster_virial(self) -> float:
+        """sum over interacting pairs of cluster particles of
+        1/2 f_ij . (R_ci - R_cj) (femmd2D/src/interactions.cpp:370-382)."""
+        v = C.c_double()
+        self._check(self._lib.fm_cluster_virial(self._h, C.byref(v)))
+        return v.value
+
+    def set_temperature(self, temp: float, ids=None):
+        """MDSystem::setTemperature (femmd2D/src/md3dsystem.cpp:92-136):
+        all particles (T = KE / N), or a group of caller ids with the
+        reference's group formula T = 2 KE_group / (3 N)."""
+        if ids is None:
+            self._check(self._lib.fm_set_temperature(self._h, float(temp), 0, None))
+        else:
+            a = _i32(ids)
+            self._check(self._lib.fm_set_temperature(self._h, float(temp), len(a), _ip(a)))
+
     def get_state(self):
         n = self.N
         out = [np.empty(n, np.float32) for _ in range(4)]

@@ -85,6 +85,10 @@ PROTOTYPES = {
     "fm_set_exclusion": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "fm_set_skin": (C.c_int, [_h, C.c_float]),
     "fm_list_stats": (C.c_int, [_h, _d, _i, _l]),
+    "fm_cluster_count": (C.c_int, [_h, _i]),
+    "fm_cluster_props": (C.c_int, [_h, _d, _f, _f]),
+    "fm_cluster_virial": (C.c_int, [_h, _d]),
+    "fm_set_temperature": (C.c_int, [_h, C.c_double, C.c_int32, _i]),
     "fm_set_mass": (C.c_int, [_h, C.c_int32, _i, _f]),
     "fm_commit": (C.c_int, [_h]),
     "fm_step": (C.c_int, [_h, C.c_int32]),

@@ -153,6 +153,8 @@ int fm_add_elastomer(fm_system *h, int32_t nv, const float *verts, int32_t nt, c
         S.rx.push_back(px);
         S.ry.push_back(py);
     }
+    S.cl_first.push_back(off);
+    S.cl_n.push_back(nv);
     for (int32_t e = 0; e < nt; ++e) {
         for (int k = 0; k < 3; ++k) S.tri.push_back(off + tris[3 * e + k] - index_base);
         S.tE.push_back(E_per_tri ? E_per_tri[e] : -1.0f);
@@ -419,6 +421,37 @@ int fm_list_stats(fm_system *h, double *mean_len, int32_t *max_len, int64_t *reb
     FM_GUARD(h)
     FM_NEED_ENGINE(h);
     return h->eng->list_stats(0, mean_len, max_len, rebuilds, h->err);
+    FM_END(h)
+}
+
+int fm_cluster_count(fm_system *h, int32_t *nc) {
+    FM_GUARD(h)
+    if (!nc) return fail(h, FM_ERR_ARG, "null output");
+    *nc = h->eng ? h->eng->ncluster(0) : (int32_t)h->hs.cl_first.size();
+    return FM_OK;
+    FM_END(h)
+}
+
+int fm_cluster_props(fm_system *h, double *props, float *ux, float *uy) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->cluster_props(0, props, ux, uy, h->err);
+    FM_END(h)
+}
+
+int fm_cluster_virial(fm_system *h, double *virial) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    if (!virial) return fail(h, FM_ERR_ARG, "null output");
+    return h->eng->cluster_virial(0, virial, h->err);
+    FM_END(h)
+}
+
+int fm_set_temperature(fm_system *h, double temp, int32_t n, const int32_t *ids) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    if (n < 0 || (n > 0 && !ids)) return fail(h, FM_ERR_ARG, "n >= 0 and ids for n > 0");
+    return h->eng->set_temperature(0, temp, n, ids, h->err);
     FM_END(h)
 }
 

@@ -231,4 +231,28 @@ struct LossArgs {
     double *part;         // per workgroup partial sum of |x - t|^2
 };
 
+// Elastomer cluster observables (one workgroup per cluster).
+struct ClusterArgs {
+    const RepDesc *reps;
+    int32_t ncl;
+    const int32_t *crep;       // replica of each cluster
+    const int32_t *off;        // cluster -> first entry of nodes (ncl + 1)
+    const int32_t *nodes;      // internal ids of the clusters' nodes
+    const float *x, *y, *vx, *vy, *mass;
+    double *out;               // kClusterOut per cluster
+    float2 *cm;                // centre of mass per cluster (the virial pass)
+    float *ux, *uy;            // unfolded positions, internal order (may be null)
+};
+// per cluster: mass, centre of mass x, y, its velocity x, y, centroid x, y
+constexpr int kClusterOut = 7;
+
+// Cluster virial: sum over interacting pairs (i, j) of particles of
+// clusters of 1/2 f_ij . (R_c(i) - R_c(j)) (minimum image), per tile.
+struct ClusterVirialArgs {
+    ForceArgs f;
+    const int32_t *cid;        // cluster of each particle (-1: not in one)
+    const float2 *cm;
+    double *out;               // per tile
+};
+
 }  // namespace fm

@@ -457,6 +457,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     if ((he = hipMemsetAsync(E->bond_part_.p, 0, E->bond_part_.n * sizeof(double), st)) != hipSuccess)
         return E->check(he, "memset", err);
 
+    rc = E->build_clusters(reps, err);
+    if (rc) return rc;
     // tiles (and LDS capacity) for the requested widths
     std::vector<int> tw(R);
     for (int r = 0; r < R; ++r) tw[r] = reps[r]->tile_w;

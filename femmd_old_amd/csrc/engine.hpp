@@ -31,6 +31,7 @@ struct HostSystem {
     std::vector<float> eps{1.0f}, rc{1.12246204830937f}, ush{1.0f};
     bool lj_on = true;
     // elements
+    std::vector<int32_t> cl_first, cl_n; // elastomer clusters: caller id ranges
     std::vector<int32_t> tri;            // 3 per element, caller ids
     std::vector<float> tE, tnu;          // per element, < 0 = global
     int32_t model = FM_NEO_HOOKEAN;
@@ -113,6 +114,13 @@ public:
     int ncells_of(int r) const { return reps_h_[r].ncx * reps_h_[r].ncy; }
     int set_dt(int r, double dt, std::string &err);
     const RepDesc &rep(int r) const { return reps_h_[r]; }
+    // elastomer cluster observables (replica r): kClusterOut doubles per
+    // cluster; unfolded positions in caller order (may be null)
+    int ncluster(int r) const;
+    int cluster_props(int r, double *out, float *ux, float *uy, std::string &err);
+    int cluster_virial(int r, double *vir, std::string &err);
+    // velocity rescale to temperature temp (the reference's setTemperature)
+    int set_temperature(int r, double temp, int n, const int32_t *ids, std::string &err);
     int tile_width(int r) const { return tile_w_[r]; }
     int block() const { return block_; }
     double density_of(int r) const;
@@ -143,6 +151,8 @@ private:
     int save_state(float *dst);
     int load_state(const float *src);
     int build_tiles(const std::vector<int> &tile_w, std::string &err);
+    int build_clusters(const std::vector<const HostSystem *> &reps, std::string &err);
+    int run_cluster_props(float *ux, float *uy);
     int ensure_primed(std::string &err);
     int check(hipError_t e, const char *what, std::string &err);
     int drop_graph();
@@ -220,6 +230,12 @@ private:
     DBuf<float> jx_, jy_, xbx_, xby_, vbx_, vby_;
     DBuf<double> dE_, pacc_, loss_part_;
     DBuf<float> ckpt_, hist_;
+
+    // elastomer clusters
+    std::vector<int32_t> cl_rep_h_, cl_off_h_;  // replica, node range per cluster
+    DBuf<int32_t> cl_rep_, cl_off_, cl_nodes_, cid_, sel_;
+    DBuf<double> cl_out_, cl_vir_, vel_part_;
+    DBuf<float2> cl_cm_;
     DBuf<int32_t> loss_ids_;
     DBuf<float2> loss_t_;
 };
@@ -245,5 +261,11 @@ hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A,
 hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s);
 hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s);
+hipError_t launch_cluster_props(const ClusterArgs &a, hipStream_t s);
+hipError_t launch_cluster_virial(bool sigma, bool types, int block, const ClusterVirialArgs &A,
+                                 hipStream_t s);
+hipError_t launch_vel_ke(const int32_t *ids, int n, const float *vx, const float *vy,
+                         const float *mass, double *part, hipStream_t s);
+hipError_t launch_vel_scale(const int32_t *ids, int n, float *vx, float *vy, float f, hipStream_t s);
 
 }  // namespace fm

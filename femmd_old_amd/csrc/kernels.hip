@@ -2165,6 +2165,256 @@ __global__ __launch_bounds__(256) void k_adj_loss(LossArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Elastomer cluster observables (femmd2D/src/feaElements.cpp:86-196,
+// re-designed: one workgroup per cluster, FP64 sums in a fixed order).
+// ---------------------------------------------------------------------------
+
+// Sums K doubles over the workgroup in a fixed order; every thread gets the
+// totals.  red holds K * (blockDim / 64) doubles.
+template <int K>
+__device__ void block_sums_all(double (&v)[K], double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (int q = 0; q < nw; ++q) t += red[q * K + k];
+        v[k] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Per cluster: the unfolded node positions (femmd2D/src/feaElements.cpp:
+// 86-133: a cluster wider than half the box has its nodes beyond L/2 moved
+// down by L, and all of it moved up by L if the mean then falls below 0),
+// mass-weighted centre of mass and its velocity (:135-149), and the
+// centroid as the circular mean of the wrapped positions (:161-193; here
+// with the y coordinate's own sums).
+__global__ __launch_bounds__(256) void k_cluster_props(ClusterArgs a) {
+    __shared__ double red[9 * 4];
+    __shared__ float mm[4 * 4];
+    const int c = blockIdx.x;
+    const RepDesc R = a.reps[a.crep[c]];
+    const int b = a.off[c], n = a.off[c + 1] - b;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (int k = tid; k < n; k += blockDim.x) {
+        const int i = a.nodes[b + k];
+        const float x = a.x[i], y = a.y[i];
+        xmin = fminf(xmin, x);
+        xmax = fmaxf(xmax, x);
+        ymin = fminf(ymin, y);
+        ymax = fmaxf(ymax, y);
+    }
+    xmin = wave_min(xmin);
+    xmax = wave_max(xmax);
+    ymin = wave_min(ymin);
+    ymax = wave_max(ymax);
+    if (lane == 0) {
+        mm[4 * w] = xmin;
+        mm[4 * w + 1] = xmax;
+        mm[4 * w + 2] = ymin;
+        mm[4 * w + 3] = ymax;
+    }
+    __syncthreads();
+    for (int q = 0; q < nw; ++q) {
+        xmin = fminf(xmin, mm[4 * q]);
+        xmax = fmaxf(xmax, mm[4 * q + 1]);
+        ymin = fminf(ymin, mm[4 * q + 2]);
+        ymax = fmaxf(ymax, mm[4 * q + 3]);
+    }
+    const bool fx = (xmax - xmin) > R.hLx, fy = (ymax - ymin) > R.hLy;
+    double m1[2] = {0.0, 0.0};
+    for (int k = tid; k < n; k += blockDim.x) {
+        const int i = a.nodes[b + k];
+        const float x = a.x[i], y = a.y[i];
+        m1[0] += (double)((fx && x > R.hLx) ? x - R.Lx : x);
+        m1[1] += (double)((fy && y > R.hLy) ? y - R.Ly : y);
+    }
+    block_sums_all<2>(m1, red);
+    const float addx = (fx && m1[0] / n < 0.0) ? R.Lx : 0.0f;
+    const float addy = (fy && m1[1] / n < 0.0) ? R.Ly : 0.0f;
+    const double twopi = 6.283185307179586;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = tid; k < n; k += blockDim.x) {
+        const int i = a.nodes[b + k];
+        const float x = a.x[i], y = a.y[i];
+        const float ux = ((fx && x > R.hLx) ? x - R.Lx : x) + addx;
+        const float uy = ((fy && y > R.hLy) ? y - R.Ly : y) + addy;
+        if (a.ux) {
+            a.ux[i] = ux;
+            a.uy[i] = uy;
+        }
+        const double m = a.mass[i];
+        v[0] += m;
+        v[1] += m * ux;
+        v[2] += m * uy;
+        v[3] += m * a.vx[i];
+        v[4] += m * a.vy[i];
+        const double tx = twopi * (double)x / (double)R.Lx, ty = twopi * (double)y / (double)R.Ly;
+        v[5] += cos(tx);
+        v[6] += sin(tx);
+        v[7] += cos(ty);
+        v[8] += sin(ty);
+    }
+    block_sums_all<9>(v, red);
+    if (tid == 0) {
+        const double M = v[0];
+        const double pi = 3.141592653589793;
+        double* o = a.out + (size_t)c * kClusterOut;
+        o[0] = M;
+        o[1] = v[1] / M;
+        o[2] = v[2] / M;
+        o[3] = v[3] / M;
+        o[4] = v[4] / M;
+        o[5] = (double)R.Lx * (atan2(-v[6], -v[5]) + pi) / twopi;
+        o[6] = (double)R.Ly * (atan2(-v[8], -v[7]) + pi) / twopi;
+        a.cm[c] = make_float2((float)o[1], (float)o[2]);
+    }
+}
+
+// Cluster virial (femmd2D/src/interactions.cpp:370-382): over the pairs the
+// force pass takes (cell walk, same predicate and exclusions), for pairs
+// whose particles both belong to clusters, 1/2 f_ij . (R_ci - R_cj) with the
+// minimum image, summed per tile in FP64.
+template <bool kSigma, bool kTypes>
+__global__ __launch_bounds__(256) void k_cluster_virial(ClusterVirialArgs A) {
+    extern __shared__ float4 dyn[];
+    __shared__ int s_cstart[kMaxSlots];
+    __shared__ int s_off[kMaxSlots + 1];
+    __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
+    __shared__ double s_red[2 * 4];
+    const ForceArgs &a = A.f;
+    const int tile = blockIdx.x;
+    PairLds L;
+    L.ent = dyn;
+    int *tail = reinterpret_cast<int *>(dyn + a.lds_cap);
+    L.typ = tail;
+    L.sp = tail + (kTypes ? a.lds_cap : 0);
+    L.cstart = s_cstart;
+    L.off = s_off;
+    L.pt = s_pt;
+    L.red = s_red;
+    const Tile T = a.tiles[tile];
+    const RepDesc R = a.reps[T.rep];
+    const int W = T.w_flags & 0xffff;
+    if (stage_tile<kTypes, false, true>(a, T, R, W, L, nullptr, nullptr) < 0) {
+        if (threadIdx.x == 0) {
+            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
+            A.out[tile] = 0.0;
+        }
+        return;
+    }
+    const bool wrapT = (T.w_flags >> 16) & 1;
+    const int row = W + 2;
+    const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
+    const int nc = c_hi - c_lo;
+    const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
+    const int nt = R.ntypes;
+    double acc = 0.0, dummy = 0.0;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int me = c_lo + k;
+        const int ci = A.cid[L.sp[me]];
+        if (ci < 0) continue;
+        int lo = row + 1, hi = row + W + 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= me) lo = mid; else hi = mid;
+        }
+        const int col = lo - row;
+        const float4 pi = L.ent[me];
+        const float xi = pi.x, yi = pi.y, xim = xi - Lx, yim = yi - Ly;
+        const int gi = __float_as_int(pi.z);
+        const int gix = gi >= 0 ? gi : -2;
+        const float si = pi.w;
+        const int ti = kTypes ? L.typ[me] : 0;
+        const float2 Ri = A.cm[ci];
+        for (int r = 0; r < 3; ++r) {
+            const int j0 = L.off[r * row + col - 1], j1 = L.off[r * row + col + 2];
+            for (int j = j0; j < j1; ++j) {
+                const int cj = A.cid[L.sp[j]];
+                if (cj < 0) continue;
+                const float4 pj = L.ent[j];
+                float dx = xi - pj.x, dy = yi - pj.y;
+                int sx = 0, sy = 0;
+                if (wrapT) {
+                    if (dx >= hLx) { dx -= Lx; sx = 1; } else if (dx < -hLx) { dx += Lx; sx = -1; }
+                    if (dy >= hLy) { dy -= Ly; sy = 1; } else if (dy < -hLy) { dy += Ly; sy = -1; }
+                }
+                const float dr2 = dx * dx + dy * dy;
+                PairEntry pe = L.pt[0];
+                if (kTypes) pe = L.pt[ti * nt + L.typ[j]];
+                float s2ij;
+                unsigned rc2m1;
+                if (kSigma || kTypes) {
+                    const float sij = kSigma ? (si + pj.w) * 0.5f : si;
+                    rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                    s2ij = sij * sij;
+                } else {
+                    rc2m1 = __float_as_uint(R.rc2_uniform) - 1u;
+                    s2ij = R.s2_uniform;
+                }
+                if (!((__float_as_uint(dr2) - 1u) < rc2m1) || __float_as_int(pj.z) == gix) continue;
+                float dxa = dx, dya = dy, r2a = dr2;
+                if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
+                if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);
+                if (sx != 0 || sy != 0) r2a = dxa * dxa + dya * dya;
+                const float inv = __builtin_amdgcn_rcpf(r2a);
+                const float sr2 = s2ij * inv;
+                const float sr6 = sr2 * sr2 * sr2;
+                const float tv = __builtin_fmaf(-0.5f, sr6, sr6 * sr6);
+                const float fc = pe.e48 * tv * inv;
+                const float2 Rj = A.cm[cj];
+                float ddx = Ri.x - Rj.x, ddy = Ri.y - Rj.y;
+                ddx = (ddx >= hLx) ? ddx - Lx : ((ddx < -hLx) ? ddx + Lx : ddx);
+                ddy = (ddy >= hLy) ? ddy - Ly : ((ddy < -hLy) ? ddy + Ly : ddy);
+                acc += 0.5 * ((double)(fc * dxa) * ddx + (double)(fc * dya) * ddy);
+            }
+        }
+    }
+    block_sum2(acc, dummy, s_red);
+    if (threadIdx.x == 0) A.out[tile] = acc;
+}
+
+// Kinetic energy sum (sum m v^2 / 2) of a set of particles (ids, or the
+// first n when null), per workgroup; and the velocity scaling of the same set.
+__global__ __launch_bounds__(256) void k_vel_ke(const int32_t *ids, int n, const float *vx,
+                                                const float *vy, const float *mass,
+                                                double *part) {
+    __shared__ double red[2 * 4];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, z = 0.0;
+    if (k < n) {
+        const int i = ids ? ids[k] : k;
+        const float v2 = vx[i] * vx[i] + vy[i] * vy[i];
+        e = 0.5 * (double)mass[i] * (double)v2;
+    }
+    block_sum2(e, z, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = e;
+}
+
+__global__ __launch_bounds__(256) void k_vel_scale(const int32_t *ids, int n, float *vx, float *vy,
+                                                   float f) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids ? ids[k] : k;
+    vx[i] *= f;
+    vy[i] *= f;
+}
+
+// ---------------------------------------------------------------------------
 // launchers (host side)
 // ---------------------------------------------------------------------------
 hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s) {
@@ -2344,6 +2594,41 @@ hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster_props(const ClusterArgs &a, hipStream_t s) {
+    if (a.ncl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cluster_props, dim3(a.ncl), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster_virial(bool sigma, bool types, int block, const ClusterVirialArgs &A,
+                                 hipStream_t s) {
+    const ForceArgs &a = A.f;
+    if (a.ntiles <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.lds_cap * (sizeof(float4) + sizeof(int) + (types ? sizeof(int) : 0));
+    const dim3 g(a.ntiles), b(block);
+    if (sigma) {
+        if (types) hipLaunchKernelGGL((k_cluster_virial<true, true>), g, b, lds, s, A);
+        else hipLaunchKernelGGL((k_cluster_virial<true, false>), g, b, lds, s, A);
+    } else {
+        if (types) hipLaunchKernelGGL((k_cluster_virial<false, true>), g, b, lds, s, A);
+        else hipLaunchKernelGGL((k_cluster_virial<false, false>), g, b, lds, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_ke(const int32_t *ids, int n, const float *vx, const float *vy,
+                         const float *mass, double *part, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vel_ke, dim3((n + 255) / 256), dim3(256), 0, s, ids, n, vx, vy, mass, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_scale(const int32_t *ids, int n, float *vx, float *vy, float f, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vel_scale, dim3((n + 255) / 256), dim3(256), 0, s, ids, n, vx, vy, f);
     return hipGetLastError();
 }
 

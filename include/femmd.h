@@ -178,6 +178,30 @@ int fm_time_forces(fm_system *h, int32_t reps, double *ms);
  * the binning when cells are walked every step), binning (list mode),
  * counting sort, list build, force pass, whole step. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
+/* Elastomer clusters (one per fm_add_elastomer call, in call order). */
+int fm_cluster_count(fm_system *h, int32_t *nc);
+/* Cluster observables at the current state (FeaElements::calcClusterProps,
+ * femmd2D/src/feaElements.cpp:86-196; MDSystem::evalProps cluster part,
+ * femmd2D/src/md3dsystem.cpp:168-174).  props[7 * c + k], k = mass, centre
+ * of mass x, y (of the unfolded nodes: a cluster wider than half the box has
+ * its nodes beyond L/2 moved down by L, all of it moved up by L if their mean
+ * is then negative), centre-of-mass velocity x, y, centroid x, y (circular
+ * mean of the wrapped positions).  ux, uy (caller order, may be null): the
+ * unfolded positions of cluster nodes, the wrapped position of any other
+ * particle.  The cluster kinetic energy sum is sum_c mass_c |v_cm,c|^2. */
+int fm_cluster_props(fm_system *h, double *props, float *ux, float *uy);
+/* Cluster virial (femmd2D/src/interactions.cpp:370-382): over the
+ * interacting pairs (i, j) of particles that both belong to clusters,
+ * sum of 1/2 f_ij . (R_c(i) - R_c(j)), minimum image.  Pairs with a particle
+ * outside every cluster add nothing (the reference indexes cluster -1 for
+ * them).  The molecular pressure is 0.5 (cluster KE sum - virial) / V. */
+int fm_cluster_virial(fm_system *h, double *virial);
+/* Velocity rescale to temperature temp (MDSystem::setTemperature,
+ * femmd2D/src/md3dsystem.cpp:92-136): n = 0, all particles, with
+ * T = sum(m v^2 / 2) / N; n > 0, the particles ids[0..n), with the
+ * reference's group formula T = 2 sum(m v^2 / 2) / (3 N), N the whole
+ * system's count.  v *= sqrt(temp / T). */
+int fm_set_temperature(fm_system *h, double temp, int32_t n, const int32_t *ids);
 /* Neighbour-list statistics: mean and max list length at the last build,
  * rebuilds so far (all zero when the cells are walked every step). */
 int fm_list_stats(fm_system *h, double *mean_len, int32_t *max_len,

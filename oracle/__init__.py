@@ -272,3 +272,98 @@ class Baseline:
         if getattr(self, "h", None):
             lib().or_bl_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------------------
+# Observables (numpy restatements; test infrastructure only)
+# ---------------------------------------------------------------------------
+def cluster_props(x, y, vx, vy, mass, ranges, Lx, Ly):
+    """Per cluster (caller-id ranges [first, first + n)): the unfolded node
+    positions and (mass, cm x, cm y, vcm x, vcm y, centroid x, centroid y).
+
+    femmd2D/src/feaElements.cpp:86-149 (FeaElements::unfoldPos): positions
+    are FP32 as the reference's real; a cluster wider than L/2 has nodes
+    beyond L/2 moved down by L and, if the mean is then negative, all moved
+    up by L.  The centre of mass and its velocity are mass weighted (sums in
+    FP64).  :161-193 (updateClustersCentroid): circular mean of the wrapped
+    positions, x and y each from their own cos/sin sums (the reference reuses
+    the x sums for y)."""
+    Lxf, Lyf = np.float32(Lx), np.float32(Ly)
+    hx, hy = np.float32(0.5) * Lxf, np.float32(0.5) * Lyf
+    x32 = np.asarray(x, np.float32)
+    y32 = np.asarray(y, np.float32)
+    ux = x32.copy()
+    uy = y32.copy()
+    out = []
+    for first, n in ranges:
+        sl = slice(first, first + n)
+        for u, h, L in ((ux, hx, Lxf), (uy, hy, Lyf)):
+            v = u[sl].copy()
+            if v.max() - v.min() > h:
+                v = np.where(v > h, v - L, v).astype(np.float32)
+                if v.astype(np.float64).sum() / n < 0:
+                    v = (v + L).astype(np.float32)
+            u[sl] = v
+        m = np.asarray(mass[sl], np.float64)
+        M = m.sum()
+        cmx = (m * ux[sl]).sum() / M
+        cmy = (m * uy[sl]).sum() / M
+        vcx = (m * np.asarray(vx[sl], np.float32)).sum() / M
+        vcy = (m * np.asarray(vy[sl], np.float32)).sum() / M
+        tx = 2 * np.pi * x32[sl].astype(np.float64) / np.float64(Lxf)
+        ty = 2 * np.pi * y32[sl].astype(np.float64) / np.float64(Lyf)
+        cx = np.float64(Lxf) * (np.arctan2(-np.sin(tx).sum(), -np.cos(tx).sum()) + np.pi) / (2 * np.pi)
+        cy = np.float64(Lyf) * (np.arctan2(-np.sin(ty).sum(), -np.cos(ty).sum()) + np.pi) / (2 * np.pi)
+        out.append((M, cmx, cmy, vcx, vcy, cx, cy))
+    return np.array(out, np.float64).reshape(-1, 7), ux, uy
+
+
+def cluster_virial(x, y, sigma, excl, cid, cm, Lx, Ly, eps=1.0, rc=2.5, sigma_mode=False):
+    """sum over ordered interacting pairs (i, j), both in clusters, of
+    1/2 f_ij . (R_ci - R_cj) with the minimum image
+    (femmd2D/src/interactions.cpp:370-382; pairs with a particle outside
+    every cluster add nothing).  Pair set: the FP32 predicate of pred32
+    (femmd2D/src/interactionsCuda.cu:79-90) with 0 < dr2; force in FP64 at
+    the image-exact separation.  O(n^2): small systems only."""
+    Lxf, Lyf = np.float32(Lx), np.float32(Ly)
+    x32 = np.asarray(x, np.float32)
+    y32 = np.asarray(y, np.float32)
+    s32 = np.asarray(sigma, np.float32)
+    cid = np.asarray(cid)
+    excl = np.asarray(excl)
+    rcf = np.float32(rc)
+    tot = 0.0
+    absum = 0.0
+    idx = np.nonzero(cid >= 0)[0]
+    for i in idx:
+        j = idx[idx != i]
+        dx = x32[i] - x32[j]
+        sx = np.where(dx >= Lxf * np.float32(0.5), 1, np.where(dx < -Lxf * np.float32(0.5), -1, 0))
+        dx = (dx - sx * Lxf).astype(np.float32)
+        dy = y32[i] - y32[j]
+        sy = np.where(dy >= Lyf * np.float32(0.5), 1, np.where(dy < -Lyf * np.float32(0.5), -1, 0))
+        dy = (dy - sy * Lyf).astype(np.float32)
+        dr2 = (dx * dx + dy * dy).astype(np.float32)
+        sij = ((s32[i] + s32[j]) / np.float32(2)).astype(np.float32)
+        if not sigma_mode:
+            sij = np.full_like(sij, s32[i])
+        rc2 = (rcf * rcf * sij * sij).astype(np.float32)
+        gi, gj = excl[i], excl[j]
+        ok = (dr2 > 0) & (dr2 < rc2) & ~((gi >= 0) & (gj == gi))
+        j = j[ok]
+        if j.size == 0:
+            continue
+        ddx = np.float64(x32[i]) - (x32[j].astype(np.float64) + sx[ok] * np.float64(Lxf))
+        ddy = np.float64(y32[i]) - (y32[j].astype(np.float64) + sy[ok] * np.float64(Lyf))
+        r2 = ddx * ddx + ddy * ddy
+        s2 = sij[ok].astype(np.float64) ** 2
+        s6 = (s2 / r2) ** 3
+        g = 48.0 * eps * (s6 * s6 - 0.5 * s6) / r2
+        dRx = cm[cid[i], 0] - cm[cid[j], 0]
+        dRy = cm[cid[i], 1] - cm[cid[j], 1]
+        dRx -= Lx * np.round(dRx / Lx)
+        dRy -= Ly * np.round(dRy / Ly)
+        t = 0.5 * (g * ddx * dRx + g * ddy * dRy)
+        tot += t.sum()
+        absum += np.abs(t).sum()
+    return tot, absum
