@@ -74,6 +74,8 @@ class System:
         self._kbond = 0.0
         self._committed = False
         self._nelem = 0
+        # caller-order records for the file writers (femmd_old_amd.io)
+        self._rec_typ, self._rec_sig, self._rec_tri, self._rec_bond = [], [], [], []
 
     # -- plumbing -------------------------------------------------------------
     def _check(self, rc: int, h="self"):
@@ -121,6 +123,8 @@ class System:
         first = C.c_int32()
         self._check(self._lib.fm_add_particles(self._h, n, _fp(x), _fp(y), _fp(vx), _fp(vy),
                                                _fp(m), _fp(sig), _ip(typ), C.byref(first)))
+        self._rec_typ.append(np.zeros(n, np.int32) if typ is None else typ.copy())
+        self._rec_sig.append(sig.copy())
         return first.value
 
     def add_elastomer(self, nodes, cells, x, y, typ=0, g=0, sig=1.0, mass=0.0, exclude=True,
@@ -146,11 +150,28 @@ class System:
             float(sig), float(node_mass), float(mass), int(bool(exclude)), _fp(Ea), _fp(nua),
             C.byref(first)))
         self._nelem += nt
+        self._rec_typ.append(np.full(nv, int(typ), np.int32))
+        self._rec_sig.append(np.full(nv, float(sig), np.float32))
+        self._rec_tri.append(t.reshape(-1, 3) - int(index_base) + first.value)
         return first.value
 
     @property
     def n_elements(self) -> int:
         return self._nelem
+
+    def particle_types(self) -> np.ndarray:
+        """Type of every particle, caller order."""
+        return np.concatenate(self._rec_typ) if self._rec_typ else np.zeros(0, np.int32)
+
+    def particle_sigmas(self) -> np.ndarray:
+        return np.concatenate(self._rec_sig) if self._rec_sig else np.zeros(0, np.float32)
+
+    def triangles(self) -> np.ndarray:
+        """(nt, 3) global 0-based node ids of every element, in order."""
+        return np.concatenate(self._rec_tri) if self._rec_tri else np.zeros((0, 3), np.int32)
+
+    def bond_pairs(self) -> np.ndarray:
+        return np.concatenate(self._rec_bond) if self._rec_bond else np.zeros((0, 2), np.int32)
 
     def set_kBond(self, k: float):
         """System.set_kBond (femmd2D/src/interface.cpp:455-465): the stiffness
@@ -168,6 +189,7 @@ class System:
         k = _f32(np.broadcast_to(np.asarray(k, np.float32), (n,)))
         r0 = None if r0 is None else _f32(np.broadcast_to(np.asarray(r0, np.float32), (n,)))
         self._check(self._lib.fm_add_bonds(self._h, n, _ip(i), _ip(j), _fp(k), _fp(r0)))
+        self._rec_bond.append(np.stack([i, j], 1))
 
     def set_pair_potential(self, eps, rc, us):
         """System.set_pair_potential(eps, rc, us) (femmd2D/src/interface.cpp:493-531):
@@ -260,6 +282,29 @@ class System:
 
     def info(self) -> dict:
         return {"N": self.N, "box": self.get_box(), "dt": self._dt}
+
+    # -- trajectory files (femmd2D/src/md3dsystem.cpp:232-377) --------------
+    def save_xyz(self, name: str, s: int = 0):
+        from . import io
+        io.save_xyz(self, name, s)
+
+    def save_vtk(self, prefix: str) -> str:
+        """System.save_vtk (femmd2D/src/interface.cpp:664)."""
+        from . import io
+        return io.save_vtk(self, prefix)
+
+    def save_vtf(self, name: str, s: int = 0):
+        """System.save_vtf (femmd2D/src/interface.cpp:665)."""
+        from . import io
+        io.save_vtf(self, name, s)
+
+    def save_cluster_vtf(self, name: str, s: int = 0, scale: float = 1.0):
+        from . import io
+        io.save_cluster_vtf(self, name, s, scale)
+
+    def save_cluster_xyz(self, name: str, s: int = 0):
+        from . import io
+        io.save_cluster_xyz(self, name, s)
 
     # -- observables (femmd2D/src/feaElements.cpp:86-196,
     #    femmd2D/src/md3dsystem.cpp:92-179) ---------------------------------
