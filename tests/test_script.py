@@ -1,0 +1,68 @@
+"""Script front end on the CPU: the set-up recorded by the ``System`` global
+is replayed into the engine at allocate_cuda (a recording stand-in engine
+here), with the reference's set_velocities (speed v0, zero net momentum),
+element masses spread as rho A / 3, and the helper functions."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from femmd_old_amd import script
+
+MESH = Path(__file__).resolve().parent / "golden" / "meshes"
+
+
+class _Rec:
+    """Records the engine calls the facade makes."""
+    last = None
+
+    def __init__(self, Lx, Ly, dt=0.005):
+        self.calls = [("create", Lx, Ly, dt)]
+        _Rec.last = self
+
+    def __getattr__(self, name):
+        def f(*a, **k):
+            self.calls.append((name,) + a)
+        return f
+
+
+def test_script_replays_setup(tmp_path):
+    S = script.ScriptSystem(engine_factory=_Rec)
+    S.init(10.0, 10.0)
+    S.set_dt(0.001)
+    S.set_box(40.0, 40.0)
+    nodes = script.read_file(str(MESH / "squareVertices.txt"), script.FILECONTENT.FLOAT)
+    cells = script.read_file(str(MESH / "squareTriangles.txt"), script.FILECONTENT.INT)
+    S.set_pair_potential([1.0], [2 ** (1 / 6)], [1.0])
+    S.set_youngs_modulus(50.0)
+    S.add_particle(1.0, 2.0, 0.0, 0.0, 1.0, 1.0, 0)
+    S.add_elastomer(nodes * 8, cells, 20.0, 20.0, 0, 0, 1.0, 67.0, True)
+    S.set_velocities(1.3)
+    info = S.info()
+    assert info["N"] == 1 + 67 and info["nelastomers"] == 1
+    S.allocate_cuda()
+    e = _Rec.last
+    names = [c[0] for c in e.calls]
+    assert names[:6] == ["create", "set_pair_potential", "add_particles", "add_elastomer",
+                         "set_youngs_modulus", "set_state"]
+    assert e.calls[0] == ("create", 40.0, 40.0, 0.001)
+    vx, vy = e.calls[5][3], e.calls[5][4]
+    # set_velocities: zero net momentum with the element-area masses
+    m = np.concatenate([[1.0], script._node_masses((nodes * 8).reshape(-1, 2).astype(float),
+                                                   cells.reshape(-1, 3) - 1, 67.0, 0.0)])
+    assert abs(np.sum(m * vx)) < 1e-9 and abs(np.sum(m * vy)) < 1e-9
+    assert m[1:].sum() == pytest.approx(67.0)
+    with pytest.raises(RuntimeError):
+        S.add_particle(0, 0, 0, 0, 1, 1, 0)
+    with pytest.raises(NotImplementedError):
+        S.set_gravity(1.0)
+
+
+def test_script_helpers(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    script.make_dir("d")
+    script.change_dir("d")
+    script.write_to_file("a.txt", "1 ")
+    script.append_to_file("a.txt", "2\n")
+    assert Path("a.txt").read_text() == "1 2\n"
+    assert script.read_file("a.txt", script.FILECONTENT.INT).tolist() == [1, 2]
