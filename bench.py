@@ -54,8 +54,9 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
     sort     as above plus spos and the sorted x,y: 48/particle (rebuild steps)
     list     r sorted (16); w slots, length: 20 + 2(L+1.5)/particle;
              r start: 4/cell                                   (rebuild steps)
-    forces   r sorted x,y (8), length, slots; w pair force (8):
-             20 + 2(L+1.5)/particle; r start: 4/cell; elements and bonds as above
+    forces   r sorted x,y (8), length, slots, internal id (4); w pair force
+             (8, internal order): 24 + 2(L+1.5)/particle; r start: 4/cell;
+             elements and bonds as above
     """
     corners = 36 * ntri + 24 * nbond
     elems = 100 * ntri + 48 * nbond
@@ -74,7 +75,7 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
             "bin": 12 * n + 8 * ncells,
             "sort": 48 * n + 12 * ncells,
             "list_build": int((20 + slots) * n) + 4 * ncells,
-            "forces": int((20 + slots) * n) + 4 * ncells + elems,
+            "forces": int((24 + slots) * n) + 4 * ncells + elems,
         }
     return b
 
