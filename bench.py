@@ -215,7 +215,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=160)
+    ap.add_argument("--cpu-threads", type=int, default=8,
+                    help="threads of the CPU baseline (the reference's pair loop runs "
+                         "num_threads(8), femmd2D/src/interactions.cpp:354)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
@@ -312,7 +315,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, 1)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 
