@@ -320,7 +320,12 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
             flag = list_decide(a, sp, R);
             last_rep = sp.rep;
         }
-        if (!flag) continue;
+        if (!flag) {
+            // on to this workgroup's first span of the next replica
+            const int e = R.ks0 + R.nks, g = (int)gridDim.x;
+            if (e > b + g) b += ((e - b - 1) / g) * g;
+            continue;
+        }
     }
     const int p = sp.begin + (int)threadIdx.x;
     const bool active = p < sp.end;
@@ -803,6 +808,9 @@ __device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R,
 // Lanes stride over all M entries of the tile, kStageBatch per lane at a
 // time with every load issued before the first store.
 constexpr int kStageBatch = 4;
+// Position of the list-padding slot staged after a tile's particles: far
+// from every particle (squared distances stay finite and beyond any cutoff).
+constexpr float kSentinel = 1.0e18f;
 template <class Ld, class St>
 __device__ void stage_copy(const Tile &T, const RepDesc &R, int W, const PairLds &L, Ld &&ld,
                            St &&st) {
@@ -976,9 +984,9 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
             }
         }
         const int m = min(cnt, cap);
-        // groups of four, padded with the particle itself (distance 0: the
-        // pair pass's cutoff test rejects it); cap is a multiple of 8
-        for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, 8 * me);
+        // groups of four, padded with the sentinel slot (staged far away:
+        // the pair pass's cutoff test rejects it); cap is a multiple of 8
+        for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, 8 * a.lds_cap);
         a.nnum[p] = m | (k << 16);  // list slot p holds particle k of the tile
         if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
@@ -1108,14 +1116,15 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
         const int rank = list_rank((m + 3) >> 2, act, cap >> 2, hist);
         if (act) {
             const int p = sbase + k0 + rank;  // list slot
-            // groups of four, padded with the particle itself (distance 0:
-            // the pair pass's cutoff test rejects it); cap is a multiple of 8.
+            // groups of four, padded with the sentinel slot (staged far
+            // away: the pair pass's cutoff test rejects it); cap is a
+            // multiple of 8.
             for (int g = 0; g < ((m + 3) >> 2); ++g) {
                 int jj[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int kk = 4 * g + t;
-                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : 8 * me;
+                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : 8 * a.lds_cap;
                 }
                 list_store_group(a, g, p, jj);
             }
@@ -1175,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // neighbours from LDS (the pair maths and accumulation of pair_tile_body;
 // the FP32 cutoff predicate on the current positions, so the interacting
 // set is the cell walk's whenever the list is complete).
-template <bool kWrap, bool kSigma, bool kEnergy>
+template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit>
 __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W, int tile,
                                     const PairLds &L, const float2 *lp, const float *ls) {
     const int row = W + 2;
@@ -1184,7 +1193,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
-    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    const float rc2_u = R.rc2_uniform;
     const uint32_t Nu = (uint32_t)a.n;
     const int capg = a.nbr_cap >> 2;
     const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
@@ -1200,12 +1209,12 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
         const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
-        const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the particle itself
+        const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the sentinel slot
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
         // one list entry, without branches: the force of a non-interacting
-        // entry (or of the padding entry, the particle itself) is computed
-        // and discarded by a select
+        // entry (or of the padding entry) is computed and discarded by a
+        // select
         auto term = [&](float2 pj, float sj) {
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
@@ -1220,17 +1229,18 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             const float dx = ex - ox;
             const float dy = ey - oy;
             const float dr2 = dx * dx + dy * dy;
-            float s2ij;
-            unsigned rc2m1;
+            float s2ij, rc2;
             if (kSigma) {
                 const float sij = (si + sj) * 0.5f;
-                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                rc2 = pe.rc * pe.rc * sij * sij;
                 s2ij = sij * sij;
             } else {
-                rc2m1 = rc2m1_u;
+                rc2 = rc2_u;
                 s2ij = R.s2_uniform;
             }
-            const bool hit = (__float_as_uint(dr2) - 1u) < rc2m1;
+            // femmd2D/src/interactionsCuda.cu:88-90 (a listed entry is never
+            // the particle itself: the padding is the far sentinel slot)
+            const bool hit = dr2 < rc2;
             float dxa = dx, dya = dy, r2a = dr2;
             if (kWrap) {
                 // image-exact difference: the far coordinate is shifted by L
@@ -1240,7 +1250,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
                 r2a = dxa * dxa + dya * dya;
             }
             const float inv = __builtin_amdgcn_rcpf(r2a);
-            const float sr2 = s2ij * inv;
+            const float sr2 = kUnit ? inv : s2ij * inv;  // sigma^2 = 1: the same bits
             const float sr4 = sr2 * sr2;
             const float sr6 = sr4 * sr2;
             const float s12 = sr6 * sr6;
@@ -1330,11 +1340,19 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
             lp[e] = make_float2(u.x, u.y);
             if (kSigma) ls[e] = u.z;
         });
+    if (threadIdx.x == 0) {  // the padding entries' slot, far from any particle
+        lp[a.lds_cap] = make_float2(kSentinel, kSentinel);
+        if (kSigma) ls[a.lds_cap] = 1.0f;
+    }
     __syncthreads();
-    if ((T.w_flags >> 16) & 1)
-        list_tile_pass_body<true, kSigma, kEnergy>(a, R, W, tile, L, lp, ls);
-    else
-        list_tile_pass_body<false, kSigma, kEnergy>(a, R, W, tile, L, lp, ls);
+    const bool wrapT = (T.w_flags >> 16) & 1;
+    if (!kSigma && R.s2_uniform == 1.0f) {
+        if (wrapT) list_tile_pass_body<true, kSigma, kEnergy, true>(a, R, W, tile, L, lp, ls);
+        else list_tile_pass_body<false, kSigma, kEnergy, true>(a, R, W, tile, L, lp, ls);
+    } else {
+        if (wrapT) list_tile_pass_body<true, kSigma, kEnergy, false>(a, R, W, tile, L, lp, ls);
+        else list_tile_pass_body<false, kSigma, kEnergy, false>(a, R, W, tile, L, lp, ls);
+    }
 }
 
 // CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
@@ -1516,7 +1534,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             L.pt = s_pt;
             L.red = s_red;
             float2 *lp = reinterpret_cast<float2 *>(dyn);
-            float *ls = reinterpret_cast<float *>(lp + a.lds_cap);
+            float *ls = reinterpret_cast<float *>(lp + a.lds_cap + 1);
             list_tile_pass<kSigma, kEnergy>(a, xcd_remap(b, a.ntiles), L, lp, ls);
         }
     } else if (b < a.ntiles) {
@@ -1900,6 +1918,11 @@ __device__ void adj_list_tile(const AdjArgs &A, int tile, PairLds &L, float2 *lp
             lw[e] = u.w;
             if (kSigma) ls[e] = u.s;
         });
+    if (threadIdx.x == 0) {  // the padding entries' slot
+        lp[a.lds_cap] = make_float2(kSentinel, kSentinel);
+        lw[a.lds_cap] = make_float2(0.0f, 0.0f);
+        if (kSigma) ls[a.lds_cap] = 1.0f;
+    }
     __syncthreads();
     if ((T.w_flags >> 16) & 1)
         adj_list_body<true, kSigma>(A, R, W, tile, L, lp, lw, ls);
@@ -2066,8 +2089,8 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
         L.red = s_red;
         if (kList) {
             float2 *lp = reinterpret_cast<float2 *>(dyn);
-            float2 *lw = lp + a.lds_cap;
-            float *ls = reinterpret_cast<float *>(lw + a.lds_cap);
+            float2 *lw = lp + a.lds_cap + 1;
+            float *ls = reinterpret_cast<float *>(lw + a.lds_cap + 1);
             adj_list_tile<kSigma>(A, xcd_remap(b, a.ntiles), L, lp, lw, ls);
         } else {
             float2 *lw = reinterpret_cast<float2 *>(dyn + a.lds_cap);
@@ -2422,7 +2445,7 @@ hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A,
     const int nblk = a.ntiles + a.nespans + a.nbspans;
     if (nblk <= 0) return hipSuccess;
     if (list) {
-        const size_t lds = (size_t)a.lds_cap * (2 * sizeof(float2) + (sigma ? sizeof(float) : 0));
+        const size_t lds = (size_t)(a.lds_cap + 1) * (2 * sizeof(float2) + (sigma ? sizeof(float) : 0));
         if (sigma)
             hipLaunchKernelGGL((k_adj_forces<true, true>), dim3(nblk), dim3(block), lds, s, A);
         else
@@ -2513,7 +2536,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
         const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
         if (nblk <= 0) return hipSuccess;
         dim3 g(nblk), b(block);
-        const size_t lds = (size_t)a.lds_cap * (sizeof(float2) + (sigma ? sizeof(float) : 0));
+        const size_t lds = (size_t)(a.lds_cap + 1) * (sizeof(float2) + (sigma ? sizeof(float) : 0));
 #define FM_LIST(S, E) hipLaunchKernelGGL((k_forces<S, false, false, true, E>), g, b, lds, s, a)
         if (sigma) { if (energy) FM_LIST(true, true); else FM_LIST(true, false); }
         else { if (energy) FM_LIST(false, true); else FM_LIST(false, false); }
