@@ -9,7 +9,9 @@ every particle at these sizes, after steps in neighbour-list mode (forces
 from lists built steps earlier).  Size-independent properties cover the
 rest of a run: the total momentum (every pair, bond and element force sums
 to zero, so only FP32 rounding moves it), bitwise repetition of a run, and
-the list and cell paths agreeing to FP32 rounding.
+the list and cell paths agreeing to FP32 rounding.  C4 (512 replicas,
+3.8 M particles) runs as one tuned batch: every replica's energies against
+the oracle, three replicas bitwise against the same system stepped alone.
 """
 import numpy as np
 import pytest
@@ -87,3 +89,45 @@ def test_full_size_c2_run_repeats_bitwise_and_paths_agree():
     assert d.max() < 1e-3
     for s in (a, b, c):
         assert s.device_errors() == 0
+
+
+def test_full_size_c4_batch():
+    """All 512 replicas (3.8 M particles) in one batch, tuned: every
+    replica's energies at its final positions against the FP64 oracle, and
+    the smallest, the largest and a disk replica bitwise equal to the same
+    system stepped alone."""
+    from femmd_old_amd import Batch
+    ws = configs.c4()
+    b = Batch([configs.build_system(w, commit=False) for w in ws])
+    b.tune()
+    b.run(40)
+    assert b.device_errors() == 0
+    sizes = np.array([w.n_total for w in ws])
+    disk = next(r for r, w in enumerate(ws) if w.elastomers)
+    picks = sorted({int(sizes.argmin()), int(sizes.argmax()), disk})
+    for r in picks:
+        s = configs.build_system(ws[r])
+        s.run(40)
+        for u, v in zip(s.get_state(), b.get_state(r)):
+            np.testing.assert_array_equal(u, v)
+    # W_LJ of a replica near zero pressure is a sum of large terms that
+    # cancel (-184 from terms of ~1e5 in magnitude): its error is measured
+    # against the scale of those terms, bounded by 1/2 rc sum_i sum_j |f_ij|
+    # (r_ij < rc; the oracle's per-particle sum of force-term magnitudes)
+    worst = {}
+    for r, w in enumerate(ws):
+        x, y, _, _ = b.get_state(r)
+        _, _, en, fs = O.forces(O.from_workload(w, x, y), pred_f32=True, scale=True)
+        e = b.energies(r)
+        scale_u = abs(en[O.OR_U_LJ]) + abs(en[O.OR_U_FEA]) + 1.0
+        smax = max([float(np.max(w.sigma))] + [float(el.sigma) for el in w.elastomers])
+        scale_w = abs(en[O.OR_W_LJ]) + 0.5 * w.rc * smax * float(fs[0].sum()) + 1.0
+        for k, v in (("u_lj", abs(e["u_lj"] - en[O.OR_U_LJ]) / scale_u),
+                     ("u_fea", abs(e["u_fea"] - en[O.OR_U_FEA]) / scale_u),
+                     ("w_lj", abs(e["w_lj"] - en[O.OR_W_LJ]) / scale_w)):
+            if v > worst.get(k, (0.0,))[0]:
+                worst[k] = (v, r, w.n_total, e[k], en[{"u_lj": O.OR_U_LJ, "u_fea": O.OR_U_FEA,
+                                                       "w_lj": O.OR_W_LJ}[k]])
+    print(f"[c4] {len(ws)} replicas, {sizes.sum()} particles: worst (rel err, replica, n, "
+          f"engine, oracle) {worst}")
+    assert max(v[0] for v in worst.values()) <= 1e-5
