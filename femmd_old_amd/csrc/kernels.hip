@@ -627,7 +627,7 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f, ush = 0.f;
         int nint = 0;
         long long s1 = 0, s2 = 0;
-        const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+        const unsigned rc2b_u = __float_as_uint(R.rc2_uniform);
         // the next candidate's LDS entry is read one iteration ahead
         float4 pn = L.ent[total > 0 ? j : 0];
         for (int t = 0; t < total; ++t) {
@@ -647,27 +647,28 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             }
             const float dr2 = dx * dx + dy * dy;
             float s2ij, e48 = 0.f, e4 = 0.f, shift = 0.f;
-            unsigned rc2m1;
+            unsigned rc2b;
             if (kTypes) {
                 const PairEntry pe = L.pt[ti * nt + (kTypes ? L.typ[j] : 0)];
                 e48 = pe.e48; e4 = pe.e4; shift = pe.shift;
                 const float sij = kSigma ? (si + pj.w) * 0.5f : si;
-                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                 s2ij = sij * sij;
             } else if (kSigma) {
                 const float sij = (si + pj.w) * 0.5f;
                 const float rc = L.pt[0].rc;
-                rc2m1 = __float_as_uint(rc * rc * sij * sij) - 1u;
+                rc2b = __float_as_uint(rc * rc * sij * sij);
                 s2ij = sij * sij;
             } else {
-                rc2m1 = rc2m1_u;
+                rc2b = rc2b_u;
                 s2ij = R.s2_uniform;
             }
-            // 0 < dr2 < rc2 on the bit patterns of non-negative floats: the
-            // cutoff predicate dr2 < rc2 exactly, and the pair with itself
-            // (dr2 == 0, the only zero-distance pair of distinct slots in a
-            // valid configuration) falls out with it
-            const bool hit = ((__float_as_uint(dr2) - 1u) < rc2m1) & (gj != gix);
+            // the cutoff predicate dr2 < rc2 on the bit patterns of
+            // non-negative floats; the centre's own slot is left out by index
+            // (i != j of femmd2D/src/nn.cpp:44-46: a distinct particle on the
+            // same spot interacts, as in the reference; the 3 x 3 stencil
+            // holds every particle once)
+            const bool hit = (__float_as_uint(dr2) < rc2b) & (j != me) & (gj != gix);
             if (hit) {
                 // force from the image-exact difference: the far coordinate
                 // is shifted by L before subtracting (exact by Sterbenz)
@@ -1672,7 +1673,7 @@ __device__ void adj_pair_body(const AdjArgs &A, const RepDesc &R, int W, int til
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
-    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    const unsigned rc2b_u = __float_as_uint(R.rc2_uniform);
     double peps = 0.0, psig = 0.0;
     for (int k = threadIdx.x; k < nc; k += blockDim.x) {
         const int me = c_lo + k;
@@ -1714,17 +1715,17 @@ __device__ void adj_pair_body(const AdjArgs &A, const RepDesc &R, int W, int til
             }
             const float dr2 = dx * dx + dy * dy;
             float s2ij, sij;
-            unsigned rc2m1;
+            unsigned rc2b;
             if (kSigma) {
                 sij = (si + pj.w) * 0.5f;
-                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                 s2ij = sij * sij;
             } else {
                 sij = R.s_uniform;
-                rc2m1 = rc2m1_u;
+                rc2b = rc2b_u;
                 s2ij = R.s2_uniform;
             }
-            const bool hit = ((__float_as_uint(dr2) - 1u) < rc2m1) & (gj != gix);
+            const bool hit = (__float_as_uint(dr2) < rc2b) & (j != me) & (gj != gix);
             if (hit) {
                 float dxa = dx, dya = dy, r2a = dr2;
                 if (kWrap) {
@@ -1798,7 +1799,7 @@ __device__ void adj_list_body(const AdjArgs &A, const RepDesc &R, int W, int til
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
-    const unsigned rc2m1_u = __float_as_uint(R.rc2_uniform) - 1u;
+    const unsigned rc2b_u = __float_as_uint(R.rc2_uniform);
     const uint32_t Nu = (uint32_t)a.n;
     const int capg = a.nbr_cap >> 2;
     const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
@@ -1829,17 +1830,17 @@ __device__ void adj_list_body(const AdjArgs &A, const RepDesc &R, int W, int til
             const float dx = ex - ox, dy = ey - oy;
             const float dr2 = dx * dx + dy * dy;
             float s2ij, sij;
-            unsigned rc2m1;
+            unsigned rc2b;
             if (kSigma) {
                 sij = (si + ls[off >> 3]) * 0.5f;
-                rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                 s2ij = sij * sij;
             } else {
                 sij = R.s_uniform;
-                rc2m1 = rc2m1_u;
+                rc2b = rc2b_u;
                 s2ij = R.s2_uniform;
             }
-            if ((__float_as_uint(dr2) - 1u) < rc2m1) {
+            if (__float_as_uint(dr2) < rc2b) {
                 float dxa = dx, dya = dy, r2a = dr2;
                 if (kWrap) {
                     dxa = (ox > 0.0f) ? (xim - pj.x) : ((ox < 0.0f) ? (xi - (pj.x - Lx)) : dx);
@@ -2380,16 +2381,16 @@ __global__ __launch_bounds__(256) void k_cluster_virial(ClusterVirialArgs A) {
                 PairEntry pe = L.pt[0];
                 if (kTypes) pe = L.pt[ti * nt + L.typ[j]];
                 float s2ij;
-                unsigned rc2m1;
+                unsigned rc2b;
                 if (kSigma || kTypes) {
                     const float sij = kSigma ? (si + pj.w) * 0.5f : si;
-                    rc2m1 = __float_as_uint(pe.rc * pe.rc * sij * sij) - 1u;
+                    rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                     s2ij = sij * sij;
                 } else {
-                    rc2m1 = __float_as_uint(R.rc2_uniform) - 1u;
+                    rc2b = __float_as_uint(R.rc2_uniform);
                     s2ij = R.s2_uniform;
                 }
-                if (!((__float_as_uint(dr2) - 1u) < rc2m1) || __float_as_int(pj.z) == gix) continue;
+                if (!(__float_as_uint(dr2) < rc2b) || j == me || __float_as_int(pj.z) == gix) continue;
                 float dxa = dx, dya = dy, r2a = dr2;
                 if (sx > 0) dxa = xim - pj.x; else if (sx < 0) dxa = xi - (pj.x - Lx);
                 if (sy > 0) dya = yim - pj.y; else if (sy < 0) dya = yi - (pj.y - Ly);

@@ -322,9 +322,9 @@ def cluster_virial(x, y, sigma, excl, cid, cm, Lx, Ly, eps=1.0, rc=2.5, sigma_mo
     """sum over ordered interacting pairs (i, j), both in clusters, of
     1/2 f_ij . (R_ci - R_cj) with the minimum image
     (femmd2D/src/interactions.cpp:370-382; pairs with a particle outside
-    every cluster add nothing).  Pair set: the FP32 predicate of pred32
-    (femmd2D/src/interactionsCuda.cu:79-90) with 0 < dr2; force in FP64 at
-    the image-exact separation.  O(n^2): small systems only."""
+    every cluster add nothing).  Pair set: i != j by index and the FP32
+    predicate of pred32 (femmd2D/src/interactionsCuda.cu:79-90); force in
+    FP64 at the image-exact separation.  O(n^2): small systems only."""
     Lxf, Lyf = np.float32(Lx), np.float32(Ly)
     x32 = np.asarray(x, np.float32)
     y32 = np.asarray(y, np.float32)
@@ -349,7 +349,7 @@ def cluster_virial(x, y, sigma, excl, cid, cm, Lx, Ly, eps=1.0, rc=2.5, sigma_mo
             sij = np.full_like(sij, s32[i])
         rc2 = (rcf * rcf * sij * sij).astype(np.float32)
         gi, gj = excl[i], excl[j]
-        ok = (dr2 > 0) & (dr2 < rc2) & ~((gi >= 0) & (gj == gi))
+        ok = (dr2 < rc2) & ~((gi >= 0) & (gj == gi))
         j = j[ok]
         if j.size == 0:
             continue

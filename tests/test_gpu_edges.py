@@ -1,0 +1,127 @@
+"""Edge cases of the device path, through the C ABI.
+
+  * an empty system (argument error) and a box narrower than 3 cells
+    (geometry error) are refused at commit
+    (the cell stencil needs 3 distinct cells per axis);
+  * one particle in free flight across the periodic edge is bit-exact with
+    the FP32 restatement x <- wrap(x + dt v) (oracle.wrap_f32, the
+    reference's pbcCalc, femmd2D/src/def.h:112-129), including a wrap that
+    rounds to exactly L, whose cell is the reference's clamp to ncx - 1
+    (femmd2D/src/nnCuda.cu:9-17) and agrees with oracle.cell_keys;
+  * the smallest box (3 x 3 cells) with particles in every cell: forces
+    against the FP64 oracle, both pair paths;
+  * two distinct particles on one spot interact (self is excluded by index,
+    as in the reference) with an infinite LJ force, which is reported as a
+    non-finite state, not a fault.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from femmd_old_amd import FemmdError, System, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(L, skin, dt=0.005):
+    s = System(L, L, dt=dt)
+    s.set_skin(skin)
+    s.set_pair_potential([1.0], [2.5], [0.0])
+    return s
+
+
+def test_empty_system_is_refused():
+    s = System(20.0, 20.0)
+    with pytest.raises(FemmdError) as e:
+        s.allocate_cuda()
+    assert e.value.code == _lib.FM_ERR_ARG
+    s.close()
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_box_below_three_cells_is_refused(skin):
+    s = _system(2.9 * (2.5 + skin), skin)
+    s.add_particles([1.0, 3.0], [1.0, 1.0])
+    with pytest.raises(FemmdError) as e:
+        s.allocate_cuda()
+    assert e.value.code == _lib.FM_ERR_GEOMETRY
+    assert "box too small" in str(e.value)
+    s.close()
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_single_particle_free_flight_bit_exact(skin):
+    L = np.float32(10.0)
+    dt = np.float32(0.005)
+    # the first step lands at -1e-7, which wraps to exactly L in FP32
+    x0, vx = np.float32(0.005), np.float32(-1.00002)
+    y0, vy = np.float32(9.999), np.float32(0.731)
+    s = _system(float(L), skin)
+    s.add_particles([x0], [y0], vx=[vx], vy=[vy])
+    s.allocate_cuda()
+    x = np.array([x0], np.float32)
+    y = np.array([y0], np.float32)
+    for k in range(1, 41):
+        s.run(1)
+        x = (x + dt * vx).astype(np.float32)
+        y = (y + dt * vy).astype(np.float32)
+        O.wrap_f32(x, y, float(L), float(L))
+        ex, ey, evx, evy = s.get_state()
+        assert ex[0] == x[0] and ey[0] == y[0], (k, ex[0], x[0], ey[0], y[0])
+        assert evx[0] == vx and evy[0] == vy
+        if k == 1:
+            assert x[0] == L  # the rounding case of the reference's wrap
+            ncx, ncy, key, _, _ = s.cells()
+            np.testing.assert_array_equal(key, O.cell_keys(ex, ey, float(L), float(L), ncx, ncy))
+            assert key[0] % ncx == ncx - 1
+    assert s.device_errors() == 0
+    s.close()
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_smallest_box_forces(skin):
+    L = 3.0 * (2.5 + skin) + 0.01  # 3 x 3 cells
+    rng = np.random.default_rng(5)
+    g = np.arange(0.4, L - 0.4, 1.1)
+    gx, gy = np.meshgrid(g, g)
+    x = (gx.ravel() + rng.uniform(-0.1, 0.1, gx.size)).astype(np.float32)
+    y = (gy.ravel() + rng.uniform(-0.1, 0.1, gy.size)).astype(np.float32)
+    s = _system(L, skin)
+    s.add_particles(x, y)
+    s.allocate_cuda()
+    ncx, ncy, key, _, _ = s.cells()
+    assert (ncx, ncy) == (3, 3)
+    xs, ys, _, _ = s.get_state()
+    Lf = O.box_f32(L)
+    osys = O.OracleSystem(Lf, Lf, xs, ys, eps=[1.0], rc=[2.5], ushift=[0.0])
+    fx, fy = s.get_forces()
+    rx, ry, en, fs = O.forces(osys, pred_f32=True, scale=True)
+    err = O.force_error(fx, fy, rx, ry, fs)
+    print(f"[3x3 cells skin {skin}] n {x.size} max rel force err {err.max():.3e}")
+    assert err.max() <= 1e-5
+    assert s.device_errors() == 0
+    s.close()
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_coincident_particles_are_reported(skin):
+    """Both pair paths leave a particle's own slot out by index, as the
+    reference does (i != j), so a distinct particle on the same spot is an
+    interacting pair (the oracle's pair sets count it) with an infinite
+    force: reported as a non-finite state, not a fault."""
+    s = _system(20.0, skin)
+    s.add_particles([5.0, 5.0, 12.0], [5.0, 5.0, 12.0])
+    s.allocate_cuda()
+    x, y, _, _ = s.get_state()
+    cnt, s1, _ = s.pair_sets()
+    ocnt, os1, _ = O.pair_sets(O.OracleSystem(O.box_f32(20.0), O.box_f32(20.0), x, y), 1)
+    np.testing.assert_array_equal(cnt, ocnt)
+    np.testing.assert_array_equal(s1, os1)
+    assert list(cnt) == [1, 1, 0]
+    with pytest.raises(FemmdError) as e:
+        s.run(2)
+    assert e.value.code == _lib.FM_ERR_DEVICE
+    assert s.device_errors() & _lib.FM_DEV_NONFINITE
+    s.clear_errors()
+    assert s.device_errors() == 0
+    s.close()
