@@ -423,6 +423,23 @@ class System:
         self._check(self._lib.fm_list_stats(self._h, C.byref(m), C.byref(mx), C.byref(nb)))
         return {"mean": m.value, "max": mx.value, "rebuilds": nb.value}
 
+    def set_reorder_interval(self, steps: int):
+        """Re-sort the internal order of the particles that no element, bond
+        or cluster refers to into the current cell order every `steps` steps
+        (default 1000; 0 = never).  Keeps the step's memory accesses local as
+        particles diffuse; caller ids never change."""
+        self._check(self._lib.fm_set_reorder_interval(self._h, int(steps)))
+
+    def reorder(self):
+        """Re-sort the internal order now (see set_reorder_interval)."""
+        self._check(self._lib.fm_reorder(self._h))
+
+    def reorder_info(self) -> dict:
+        """{"interval", "count"}: the re-sort interval and the re-sorts so far."""
+        iv, n = C.c_int32(), C.c_int64()
+        self._check(self._lib.fm_reorder_info(self._h, C.byref(iv), C.byref(n)))
+        return {"interval": iv.value, "count": n.value}
+
     # -- adjoint (no reference counterpart) -----------------------------------
     def force_jvp(self, wx, wy):
         """(jx, jy, dE, deps, dsig): j = (df/dx) w at the current positions and
@@ -507,6 +524,10 @@ class Batch:
     def set_tuning(self, tile_cells: int = 0, block: int = 0, use_graph: int = -1):
         self._check(self._lib.fm_batch_set_tuning(self._h, int(tile_cells), int(block),
                                                   int(use_graph)))
+
+    def set_reorder_interval(self, steps: int):
+        """As System.set_reorder_interval, for every replica of the batch."""
+        self._check(self._lib.fm_batch_set_reorder_interval(self._h, int(steps)))
 
     def tune(self, cache_path: Optional[str] = None):
         p = None if cache_path is None else str(cache_path).encode()

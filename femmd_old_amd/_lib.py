@@ -84,6 +84,9 @@ PROTOTYPES = {
     "fm_set_lj": (C.c_int, [_h, C.c_int32]),
     "fm_set_exclusion": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "fm_set_skin": (C.c_int, [_h, C.c_float]),
+    "fm_set_reorder_interval": (C.c_int, [_h, C.c_int32]),
+    "fm_reorder": (C.c_int, [_h]),
+    "fm_reorder_info": (C.c_int, [_h, _i, _l]),
     "fm_list_stats": (C.c_int, [_h, _d, _i, _l]),
     "fm_cluster_count": (C.c_int, [_h, _i]),
     "fm_cluster_props": (C.c_int, [_h, _d, _f, _f]),
@@ -117,6 +120,7 @@ PROTOTYPES = {
     "fm_batch_get_energies": (C.c_int, [_h, C.c_int32, C.POINTER(FmEnergies)]),
     "fm_batch_set_tuning": (C.c_int, [_h, C.c_int32, C.c_int32, C.c_int32]),
     "fm_batch_tune": (C.c_int, [_h, C.c_char_p]),
+    "fm_batch_set_reorder_interval": (C.c_int, [_h, C.c_int32]),
     "fm_batch_device_errors": (C.c_uint32, [_h]),
 }
 

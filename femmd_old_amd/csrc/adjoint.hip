@@ -165,6 +165,8 @@ int Engine::adjoint_op(const float *wx, const float *wy, float *jx, float *jy, d
                        double *deps, double *dsig, std::string &err) {
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
     if (rc) return rc;
+    rc = ensure_order(err);
+    if (rc) return rc;
     rc = adj_alloc(err);
     if (rc) return rc;
     const int N = N_;
@@ -214,6 +216,8 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         return FM_ERR_ARG;
     }
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_order(err);
     if (rc) return rc;
     rc = adj_alloc(err);
     if (rc) return rc;

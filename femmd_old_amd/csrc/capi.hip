@@ -245,6 +245,23 @@ int fm_set_skin(fm_system *h, float skin) {
     FM_END(h)
 }
 
+int fm_set_reorder_interval(fm_system *h, int32_t steps) {
+    FM_GUARD(h)
+    if (steps < 0) return fail(h, FM_ERR_ARG, "steps must be >= 0");
+    h->hs.reorder_every = steps;
+    if (h->eng) h->eng->set_reorder_interval(steps);
+    return FM_OK;
+    FM_END(h)
+}
+
+int fm_reorder_info(fm_system *h, int32_t *interval, int64_t *count) {
+    FM_GUARD(h)
+    if (interval) *interval = h->eng ? h->eng->reorder_interval() : h->hs.reorder_every;
+    if (count) *count = h->eng ? h->eng->reorders() : 0;
+    return FM_OK;
+    FM_END(h)
+}
+
 int fm_set_exclusion(fm_system *h, int32_t i, int32_t group) {
     FM_GUARD(h)
     if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
@@ -282,6 +299,13 @@ int fm_commit(fm_system *h) {
 
 #define FM_NEED_ENGINE(h) \
     if (!(h)->eng) return fail(h, FM_ERR_STATE, "call fm_commit first")
+
+int fm_reorder(fm_system *h) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->reorder(h->err);
+    FM_END(h)
+}
 
 int fm_step(fm_system *h, int32_t nsteps) {
     FM_GUARD(h)
@@ -543,6 +567,14 @@ int fm_batch_set_tuning(fm_batch *b, int32_t tile_cells, int32_t block, int32_t 
     if (block != 0 && block != 64 && block != 128 && block != 192 && block != 256)
         return failb(b, FM_ERR_ARG, "block must be 0, 64, 128, 192 or 256");
     return b->eng->set_tuning(tile_cells, block, use_graph, b->err);
+}
+
+int fm_batch_set_reorder_interval(fm_batch *b, int32_t steps) {
+    if (!b) return FM_ERR_ARG;
+    FM_NEED_BENGINE(b);
+    if (steps < 0) return failb(b, FM_ERR_ARG, "steps must be >= 0");
+    b->eng->set_reorder_interval(steps);
+    return FM_OK;
 }
 
 uint32_t fm_batch_device_errors(fm_batch *b) {

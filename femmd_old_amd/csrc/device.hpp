@@ -255,4 +255,25 @@ struct ClusterVirialArgs {
     double *out;               // per tile
 };
 
+// Re-ordering of the internal order (particles with no element, bond or
+// cluster -- "movable" -- re-assigned to the movable internal slots in the
+// order of the last counting sort).  Sorted slots are processed in chunks of
+// kReorderChunk (256 lanes x 4 consecutive slots).
+constexpr int kReorderChunk = 1024;
+constexpr int kReorderArrays = 12;
+struct ReorderArgs {
+    int n;                     // particles (all replicas)
+    int nblk;                  // chunks of kReorderChunk sorted slots
+    const uint8_t *mov;        // per internal id: movable
+    const int32_t *movids;     // the movable internal ids, ascending
+    int nmov;                  // their count
+    int32_t *perm;             // sorted slot -> internal id (rewritten)
+    int32_t *spos;             // internal id -> sorted slot (list mode; rewritten)
+    int32_t *bsum;             // per chunk: movable count, then its exclusive prefix
+    int32_t *oldof;            // new internal id -> old internal id
+    int narr;                  // per-particle 4-byte arrays to permute
+    uint32_t *arr[kReorderArrays];
+    uint32_t *tmp;             // narr * n words
+};
+
 }  // namespace fm

@@ -108,6 +108,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->model_ = reps[0]->model;
     E->block_ = reps[0]->block;
     E->use_graph_ = reps[0]->use_graph;
+    E->reorder_every_ = reps[0]->reorder_every;
 
     // neighbour-list mode: one skin for the batch, single-type pair tables
     const float skin = reps[0]->skin;
@@ -392,6 +393,24 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_UP(goff_, goff);
     FM_UP(gent_, gent);
     FM_UP(gpos_, gpos);
+    {
+        // movable particles: no element, bond or cluster refers to them, so
+        // a re-ordering may give them new internal ids
+        std::vector<uint8_t> hmov((size_t)N, 1);
+        for (const std::vector<int32_t> *v : {&ht0, &ht1, &ht2, &hbi, &hbj})
+            for (int32_t q : *v) hmov[q] = 0;
+        for (int r = 0; r < R; ++r)
+            for (size_t c = 0; c < reps[r]->cl_first.size(); ++c)
+                for (int32_t k = 0; k < reps[r]->cl_n[c]; ++k)
+                    hmov[E->iid_h_[r][reps[r]->cl_first[c] + k]] = 0;
+        std::vector<int32_t> hmovids;
+        for (int q = 0; q < N; ++q)
+            if (hmov[q]) hmovids.push_back(q);
+        E->nmov_ = (int)hmovids.size();
+        if (hmovids.empty()) hmovids.push_back(0);
+        FM_UP(mov_, hmov);
+        FM_UP(movids_, hmovids);
+    }
 #undef FM_UP
 #define FM_AL(buf, cnt) \
     if ((he = E->buf.alloc(cnt)) != hipSuccess) return E->check(he, "alloc " #buf, err)
@@ -822,6 +841,70 @@ int Engine::ensure_primed(std::string &err) {
     return FM_OK;
 }
 
+// Re-sort of the internal order (kernels in kernels.hip): the movable
+// particles take the movable internal slots in the order of the last
+// counting sort.  Every per-particle array is permuted; perm and spos are
+// rewritten, so the sorted order, the neighbour lists and the forces stay
+// valid.  Asynchronous; the host id maps are refreshed on first use.
+int Engine::launch_reorder_now() {
+    if (nmov_ <= 1) return 0;
+    hipError_t he;
+    const int nblk = (N_ + kReorderChunk - 1) / kReorderChunk;
+    ReorderArgs a{};
+    a.narr = 0;
+    for (DBuf<float> *b : {&x_, &y_, &vx_, &vy_, &fx_, &fy_, &mass_, &inv_m_, &sig_})
+        if (b->n >= (size_t)N_) a.arr[a.narr++] = reinterpret_cast<uint32_t *>(b->p);
+    for (DBuf<int32_t> *b : {&grp_, &typ_, &uid_})
+        if (b->n >= (size_t)N_) a.arr[a.narr++] = reinterpret_cast<uint32_t *>(b->p);
+    if (ro_bsum_.n < (size_t)nblk && (he = ro_bsum_.alloc(nblk)) != hipSuccess) return -1;
+    if (ro_oldof_.n < (size_t)N_) {
+        if ((he = ro_oldof_.alloc(N_)) != hipSuccess) return -1;
+        if ((he = hipMemsetAsync(ro_oldof_.p, 0, (size_t)N_ * sizeof(int32_t), stream_)) != hipSuccess)
+            return -1;
+    }
+    if (ro_tmp_.n < (size_t)a.narr * N_ && (he = ro_tmp_.alloc((size_t)a.narr * N_)) != hipSuccess)
+        return -1;
+    a.n = N_;
+    a.nblk = nblk;
+    a.mov = mov_.p;
+    a.movids = movids_.p;
+    a.nmov = nmov_;
+    a.perm = perm_.p;
+    a.spos = list_ ? spos_.p : nullptr;
+    a.bsum = ro_bsum_.p;
+    a.oldof = ro_oldof_.p;
+    a.tmp = ro_tmp_.p;
+    if (launch_reorder(a, stream_) != hipSuccess) return -1;
+    order_stale_ = true;
+    ++nreorder_;
+    return 0;
+}
+
+int Engine::reorder(std::string &err) {
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_primed(err);  // perm of the current positions
+    if (rc) return rc;
+    if (launch_reorder_now()) return check(hipGetLastError(), "reorder", err);
+    since_reorder_ = 0;
+    return sync(err);
+}
+
+int Engine::ensure_order(std::string &err) {
+    if (!order_stale_) return FM_OK;
+    int rc = check(hipMemcpyAsync(uid_h_.data(), uid_.p, (size_t)N_ * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, stream_), "uid", err);
+    if (rc) return rc;
+    rc = check(hipStreamSynchronize(stream_), "uid", err);
+    if (rc) return rc;
+    for (int r = 0; r < R_; ++r) {
+        std::vector<int32_t> &iid = iid_h_[r];
+        for (int q = reps_h_[r].p0; q < reps_h_[r].p1; ++q) iid[uid_h_[q]] = q;
+    }
+    order_stale_ = false;
+    return FM_OK;
+}
+
 int Engine::step(int nsteps, std::string &err) {
     if (nsteps < 0) { err = "nsteps must be >= 0"; return FM_ERR_ARG; }
     if (nsteps == 0) return FM_OK;
@@ -837,6 +920,15 @@ int Engine::step(int nsteps, std::string &err) {
     int bad0 = launch_step_body(false, true);
     energy_ = false;
     if (bad0) { energy_ = true; return check(hipGetLastError(), "step", err); }
+    // the internal order is re-sorted between step bodies every
+    // reorder_every_ steps (asynchronously, outside the step graph)
+    auto advance = [&](int done) -> int {
+        since_reorder_ += done;
+        if (reorder_every_ <= 0 || since_reorder_ < reorder_every_) return 0;
+        since_reorder_ = 0;
+        return launch_reorder_now();
+    };
+    if (advance(1)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
     int rem = nsteps - 1;
     if (use_graph_ && rem >= kGraphSteps + 1) {
         if (!graph_exec_) {
@@ -861,11 +953,13 @@ int Engine::step(int nsteps, std::string &err) {
             rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);
             if (rc) { energy_ = true; return rc; }
             rem -= graph_steps_;
+            if (advance(graph_steps_)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
         }
     }
     for (; rem > 0; --rem) {
         energy_ = rem == 1;
         if (launch_step_body(true, true)) { energy_ = true; return check(hipGetLastError(), "step", err); }
+        if (rem > 1 && advance(1)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
     }
     energy_ = true;
     if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
@@ -956,6 +1050,10 @@ int Engine::energies(int r, fm_energies *out, std::string &err) {
 
 int Engine::get_state(int r, float *x, float *y, float *vx, float *vy, std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    {
+        const int orc = ensure_order(err);
+        if (orc) return orc;
+    }
     const RepDesc &D = reps_h_[r];
     const int n = D.p1 - D.p0;
     std::vector<float> buf(n);
@@ -977,6 +1075,10 @@ int Engine::get_state(int r, float *x, float *y, float *vx, float *vy, std::stri
 int Engine::set_state(int r, const float *x, const float *y, const float *vx, const float *vy,
                       std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    {
+        const int orc = ensure_order(err);
+        if (orc) return orc;
+    }
     const RepDesc &D = reps_h_[r];
     const int n = D.p1 - D.p0;
     std::vector<float> buf(n);
@@ -1003,6 +1105,10 @@ int Engine::set_state(int r, const float *x, const float *y, const float *vx, co
 
 int Engine::get_forces(int r, float *fx, float *fy, std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    {
+        const int orc = ensure_order(err);
+        if (orc) return orc;
+    }
     int rc = ensure_primed(err);
     if (rc) return rc;
     // corner forces are held in gather order (no entry table)
@@ -1031,6 +1137,10 @@ int Engine::get_forces(int r, float *fx, float *fy, std::string &err) {
 int Engine::get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *start,
                       int32_t *perm, std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    {
+        const int orc = ensure_order(err);
+        if (orc) return orc;
+    }
     int rc = ensure_primed(err);
     if (rc) return rc;
     // bin and sort the current positions (list mode sorts only on rebuilds;
@@ -1067,6 +1177,10 @@ int Engine::get_cells(int r, int32_t *ncx, int32_t *ncy, int32_t *key, int32_t *
 
 int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err) {
     if (R_ != 1 || r != 0) { err = "pair_sets is available for single systems"; return FM_ERR_STATE; }
+    {
+        const int orc = ensure_order(err);
+        if (orc) return orc;
+    }
     int rc = ensure_primed(err);
     if (rc) return rc;
     hipError_t he;

@@ -47,6 +47,8 @@ struct HostSystem {
     float skin = 0.6f;
     // launch tuning
     int32_t tile_w = 0, block = 256, use_graph = 1;
+    // steps between re-orderings of the internal order (0: never)
+    int32_t reorder_every = 1000;
 };
 
 template <class T>
@@ -103,6 +105,12 @@ public:
                     std::string &err);
     int ntri() const { return ntri_; }
     int set_tuning(int tile_w, int block, int use_graph, std::string &err);
+    // internal order: re-sort the movable particles to the last counting
+    // sort now, or every `every` steps of step() (0: never)
+    int reorder(std::string &err);
+    void set_reorder_interval(int every) { reorder_every_ = every; }
+    int reorder_interval() const { return reorder_every_; }
+    int64_t reorders() const { return nreorder_; }
     uint32_t errors();
     int clear_errors();
     int sync(std::string &err);
@@ -154,6 +162,10 @@ private:
     int build_clusters(const std::vector<const HostSystem *> &reps, std::string &err);
     int run_cluster_props(float *ux, float *uy);
     int ensure_primed(std::string &err);
+    int launch_reorder_now();
+    // the host maps uid_h_ / iid_h_ follow the device's uid after a
+    // re-ordering (refreshed on first use)
+    int ensure_order(std::string &err);
     int check(hipError_t e, const char *what, std::string &err);
     int drop_graph();
 
@@ -173,6 +185,10 @@ private:
     int energy_pass(std::string &err);
     std::vector<char> lj_on_;
     int64_t steps_ = 0;
+    int reorder_every_ = 1000;
+    int64_t since_reorder_ = 0, nreorder_ = 0;
+    bool order_stale_ = false;
+    int nmov_ = 0;
     int model_ = 0;
     double rho_cp_ = 1.16;
 
@@ -189,6 +205,9 @@ private:
     DBuf<PairEntry> ptab_;
     DBuf<float> x_, y_, vx_, vy_, fx_, fy_, mass_, inv_m_, sig_;
     DBuf<int32_t> grp_, typ_, uid_;
+    DBuf<uint8_t> mov_;                // per internal id: movable
+    DBuf<int32_t> movids_, ro_bsum_, ro_oldof_;
+    DBuf<uint32_t> ro_tmp_;
     DBuf<int32_t> count_, tmp_, start_, perm_, sorted_typ_;
     DBuf<unsigned long long> sort_status_, sort_ticket_;
     int nsort_ = 0;
@@ -267,5 +286,6 @@ hipError_t launch_cluster_virial(bool sigma, bool types, int block, const Cluste
 hipError_t launch_vel_ke(const int32_t *ids, int n, const float *vx, const float *vy,
                          const float *mass, double *part, hipStream_t s);
 hipError_t launch_vel_scale(const int32_t *ids, int n, float *vx, float *vy, float f, hipStream_t s);
+hipError_t launch_reorder(const ReorderArgs &a, hipStream_t s);
 
 }  // namespace fm

@@ -2656,4 +2656,121 @@ hipError_t launch_vel_scale(const int32_t *ids, int n, float *vx, float *vy, flo
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Re-ordering of the internal order.  The internal order is the cell order
+// of the particles when the system was committed; as particles diffuse it
+// decorrelates from the cells, and the kick's mirror into sorted order, the
+// sort's gathers and the pair pass's force writes turn into scattered
+// accesses.  A re-ordering gives the movable particles (no element, bond or
+// cluster refers to them) new internal ids in the order of the last
+// counting sort, on the same set of internal slots, so no topology changes:
+// the k-th movable particle in sorted order takes the k-th movable slot.
+// perm and spos are rewritten for the new ids (the sorted order itself, and
+// so the neighbour lists, stay valid); the per-particle arrays are gathered
+// through oldof.
+//   k_ro_count   movable count per chunk of sorted slots
+//   k_ro_scan    exclusive prefix of the chunk counts (one workgroup)
+//   k_ro_map     rank of each movable particle -> new id; oldof, perm, spos
+//   k_ro_gather  tmp[k][q] = arr[k][oldof[q]]
+//   k_ro_copy    arr[k][q] = tmp[k][q]
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ro_flags(const ReorderArgs &a, int b, int4 &ids) {
+    const int s0 = b * kReorderChunk + 4 * (int)threadIdx.x;
+    int v[4] = {-1, -1, -1, -1};
+    if (s0 + 3 < a.n) {
+        const int4 q = *reinterpret_cast<const int4 *>(a.perm + s0);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        for (int u = 0; u < 4; ++u)
+            if (s0 + u < a.n) v[u] = a.perm[s0 + u];
+    }
+    ids = make_int4(v[0], v[1], v[2], v[3]);
+    int m = 0;
+    for (int u = 0; u < 4; ++u) m |= (v[u] >= 0 && a.mov[v[u]]) ? (1 << u) : 0;
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_ro_count(ReorderArgs a) {
+    __shared__ int red[4];
+    int4 ids;
+    const int m = ro_flags(a, blockIdx.x, ids);
+    const int c = wave_sum_int(__popc(m));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) a.bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(1024) void k_ro_scan(ReorderArgs a) {
+    __shared__ int w[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < a.nblk; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const int v = i < a.nblk ? a.bsum[i] : 0;
+        const int incl = wave_incl_scan(v);
+        if (lane == 63) w[wv] = incl;
+        __syncthreads();
+        int wb = carry;
+        for (int k = 0; k < wv; ++k) wb += w[k];
+        if (i < a.nblk) a.bsum[i] = wb + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = wb + incl;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ro_map(ReorderArgs a) {
+    __shared__ int w[4];
+    int4 ids;
+    const int m = ro_flags(a, blockIdx.x, ids);
+    const int c = __popc(m);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int incl = wave_incl_scan(c);
+    if (lane == 63) w[wv] = incl;
+    __syncthreads();
+    int rank = a.bsum[blockIdx.x];
+    for (int k = 0; k < wv; ++k) rank += w[k];
+    rank += incl - c;
+    const int s0 = blockIdx.x * kReorderChunk + 4 * (int)threadIdx.x;
+    const int v[4] = {ids.x, ids.y, ids.z, ids.w};
+    for (int u = 0; u < 4; ++u) {
+        const int id = v[u];
+        if (id < 0) continue;
+        // (a sort that dropped particles, reported as a cell overflow,
+        // can repeat an id: the rank stays inside the movable slots)
+        const bool mv = ((m >> u) & 1) && rank < a.nmov;
+        const int nid = mv ? a.movids[rank++] : id;
+        a.oldof[nid] = id;
+        a.perm[s0 + u] = nid;
+        if (a.spos) a.spos[nid] = s0 + u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ro_gather(ReorderArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    int o = a.oldof[q];
+    if ((unsigned)o >= (unsigned)a.n) o = q;  // only after a dropped particle
+    for (int k = 0; k < a.narr; ++k) a.tmp[(size_t)k * a.n + q] = a.arr[k][o];
+}
+
+__global__ __launch_bounds__(256) void k_ro_copy(ReorderArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    for (int k = 0; k < a.narr; ++k) a.arr[k][q] = a.tmp[(size_t)k * a.n + q];
+}
+
+hipError_t launch_reorder(const ReorderArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    const int g = (a.n + 255) / 256;
+    hipLaunchKernelGGL(k_ro_count, dim3(a.nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ro_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_ro_map, dim3(a.nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ro_gather, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ro_copy, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace fm

@@ -79,6 +79,8 @@ int Engine::cluster_props(int r, double *out, float *ux, float *uy, std::string 
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
     if (rc) return rc;
+    rc = ensure_order(err);
+    if (rc) return rc;
     const bool unf = ux || uy;
     if (unf) {
         // unfolded positions of the cluster nodes, the wrapped positions of
@@ -168,6 +170,8 @@ int Engine::set_temperature(int r, double temp, int n, const int32_t *ids, std::
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     if (!(temp >= 0.0) || !std::isfinite(temp)) { err = "temperature must be >= 0"; return FM_ERR_ARG; }
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_order(err);
     if (rc) return rc;
     const RepDesc &D = reps_h_[r];
     const int np = D.p1 - D.p0;

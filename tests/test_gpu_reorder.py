@@ -1,0 +1,94 @@
+"""Re-ordering of the internal order (no reference counterpart).
+
+A re-sort gives the particles that no element, bond or cluster refers to
+new internal ids in the current cell order.  It is a pure relabelling: the
+state, the forces, the cluster properties, the interacting sets and the
+cell keys read in caller order are bitwise the same just after it; J w
+(recomputed from lists built after it) agrees to FP32 summation order.
+After further steps the run stays within the parity bars, and a batch and
+the same systems stepped alone still agree bitwise when both re-sort on
+the same schedule.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from femmd_old_amd import Batch, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _shuffled(w, skin=None, every=0):
+    """The workload with its solvent's positions and velocities shuffled
+    among the solvent slots (the internal order no longer follows the
+    cells, as after a long run)."""
+    s = configs.build_system(w, commit=False, skin=skin)
+    s.set_reorder_interval(every)
+    s.allocate_cuda()
+    x, y, vx, vy = s.get_state()
+    ns = w.n_solvent
+    p = np.random.default_rng(3).permutation(ns)
+    for a in (x, y, vx, vy):
+        a[:ns] = a[:ns][p]
+    s.set_state(x, y, vx, vy)
+    return s
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_reorder_is_a_pure_relabelling(skin):
+    w = configs.c2(scale=0.02)
+    s = _shuffled(w, skin)
+    s.run(3)
+    # state, forces and cluster properties: bitwise the same in caller order
+    before = (*s.get_state(), *s.get_forces(), s.cluster_props()["cm"])
+    s.reorder()
+    after = (*s.get_state(), *s.get_forces(), s.cluster_props()["cm"])
+    for u, v in zip(before, after):
+        np.testing.assert_array_equal(u, v)
+    # interacting sets and cell keys (order independent)
+    before = (*s.pair_sets(), s.cells()[2])
+    s.reorder()
+    after = (*s.pair_sets(), s.cells()[2])
+    for u, v in zip(before, after):
+        np.testing.assert_array_equal(u, v)
+    # J w: the same up to FP32 summation order within a cell
+    rng = np.random.default_rng(4)
+    wx = rng.standard_normal(s.N).astype(np.float32)
+    wy = rng.standard_normal(s.N).astype(np.float32)
+    j0 = s.force_jvp(wx, wy)[:2]
+    s.reorder()
+    j1 = s.force_jvp(wx, wy)[:2]
+    for u, v in zip(j0, j1):
+        np.testing.assert_allclose(v, u, rtol=1e-4, atol=1e-4 * float(np.abs(u).max()))
+    assert s.reorder_info()["count"] == 3
+    # further steps: the forces still meet the parity bar
+    for k in range(2):
+        s.run(30)
+        x, y, _, _ = s.get_state()
+        fx, fy = s.get_forces()
+        rx, ry, en, fs = O.forces(O.from_workload(w, x, y), pred_f32=True, scale=True)
+        err = O.force_error(fx, fy, rx, ry, fs)
+        print(f"[reorder skin {skin}] max rel force err {err.max():.3e}")
+        assert err.max() <= 1e-5
+    assert s.device_errors() == 0
+
+
+def test_reorder_schedule_batched_equals_looped():
+    ws = configs.c4(n_replicas=5, n_max=4000)
+    singles = []
+    for w in ws:
+        s = configs.build_system(w, commit=False)
+        s.set_reorder_interval(7)
+        s.allocate_cuda()
+        s.run(60)
+        assert s.reorder_info()["count"] >= 3
+        singles.append(s.get_state())
+    systems = [configs.build_system(w, commit=False) for w in ws]
+    for s in systems:
+        s.set_reorder_interval(7)
+    b = Batch(systems)
+    b.run(60)
+    for r, st in enumerate(singles):
+        for a, c in zip(st, b.get_state(r)):
+            np.testing.assert_array_equal(a, c)
+    assert b.device_errors() == 0
