@@ -98,4 +98,8 @@ def test_set_mass_and_tuning_validation():
     with pytest.raises(FemmdError):
         s.set_tuning(block=96)
     s.set_tuning(tile_cells=16, block=128, use_graph=0)
+    with pytest.raises(FemmdError):
+        s.set_reorder_interval(-1)
+    s.set_reorder_interval(0)
+    assert s.reorder_info() == {"interval": 0, "count": 0}
     s.close()
