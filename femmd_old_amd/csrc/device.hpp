@@ -101,7 +101,39 @@ struct KickArgs {
                                // y to rec[n + p] (trajectory recording)
     int32_t n;                 // particles (stride of rec)
     int32_t nspans;            // spans (grid-stride binning)
+    const int32_t *fspan;      // gated batches: the rebuilding replicas' spans
+    const int32_t *nfs;        // (decided by k_decide), and their count
 };
+
+// Rebuild decision of a batch (one workgroup): per replica the flag, and
+// the compact lists of the rebuilding replicas' kick spans and tiles that
+// the gated binning and list build walk instead of every span and tile.
+// It also lists the sort blocks that hold a cell of a rebuilding replica (in
+// increasing order), restarts the sort's per-step ticket counter and
+// advances its step epoch.
+struct DecideArgs {
+    const RepDesc *reps;
+    Ctl *ctl;
+    const uint32_t *dmax;
+    const int32_t *rng;        // 8 per replica: spans [0,1), tiles [2,3), ...
+    int32_t nrep;
+    int32_t *any;
+    int32_t *fspan, *ftile;    // out: span / tile indices
+    int32_t *fblock;           // out: sort blocks
+    int32_t nsort;
+    int32_t *nf;               // out: [0] spans, [1] tiles, [2] sort blocks,
+                               // [3] sort ticket counter (reset), [4] step epoch
+    int4 *rinfo;               // per replica (k_flags): spans, tiles to list
+                               // (0 if it does not rebuild), first span, first tile
+    uint8_t *mark;             // per sort block: holds a rebuilding replica's
+                               // cell (set by k_flags, cleared by k_decide)
+};
+
+// Batches of more than this many replicas take the rebuild decision in one
+// small kernel before the gated binning, which with the list build then
+// walks compact lists of the rebuilding replicas' spans and tiles; smaller
+// systems let every binning workgroup take it for the replicas it meets.
+constexpr int kDecideMinReps = 8;
 
 // Single-pass counting sort: workgroup b scans cells [b * kSortCells, ...),
 // takes its carry-in from its predecessors' status words (decoupled
@@ -112,10 +144,14 @@ struct ScanArgs {
     int32_t ncells;
     int32_t cap;
     int32_t nsort;             // sort workgroups per launch
+    const int2 *brep;          // per sort block: first and last replica of its cells
     int32_t *count;            // per-cell counters, zeroed once read
     const int32_t *tmp;        // cell * cap + slot -> particle
     unsigned long long *status;  // nsort status words (value, state, launch)
     unsigned long long *ticket;  // workgroup tickets, never reset
+    const int32_t *fblock;     // listed (gated batches): the blocks to sort,
+    int32_t *nf;               // their count nf[2], tickets nf[3], epoch nf[4]
+    unsigned long long *status2; // status words of the listed sort
     const Ctl *ctl;            // list mode gating (null: always run)
     const int32_t *any;
     int32_t *start;            // per cell: first sorted position (replica based)
@@ -179,6 +215,8 @@ struct ForceArgs {
     const float2 *xys;         // list mode: positions in sorted order
     const float *sigs;         // list mode: sigma in sorted order (sigma path)
     const Ctl *ctl;            // list build gating (null: always build)
+    const int32_t *ftile;      // gated batches: the rebuilding replicas' tiles
+    const int32_t *nft;        // and their count
     int32_t *any;              // reset by the force pass after the rebuild
     uint32_t *dmax;            // displacement slots, cleared by the force pass
     Ctl *ctl_w;                // displacement bounds, updated by the force pass

@@ -422,6 +422,35 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
     E->nsort_ = (E->ncells_ + kSortCells - 1) / kSortCells;
     FM_AL(sort_status_, std::max(E->nsort_, 1));
+    {
+        // per sort block: the replicas its cells belong to (replica r owns
+        // cells [cell0, cell0 + ncx * ncy + 1))
+        std::vector<int2> hbrep(std::max(E->nsort_, 1));
+        int r = 0;
+        for (int b = 0; b < E->nsort_; ++b) {
+            const int c0 = b * kSortCells, c1 = std::min(c0 + kSortCells, E->ncells_) - 1;
+            while (r + 1 < R && rd[r + 1].cell0 <= c0) ++r;
+            int r1 = r;
+            while (r1 + 1 < R && rd[r1 + 1].cell0 <= c1) ++r1;
+            hbrep[b] = make_int2(r, r1);
+        }
+        if ((he = upload(E->brep_, hbrep, st)) != hipSuccess) return E->check(he, "upload brep", err);
+    }
+    if (E->list_ && R > kDecideMinReps) {
+        FM_AL(fspan_, std::max<size_t>(ks.size(), 1));
+        FM_AL(rinfo_, R);
+        FM_AL(mark_, std::max(E->nsort_, 1));
+        if ((he = hipMemsetAsync(E->mark_.p, 0, E->mark_.n, st)) != hipSuccess)
+            return E->check(he, "memset", err);
+        FM_AL(nf_, 5);
+        if ((he = hipMemsetAsync(E->nf_.p, 0, 5 * sizeof(int32_t), st)) != hipSuccess)
+            return E->check(he, "memset", err);
+        FM_AL(fblock_, std::max(E->nsort_, 1));
+        FM_AL(sort_status2_, std::max(E->nsort_, 1));
+        if ((he = hipMemsetAsync(E->sort_status2_.p, 0, E->sort_status2_.n * sizeof(unsigned long long),
+                                 st)) != hipSuccess)
+            return E->check(he, "memset", err);
+    }
     FM_AL(sort_ticket_, 1);
     FM_AL(start_, E->ncells_ + 1);
     FM_AL(perm_, N);
@@ -543,6 +572,9 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
     if (list_) lb_grid_ = list_build_resident(sigma_, block_, force_args(), device_);
     if ((he = upload(tiles_, tiles_h_, stream_)) != hipSuccess) return check(he, "tiles", err);
     if ((he = upload(rng_, rng_h_, stream_)) != hipSuccess) return check(he, "ranges", err);
+    if (list_ && R_ > kDecideMinReps &&
+        (he = ftile_.alloc(std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
+        return check(he, "tile list", err);
     if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
         return check(he, "lj_part", err);
     if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess)
@@ -608,6 +640,25 @@ int Engine::launch_bin(bool gated) {
         a.ctl = ctl_.p;
         a.any = any_.p;
         a.dmax = dmax_.p;
+        if (R_ > kDecideMinReps) {
+            DecideArgs d{};
+            d.reps = reps_.p;
+            d.ctl = ctl_.p;
+            d.dmax = dmax_.p;
+            d.rng = rng_.p;
+            d.nrep = R_;
+            d.any = any_.p;
+            d.fspan = fspan_.p;
+            d.ftile = ftile_.p;
+            d.fblock = fblock_.p;
+            d.nsort = nsort_;
+            d.nf = nf_.p;
+            d.rinfo = rinfo_.p;
+            d.mark = mark_.p;
+            if (launch_decide(d, stream_) != hipSuccess) return -1;
+            a.fspan = fspan_.p;
+            a.nfs = nf_.p;
+        }
     }
     return launch_kick(0, false, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
@@ -664,6 +715,7 @@ int Engine::launch_sort(bool gated) {
     s.ncells = ncells_;
     s.cap = cap_;
     s.nsort = nsort_;
+    s.brep = brep_.p;
     s.count = count_.p;
     s.tmp = tmp_.p;
     s.status = sort_status_.p;
@@ -686,6 +738,11 @@ int Engine::launch_sort(bool gated) {
     if (gated) {
         s.ctl = ctl_.p;
         s.any = any_.p;
+        if (R_ > kDecideMinReps) {  // the blocks listed by k_decide
+            s.fblock = fblock_.p;
+            s.nf = nf_.p;
+            s.status2 = sort_status2_.p;
+        }
     }
     if (::fm::launch_sort(s, stream_) != hipSuccess) return -1;
     if (prof_ev_) (void)hipEventRecord(prof_ev_[3], stream_);
@@ -696,6 +753,10 @@ int Engine::launch_sort(bool gated) {
 int Engine::launch_list_build(bool gated) {
     ForceArgs f = force_args();
     if (gated) f.ctl = ctl_.p;
+    if (gated && R_ > kDecideMinReps) {  // the tiles listed by k_decide
+        f.ftile = ftile_.p;
+        f.nft = nf_.p + 1;
+    }
     const hipError_t e = ::fm::launch_list_build(sigma_, block_, f, stream_);
     if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);
     return e == hipSuccess ? 0 : -1;

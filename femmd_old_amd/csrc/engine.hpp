@@ -207,6 +207,11 @@ private:
     DBuf<int32_t> grp_, typ_, uid_;
     DBuf<uint8_t> mov_;                // per internal id: movable
     DBuf<int32_t> movids_, ro_bsum_, ro_oldof_;
+    DBuf<int32_t> fspan_, ftile_, fblock_, nf_;  // batches: compact rebuild work lists
+    DBuf<int4> rinfo_;
+    DBuf<uint8_t> mark_;
+    DBuf<unsigned long long> sort_status2_;     // status words of the listed sort
+    DBuf<int2> brep_;                   // per sort block: replica range
     DBuf<uint32_t> ro_tmp_;
     DBuf<int32_t> count_, tmp_, start_, perm_, sorted_typ_;
     DBuf<unsigned long long> sort_status_, sort_ticket_;
@@ -287,5 +292,6 @@ hipError_t launch_vel_ke(const int32_t *ids, int n, const float *vx, const float
                          const float *mass, double *part, hipStream_t s);
 hipError_t launch_vel_scale(const int32_t *ids, int n, float *vx, float *vy, float f, hipStream_t s);
 hipError_t launch_reorder(const ReorderArgs &a, hipStream_t s);
+hipError_t launch_decide(const DecideArgs &d, hipStream_t s);
 
 }  // namespace fm

@@ -310,12 +310,120 @@ __device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R) 
     return flag;
 }
 
+// The rebuild decision of a batch, in one workgroup: each lane takes a
+// replica (its bound as list_disp forms it), then the rebuilding replicas'
+// kick spans and tiles are written as two compact lists (inclusive scans
+// over the replicas, in replica order; each entry found by a binary search
+// of those scans in LDS), which the gated binning and list build walk
+// instead of every span and tile.  The sort blocks holding a cell of a
+// rebuilding replica are marked in LDS and listed in increasing order.
+__device__ __forceinline__ int block_scan_incl(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int is = wave_incl_scan(v);
+    if (lane == 63) wsum[wv] = is;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < wv; ++k) base += wsum[k];
+    total = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) total += wsum[k];
+    __syncthreads();
+    return base + is;
+}
+
+// One wavefront per replica: the bound as list_disp forms it, the flag, and
+// what the replica adds to the work lists.
+__global__ __launch_bounds__(256) void k_flags(DecideArgs d) {
+    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (r >= d.nrep) return;  // uniform per wavefront
+    const RepDesc &R = d.reps[r];
+    const float bound = list_disp(d.dmax, d.ctl, r, R);
+    if ((threadIdx.x & 63) == 0) {
+        const int flag = bound > 0.5f * R.skin;
+        d.ctl[r].flag = flag;
+        int4 info = make_int4(0, 0, d.rng[8 * r], d.rng[8 * r + 2]);
+        if (flag) {
+            d.ctl[r].nbuild += 1u;
+            *d.any = 1;  // every writer stores the same value
+            info.x = d.rng[8 * r + 1] - d.rng[8 * r];
+            info.y = d.rng[8 * r + 3] - d.rng[8 * r + 2];
+            if (d.fblock) {
+                // the replica's cells, with its closing empty cell
+                const int b0 = R.cell0 / kSortCells;
+                const int b1 = (R.cell0 + R.ncx * R.ncy) / kSortCells;
+                for (int b = b0; b <= b1; ++b) d.mark[b] = 1;
+            }
+        }
+        d.rinfo[r] = info;
+    }
+}
+
+// The work lists of a batch, in one workgroup: the rebuilding replicas' kick
+// spans and tiles (inclusive scans over the replicas, in replica order; each
+// entry found by a binary search of those scans in LDS) and the marked sort
+// blocks in increasing order (the marks are cleared for the next step).
+__global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
+    __shared__ int inS[1024], inT[1024], fS[1024], fT[1024], wsum[16];
+    const int t = threadIdx.x;
+    int baseS = 0, baseT = 0;
+    for (int c0 = 0; c0 < d.nrep; c0 += 1024) {
+        const int r = c0 + t;
+        const int4 info = r < d.nrep ? d.rinfo[r] : make_int4(0, 0, 0, 0);
+        int totS, totT;
+        inS[t] = block_scan_incl(info.x, wsum, totS);
+        inT[t] = block_scan_incl(info.y, wsum, totT);
+        fS[t] = info.z;
+        fT[t] = info.w;
+        __syncthreads();
+        // entry v of this chunk's list belongs to the first replica whose
+        // inclusive count exceeds v
+        auto fill = [&](const int *in, const int *first, int tot, int base, int32_t *out) {
+            for (int v = t; v < tot; v += 1024) {
+                int lo = 0, hi = 1023;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (in[mid] > v) hi = mid; else lo = mid + 1;
+                }
+                const int ex = lo > 0 ? in[lo - 1] : 0;
+                out[base + v] = first[lo] + (v - ex);
+            }
+        };
+        fill(inS, fS, totS, baseS, d.fspan);
+        fill(inT, fT, totT, baseT, d.ftile);
+        baseS += totS;
+        baseT += totT;
+        __syncthreads();
+    }
+    int nfb = 0;
+    if (d.fblock) {
+        for (int c0 = 0; c0 < d.nsort; c0 += 1024) {
+            const int b = c0 + t;
+            const int m = b < d.nsort ? d.mark[b] : 0;
+            int tot;
+            const int ib = block_scan_incl(m, wsum, tot);
+            if (m) {
+                d.fblock[nfb + ib - 1] = b;
+                d.mark[b] = 0;
+            }
+            nfb += tot;
+        }
+    }
+    if (t == 0) {
+        d.nf[0] = baseS;
+        d.nf[1] = baseT;
+        d.nf[2] = nfb;
+        d.nf[3] = 0;                  // the listed sort's tickets
+        d.nf[4] = d.nf[4] + 1;        // and its step epoch
+    }
+}
+
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
   int last_rep = -1, flag = 1;
-  for (int b = blockIdx.x; b < a.nspans; b += gridDim.x) {
-    const Span sp = a.spans[b];
+  // decided (batches): only the rebuilding replicas' spans, listed
+  const int nsp = a.fspan ? *a.nfs : a.nspans;
+  for (int b = blockIdx.x; b < nsp; b += gridDim.x) {
+    const Span sp = a.spans[a.fspan ? a.fspan[b] : b];
     const RepDesc R = a.reps[sp.rep];
-    if (a.ctl) {  // list mode: bin only the replicas that rebuild this step
+    if (a.ctl && !a.fspan) {  // list mode: bin only the replicas that rebuild this step
         if (sp.rep != last_rep) {  // uniform per workgroup
             flag = list_decide(a, sp, R);
             last_rep = sp.rep;
@@ -369,8 +477,11 @@ __device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, unsigned l
     return ((unsigned long long)(epoch & 0x3fffffffu) << 34) | (state << 32) | (uint32_t)v;
 }
 
-__device__ __forceinline__ int rep_of_cell(const ScanArgs &a, int c) {
-    int lo = 0, hi = a.nrep;
+// Replica of cell c of sort block b: a binary search over the block's
+// replicas (brep, usually one or two).
+__device__ __forceinline__ int rep_of_cell(const ScanArgs &a, int b, int c) {
+    const int2 br = a.brep[b];
+    int lo = br.x, hi = br.y + 1;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (a.reps[mid].cell0 <= c) lo = mid; else hi = mid;
@@ -388,18 +499,40 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ uint8_t s_cl[kSortIds];    // their cell (local index)
     if (a.any && !*a.any) return;  // list mode, no replica rebuilds (uniform)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) s_tk = atomicAdd(a.ticket, 1ull);
+    // listed (gated batches): workgroups beyond the listed blocks leave at
+    // once, so only the listed blocks take tickets; they take the blocks in
+    // ticket order, with the step's epoch and a status array of their own
+    const bool listed = a.fblock != nullptr;
+    if (listed && (int)blockIdx.x >= a.nf[2]) return;
+    if (t == 0)
+        s_tk = listed ? (unsigned long long)atomicAdd(a.nf + 3, 1) : atomicAdd(a.ticket, 1ull);
     __syncthreads();
     const unsigned long long tk = s_tk;
-    const int b = (int)(tk % (unsigned long long)a.nsort);
-    const uint32_t epoch = (uint32_t)(tk / (unsigned long long)a.nsort);
+    const int b = listed ? a.fblock[tk] : (int)(tk % (unsigned long long)a.nsort);
+    const uint32_t epoch =
+        listed ? (uint32_t)a.nf[4] : (uint32_t)(tk / (unsigned long long)a.nsort);
+    unsigned long long *const status = listed ? a.status2 : a.status;
     const int c0 = b * kSortCells, c = c0 + t;
+    if (a.ctl && !listed) {
+        // gated (list mode): a block holding no cell of a rebuilding replica
+        // has nothing to count, scan or scatter, and no rebuilding replica
+        // looks back past its own first block, so it is skipped whole
+        __syncthreads();
+        if (t == 0) {
+            const int2 br = a.brep[b];
+            int f = 0;
+            for (int q = br.x; q <= br.y; ++q) f |= a.ctl[q].flag;
+            s_head = f;
+        }
+        __syncthreads();
+        if (!s_head) return;
+    }
     int n = 0;
     if (c < a.ncells) {
         n = min(a.count[c], a.cap);
         a.count[c] = 0;  // ready for the next binning
     }
-    const int r = rep_of_cell(a, min(c, a.ncells - 1));
+    const int r = rep_of_cell(a, b, min(c, a.ncells - 1));
     const RepDesc &R = a.reps[r];
     const int hl = R.cell0 - c0;  // replica's first cell, local (< 0: earlier block)
     // block exclusive scan
@@ -420,11 +553,12 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
         const unsigned long long v = (hl >= 0)
             ? lb_word(epoch, kLbInclusive, R.p0 + tot - s_ex[hl])
             : lb_word(epoch, kLbAggregate, tot);
-        __hip_atomic_store(a.status + b, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(status + b, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // look-back, needed when the block's first cell continues a replica
     // begun in an earlier block (uniform: read from the first cell)
-    if (t == 0) s_carry = (hl >= 0) ? -1 : 0;
+    // (not for a replica that does not rebuild: its starts are not written)
+    if (t == 0) s_carry = (hl >= 0 || (a.ctl && !a.ctl[r].flag)) ? -1 : 0;
     __syncthreads();
     const bool need = s_carry == 0;
     if (need && w == 0) {
@@ -434,7 +568,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
         while (true) {
             const int q = j - lane;
             unsigned long long v = 0;
-            if (q >= 0) v = __hip_atomic_load(a.status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (q >= 0) v = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const bool cur = (uint32_t)(v >> 34) == (epoch & 0x3fffffffu);
             const unsigned long long st = (v >> 32) & 3ull;
             // q < 0 cannot be reached before an inclusive word (block 0
@@ -458,7 +592,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
         if (lane == 0) {
             s_carry = carry;
             if (!s_head)  // a block holding a replica's first cell published already
-                __hip_atomic_store(a.status + b, lb_word(epoch, kLbInclusive, carry + tot),
+                __hip_atomic_store(status + b, lb_word(epoch, kLbInclusive, carry + tot),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -1153,9 +1287,11 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     // list buffer after the staging and its read headroom of 3 cells
     uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
     if (a.ctl && a.any && !*a.any) return;  // gated, and no replica rebuilds
-    for (int b = blockIdx.x; b < a.ntiles; b += gridDim.x) {
-        const Tile T = a.tiles[xcd_remap(b, a.ntiles)];
-        if (a.ctl && !a.ctl[T.rep].flag) continue;  // uniform per workgroup
+    // gated batches walk the listed tiles of the rebuilding replicas
+    const int nt = a.ftile ? *a.nft : a.ntiles;
+    for (int b = blockIdx.x; b < nt; b += gridDim.x) {
+        const Tile T = a.tiles[a.ftile ? a.ftile[xcd_remap(b, nt)] : xcd_remap(b, nt)];
+        if (a.ctl && !a.ftile && !a.ctl[T.rep].flag) continue;  // uniform per workgroup
         const RepDesc R = a.reps[T.rep];
         const int W = T.w_flags & 0xffff;
         if (stage_tile<false, false>(a, T, R, W, L, nullptr, nullptr) < 0) {
@@ -2760,6 +2896,12 @@ __global__ __launch_bounds__(256) void k_ro_copy(ReorderArgs a) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= a.n) return;
     for (int k = 0; k < a.narr; ++k) a.arr[k][q] = a.tmp[(size_t)k * a.n + q];
+}
+
+hipError_t launch_decide(const DecideArgs &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_flags, dim3((d.nrep + 3) / 4), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1024), 0, s, d);
+    return hipGetLastError();
 }
 
 hipError_t launch_reorder(const ReorderArgs &a, hipStream_t s) {

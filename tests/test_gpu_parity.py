@@ -168,9 +168,12 @@ def test_run_twice_bitwise_and_graph_invariance(skin):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("nrep", [6, 20])
 @pytest.mark.parametrize("skin", SKINS)
-def test_batched_equals_looped_bitwise(skin):
-    ws = configs.c4(n_replicas=6, n_max=3000)
+def test_batched_equals_looped_bitwise(skin, nrep):
+    # 20 replicas: the rebuild decision is taken by its own kernel and the
+    # sort skips the blocks of replicas that do not rebuild
+    ws = configs.c4(n_replicas=nrep, n_max=3000)
     singles = []
     for w in ws:
         s = configs.build_system(w, skin=skin)
