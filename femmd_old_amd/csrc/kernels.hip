@@ -1240,7 +1240,8 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
                 }
                 bool hit = (t < len) & (dr2 < rl2) & (__float_as_int(pj.z) != gix);
                 if (r == 1) hit &= (j != me);
-                buf[min(cnt, capm1) * bd + tid] = (uint16_t)(8 * j);
+                // (24-bit multiply: full rate; cnt <= cap <= 512, bd <= 256)
+                buf[__umul24(min(cnt, capm1), bd) + tid] = (uint16_t)(8 * j);
                 cnt += hit ? 1 : 0;
             }
         }
@@ -1259,7 +1260,7 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int kk = 4 * g + t;
-                    jj[t] = kk < m ? (int)buf[kk * bd + tid] : 8 * a.lds_cap;
+                    jj[t] = kk < m ? (int)buf[__umul24(kk, bd) + tid] : 8 * a.lds_cap;
                 }
                 list_store_group(a, g, p, jj);
             }
