@@ -98,7 +98,7 @@ def measured_traffic(kernel: str, mode: str):
     return None
 
 
-def batched_leg(steps: int, loop_steps: int, tune_cache):
+def batched_leg(steps: int, loop_steps: int, tune_cache, looped_leg: bool = True):
     """C4: 512 replicas, N ~ logU[256, 32768]; one batched, graph-captured
     step for all of them against looping the plain step over the replicas
     (each replica stepping loop_steps with its own captured graph)."""
@@ -116,14 +116,16 @@ def batched_leg(steps: int, loop_steps: int, tune_cache):
     b.run(steps)
     tb1 = time.perf_counter()
     batched = ntot * steps / (tb1 - tb0)
-    for s in systems:  # warm-up also captures each replica's step graph
-        s.allocate_cuda()
-        s.run(loop_steps)
-    tl0 = time.perf_counter()
-    for s in systems:
-        s.run(loop_steps)
-    tl1 = time.perf_counter()
-    looped = ntot * loop_steps / (tl1 - tl0)
+    looped, tl0, tl1 = None, 0.0, 0.0
+    if looped_leg:
+        for s in systems:  # warm-up also captures each replica's step graph
+            s.allocate_cuda()
+            s.run(loop_steps)
+        tl0 = time.perf_counter()
+        for s in systems:
+            s.run(loop_steps)
+        tl1 = time.perf_counter()
+        looped = ntot * loop_steps / (tl1 - tl0)
     errs = b.device_errors()
     print(f"[bench] c4: {len(ws)} replicas, {ntot} particles, setup {t_tune and time.time() - t0:.1f}s, "
           f"tune {t_tune:.1f}s", file=sys.stderr, flush=True)
@@ -131,8 +133,8 @@ def batched_leg(steps: int, loop_steps: int, tune_cache):
             "batched_particle_steps_per_s": batched,
             "batched_ms_per_step": 1e3 * (tb1 - tb0) / steps,
             "looped_particle_steps_per_s": looped,
-            "looped_ms_per_step": 1e3 * (tl1 - tl0) / loop_steps,
-            "speedup": batched / looped, "device_errors": errs}
+            "looped_ms_per_step": 1e3 * (tl1 - tl0) / loop_steps if looped else None,
+            "speedup": batched / looped if looped else None, "device_errors": errs}
 
 
 def adjoint_leg(steps: int):
@@ -222,6 +224,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-looped", action="store_true",
+                    help="skip the looped single-replica comparison of the batched leg")
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--skin", type=float, default=0.6,
                     help="neighbour-list skin (0: walk the cells every step)")
@@ -311,7 +315,7 @@ def main():
     if not args.no_batched:
         del s
         out["batched"] = batched_leg(args.batched_steps, max(4, args.batched_steps // 10),
-                                     args.tune_cache)
+                                     args.tune_cache, not args.no_looped)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu:
