@@ -465,7 +465,12 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
 // relaxed atomic (value, state, launch epoch), so no fences are needed;
 // the cell starts and the scattered particles are read by later kernels
 // only.  A workgroup waits only on lower tickets, whose workgroups are
-// already running and publish before they wait.
+// already running and publish before they wait.  Gated (list mode), a block
+// with no cell of a rebuilding replica is skipped, and a block whose first
+// cell's replica does not rebuild does not look back.  Listed (gated
+// batches): the blocks to sort come from k_decide's list, taken in ticket
+// order from a per-step counter, with the step's epoch and a status array
+// of their own; workgroups beyond the list leave before taking a ticket.
 // ---------------------------------------------------------------------------
 constexpr unsigned long long kLbAggregate = 1ull, kLbInclusive = 2ull;
 constexpr int kSortIds = 4096;  // slot entries a sort workgroup copies to LDS
