@@ -1,0 +1,90 @@
+"""Pair tables with several types and per-particle sigma, against the FP64
+oracle (femmd2D/src/interactionsCuda.cu:43-100: sigma_ij = (sigma_i +
+sigma_j) / 2, eps and rc (in sigma units) and the additive shift from the
+row-major ntypes x ntypes tables, femmd2D/src/interactions.cpp:65-79).
+
+  * two types with different eps, rc and shift, mixed sigmas: the typed
+    cell walk (typed tables always walk the cells);
+  * one type with mixed sigmas: the neighbour-list path with per-particle
+    sigma (the list radius from the largest sigma pair) and the cell walk.
+Interacting sets are bit-exact with the oracle's FP32 predicate; forces
+meet the 1e-5 bar of oracle.force_error; energies and virials 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from femmd_old_amd import System
+
+pytestmark = pytest.mark.gpu
+
+
+def _liquid(n, L, seed):
+    """n particles on a jittered square lattice of the box (spacing >= 1.2,
+    jitter 0.08: no close contacts)."""
+    rng = np.random.default_rng(seed)
+    m = int(np.ceil(np.sqrt(n)))
+    g = (np.arange(m) + 0.5) * (L / m)
+    gx, gy = np.meshgrid(g, g)
+    x = (gx.ravel()[:n] + rng.uniform(-0.08, 0.08, n)).astype(np.float32)
+    y = (gy.ravel()[:n] + rng.uniform(-0.08, 0.08, n)).astype(np.float32)
+    vx = rng.normal(0, 0.5, n).astype(np.float32)
+    vy = rng.normal(0, 0.5, n).astype(np.float32)
+    return x, y, vx, vy, rng
+
+
+def _check(s, osys_of, steps=(0, 25, 25)):
+    for k in steps:
+        if k:
+            s.run(k)
+        x, y, _, _ = s.get_state()
+        osys = osys_of(x, y)
+        cnt, s1, s2 = s.pair_sets()
+        ocnt, os1, os2 = O.pair_sets(osys, 1)
+        np.testing.assert_array_equal(cnt, ocnt)
+        np.testing.assert_array_equal(s1, os1)
+        fx, fy = s.get_forces()
+        rx, ry, en, fs = O.forces(osys, pred_f32=True, scale=True)
+        err = O.force_error(fx, fy, rx, ry, fs)
+        e = s.energies()
+        print(f"  max rel force err {err.max():.3e}, U {e['u_lj']:.6f} / {en[O.OR_U_LJ]:.6f}")
+        assert err.max() <= 1e-5
+        assert abs(e["u_lj"] - en[O.OR_U_LJ]) <= 1e-5 * (abs(en[O.OR_U_LJ]) + 1.0)
+        assert abs(e["w_lj"] - en[O.OR_W_LJ]) <= 1e-5 * (abs(en[O.OR_W_LJ]) + 1.0)
+    assert s.device_errors() == 0
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_two_type_table_with_mixed_sigma(skin):
+    L, n = 60.0, 2400
+    x, y, vx, vy, rng = _liquid(n, L, 11)
+    sig = rng.choice([0.9, 1.1], n).astype(np.float32)
+    typ = rng.integers(0, 2, n).astype(np.int32)
+    eps = np.array([1.0, 0.6, 0.6, 1.4], np.float32)
+    rc = np.array([2.5, 1.8, 1.8, 2.2], np.float32)
+    us = np.array([0.0, 0.05, 0.05, -0.02], np.float32)
+    s = System(L, L, dt=0.002)
+    s.set_skin(skin)
+    s.set_pair_potential(eps, rc, us)
+    s.add_particles(x, y, vx, vy, 1.0, sig, typ)
+    s.allocate_cuda()
+    Lf = O.box_f32(L)
+    _check(s, lambda px, py: O.OracleSystem(Lf, Lf, px, py, sigma=sig, type=typ, eps=eps, rc=rc,
+                                            ushift=us))
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.4, 0.0])
+def test_single_type_mixed_sigma(skin):
+    L, n = 75.0, 3600
+    x, y, vx, vy, rng = _liquid(n, L, 12)
+    sig = rng.uniform(0.85, 1.15, n).astype(np.float32)
+    s = System(L, L, dt=0.002)
+    s.set_skin(skin)
+    s.set_pair_potential([1.0], [2.5], [0.0])
+    s.add_particles(x, y, vx, vy, 1.0, sig)
+    s.allocate_cuda()
+    Lf = O.box_f32(L)
+    _check(s, lambda px, py: O.OracleSystem(Lf, Lf, px, py, sigma=sig, eps=[1.0], rc=[2.5],
+                                            ushift=[0.0]))
+    if skin > 0:
+        assert s.list_stats()["rebuilds"] >= 1
