@@ -447,3 +447,24 @@ def test_baseline_matches_fp64_oracle_short():
     d = bx - x
     d -= w.Lx * np.round(d / w.Lx)
     assert np.abs(d).max() < 1e-4
+
+
+def test_oracle_under_host_sanitizers():
+    """Every oracle entry point on a mixed system (two types, mixed sigma,
+    exclusions, bonds, CST elements of each model, the threaded baseline)
+    built with AddressSanitizer and UndefinedBehaviorSanitizer: no invalid
+    access, overflow or undefined operation (make -C oracle sanitize)."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+    if shutil.which("make") is None or shutil.which("g++") is None:
+        pytest.skip("make / g++ not available")
+    odir = Path(__file__).resolve().parent.parent / "oracle"
+    r = subprocess.run(["make", "-s", "-C", str(odir), "sanitize"], capture_output=True, text=True,
+                       timeout=600)
+    try:
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert "oracle sanitizer run ok" in r.stdout
+        assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
+    finally:
+        (odir / "sanitize_check").unlink(missing_ok=True)
