@@ -12,7 +12,9 @@
     against the FP64 oracle, both pair paths;
   * two distinct particles on one spot interact (self is excluded by index,
     as in the reference) with an infinite LJ force, which is reported as a
-    non-finite state, not a fault.
+    non-finite state, not a fault;
+  * more list partners than the list capacity are reported as a list
+    overflow, not written past the lists.
 """
 import numpy as np
 import pytest
@@ -122,6 +124,29 @@ def test_coincident_particles_are_reported(skin):
         s.run(2)
     assert e.value.code == _lib.FM_ERR_DEVICE
     assert s.device_errors() & _lib.FM_DEV_NONFINITE
+    s.clear_errors()
+    assert s.device_errors() == 0
+    s.close()
+
+
+def test_list_overflow_is_reported_not_faulting():
+    """60 particles within 1.5 of a cell corner (15 per cell, under the cell
+    capacity of 32) have 59 list partners each, over the list capacity of
+    48 (1.3 x the close-packed count of the list disc): reported as
+    FM_DEV_LIST_OVERFLOW, no write past the lists."""
+    rng = np.random.default_rng(9)
+    r = 1.5 * np.sqrt(rng.uniform(0, 1, 60))
+    t = rng.uniform(0, 2 * np.pi, 60)
+    x = np.concatenate([20.0 + r * np.cos(t), rng.uniform(2, 8, 20)]).astype(np.float32)
+    y = np.concatenate([20.0 + r * np.sin(t), rng.uniform(2, 8, 20)]).astype(np.float32)
+    s = _system(40.0, 0.6)
+    s.add_particles(x, y)
+    with pytest.raises(FemmdError) as e:
+        s.allocate_cuda()
+        s.calc_forces()
+    assert e.value.code == _lib.FM_ERR_DEVICE
+    assert s.device_errors() & _lib.FM_DEV_LIST_OVERFLOW
+    assert not s.device_errors() & _lib.FM_DEV_CELL_OVERFLOW
     s.clear_errors()
     assert s.device_errors() == 0
     s.close()
