@@ -15,9 +15,12 @@ first kernel).  --skin 0 walks the cells every step instead.  Inputs are
 resident in HBM when the timed region starts.
 
 Prints one JSON line: value = particle-steps/s; roofline for the dominant
-kernel (live HIP-event timing on the engine's stream, algorithmic bytes);
-cpu_baseline = the reference CPU path restated in C++ (oracle/), timed on a
-bounded sample of the same workload on this host.
+kernel (live HIP-event timing on the engine's stream, algorithmic bytes),
+with the step's fused lower bound (bmin) and a measured 1 GiB device copy
+beside the 8 TB/s peak; cpu_baseline = the reference CPU path restated in
+C++ (oracle/), timed on a bounded sample of the same workload on this host
+with 8 threads (the reference's OpenMP pair loop), cpu_baseline_1core as
+shipped (one thread, a shorter sample).
 """
 from __future__ import annotations
 
@@ -194,6 +197,53 @@ def copy_probe(nbytes: int):
     print(f"[bench] copy probe: {5} launches of {n * 4} bytes read + written", file=sys.stderr)
 
 
+def hbm_copy_probe(nbytes: int = 1 << 30, reps: int = 5):
+    """Measured device copy bandwidth (read + write of nbytes, larger than the
+    256 MB Infinity Cache, timed with HIP events) and the device's name and
+    compute-unit count, reported beside the 8 TB/s peak.  Runs in its own
+    process (bench.py --copy-bw): torch's HIP runtime and the engine's are
+    not loaded into one process."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    p = torch.cuda.get_device_properties(0)
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1), {"name": p.name, "compute_units": p.multi_processor_count,
+                           "memory_GB": round(p.total_memory / 1e9, 1)}
+
+
+def copy_bw_child():
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--copy-bw"],
+                           capture_output=True, text=True, timeout=300)
+        d = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+        return d["gbps"], d["device"]
+    except Exception as e:  # reported, not fatal: the step timing stands alone
+        return None, {"error": str(e)[:200]}
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpus": os.cpu_count(), "model": model}
+
+
 def cpu_baseline(w, steps, threads):
     import numpy as np
     import oracle as O
@@ -233,6 +283,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (5 launches), the "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--copy-bw", action="store_true",
+                    help="only measure the device copy bandwidth (1 GiB, JSON line)")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests)")
     args = ap.parse_args()
@@ -244,6 +296,10 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.copy_bw:
+        gbs, dev = hbm_copy_probe()
+        print(json.dumps({"gbps": gbs, "device": dev}), flush=True)
         return
 
     from femmd_old_amd import configs
@@ -286,6 +342,13 @@ def main():
             "step_kernels_ms": {k: round(v, 5) for k, v in prof.items()},
             "step_bytes": int(step_bytes),
             "step_frac": round(step_bytes / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the fused lower bound of a step: each particle's state, element and
+    # bond record read and written once (36 B/particle, 36 B/element,
+    # 16 B/bond); its fraction of the roofline bounds any fusion of the passes
+    bmin = 36 * w.n_total + 36 * w.n_tri + 16 * w.n_bonds
+    roof["bmin_bytes"] = bmin
+    roof["bmin_frac"] = round(bmin / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
+    roof["measured_copy_GBps"], roof["device"] = copy_bw_child()
 
     E0 = e0["ke"] + e0["u_lj"] + e0["u_fea"] + e0["u_bond"]
     E1 = e1["ke"] + e1["u_lj"] + e1["u_fea"] + e1["u_bond"]
@@ -320,6 +383,10 @@ def main():
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
+        out["cpu_baseline"]["host"] = host_info()
+        if args.cpu_threads != 1:
+            # the reference as shipped (no OpenMP): one core, a shorter sample
+            out["cpu_baseline_1core"] = cpu_baseline(w, max(4, args.cpu_steps // 8), 1)
     print(json.dumps(out), flush=True)
 
 
