@@ -84,9 +84,8 @@ def test_reorder_schedule_batched_equals_looped():
         assert s.reorder_info()["count"] >= 3
         singles.append(s.get_state())
     systems = [configs.build_system(w, commit=False) for w in ws]
-    for s in systems:
-        s.set_reorder_interval(7)
     b = Batch(systems)
+    b.set_reorder_interval(7)  # the batch's own entry point
     b.run(60)
     for r, st in enumerate(singles):
         for a, c in zip(st, b.get_state(r)):
