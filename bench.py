@@ -15,7 +15,8 @@ first kernel).  --skin 0 walks the cells every step instead.  Inputs are
 resident in HBM when the timed region starts.
 
 Prints one JSON line: value = particle-steps/s; roofline for the dominant
-kernel (live HIP-event timing on the engine's stream, algorithmic bytes),
+kernel (its dispatches' start and end timestamps, taken live by HIP events
+armed on each launch, and its algorithmic bytes),
 with the step's fused lower bound (bmin) and a measured 1 GiB device copy
 beside the 8 TB/s peak; cpu_baseline = the reference CPU path restated in
 C++ (oracle/), timed on a bounded sample of the same workload on this host

@@ -405,7 +405,8 @@ class System:
         return ms.value
 
     def profile(self, nsteps: int = 10) -> dict:
-        """Per-step device ms of each kernel (HIP events, engine stream)."""
+        """Per-step device ms of each kernel (its dispatch's start and end
+        timestamps) and of the whole step body."""
         ms = (C.c_double * 6)()
         self._check(self._lib.fm_profile_step(self._h, int(nsteps), ms))
         return dict(zip(("kick", "bin", "sort", "list_build", "forces", "step"), list(ms)))

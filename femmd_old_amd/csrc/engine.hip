@@ -692,16 +692,18 @@ int Engine::launch_step_body(bool kick2, bool drift) {
         k.xys = xys_.p;
     }
     const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[0], stream_);
+    time_next(0);
     e = launch_kick(mode, gather_, energy_, (int)kspans_h_.size(), k, stream_);
+    arm_kernel_timer(nullptr, nullptr);
     if (e != hipSuccess) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[1], stream_);
     if (!drift) return 0;
     if (!list_) return launch_forces_only(false);
     // list mode: binning, sort and list build run only for the replicas
     // where some particle moved more than skin / 2 since the last build
-    if (launch_bin(true)) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);
+    time_next(1);
+    const int bad = launch_bin(true);
+    arm_kernel_timer(nullptr, nullptr);
+    if (bad) return -1;
     if (launch_sort(true)) return -1;
     if (launch_list_build(true)) return -1;
     return launch_pair_pass();
@@ -744,9 +746,10 @@ int Engine::launch_sort(bool gated) {
             s.status2 = sort_status2_.p;
         }
     }
-    if (::fm::launch_sort(s, stream_) != hipSuccess) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[3], stream_);
-    return 0;
+    time_next(2);
+    const hipError_t e = ::fm::launch_sort(s, stream_);
+    arm_kernel_timer(nullptr, nullptr);
+    return e == hipSuccess ? 0 : -1;
 }
 
 // Neighbour lists of the replicas flagged for a rebuild (gated), or of all.
@@ -757,29 +760,30 @@ int Engine::launch_list_build(bool gated) {
         f.ftile = ftile_.p;
         f.nft = nf_.p + 1;
     }
+    time_next(3);
     const hipError_t e = ::fm::launch_list_build(sigma_, block_, f, stream_);
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);
+    arm_kernel_timer(nullptr, nullptr);
     return e == hipSuccess ? 0 : -1;
 }
 
 int Engine::launch_pair_pass() {
     const ForceArgs f = force_args();
+    time_next(4);
     const hipError_t e = launch_forces(sigma_, types_, false, true, energy_, block_, f, stream_);
+    arm_kernel_timer(nullptr, nullptr);
     ene_valid_ = energy_;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
 
 // Cell-walk force pass over the current binning (sort first).
 int Engine::launch_forces_only(bool debug) {
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[2], stream_);  // binning is in the kick
-    if (launch_sort()) return -1;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[4], stream_);  // no list build
+    if (launch_sort()) return -1;  // (the binning is in the kick)
     const ForceArgs f = force_args();
+    time_next(4);
     const hipError_t e =
         launch_forces(sigma_, types_, debug, false, energy_ || debug, block_, f, stream_);
+    arm_kernel_timer(nullptr, nullptr);
     ene_valid_ = energy_ || debug;
-    if (prof_ev_) (void)hipEventRecord(prof_ev_[kProfEvents - 1], stream_);
     return e == hipSuccess ? 0 : -1;
 }
 
@@ -850,7 +854,7 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     if (nsteps <= 0) { err = "nsteps must be > 0"; return FM_ERR_ARG; }
     int rc = ensure_primed(err);
     if (rc) return rc;
-    const int K = kProfEvents;
+    const int K = 2 * kProfKernels;
     std::vector<hipEvent_t> ev(K * nsteps);
     for (auto &e : ev) {
         rc = check(hipEventCreate(&e), "event", err);
@@ -870,19 +874,20 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps + 1;
     rc = sync(err);
-    double acc[kProfEvents] = {0};
+    // out[k] = dispatch duration of kernel k (0 where the mode launches no
+    // such kernel: its events were never recorded), out[5] = whole body,
+    // from the kick's start to the force pass's end
+    double acc[kProfKernels + 1] = {0};
     for (int s = 0; s < nsteps && !rc; ++s) {
         float t;
         hipEvent_t *e = &ev[K * s];
-        for (int k = 0; k + 1 < K; ++k) {
-            (void)hipEventElapsedTime(&t, e[k], e[k + 1]);
-            acc[k] += t;
-        }
-        (void)hipEventElapsedTime(&t, e[0], e[K - 1]);
-        acc[K - 1] += t;  // out[K - 1] = whole body (from the first event)
+        for (int k = 0; k < kProfKernels; ++k)
+            if (hipEventElapsedTime(&t, e[2 * k], e[2 * k + 1]) == hipSuccess) acc[k] += t;
+        if (hipEventElapsedTime(&t, e[0], e[K - 1]) == hipSuccess) acc[kProfKernels] += t;
     }
+    (void)hipGetLastError();  // the elapsed times of unrecorded events failed
     for (auto &e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k < K; ++k) out[k] = acc[k] / nsteps;
+    for (int k = 0; k <= kProfKernels; ++k) out[k] = acc[k] / nsteps;
     return rc;
 }
 

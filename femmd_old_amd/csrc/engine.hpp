@@ -19,6 +19,10 @@
 
 namespace fm {
 
+// the next timed launch (kick, binning, sort, list build, forces) records
+// its dispatch's start and end on these events; (nullptr, nullptr) disarms
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop);
+
 struct HostSystem {
     double Lx = 0, Ly = 0, dt = 0.005;
     int device = 0;
@@ -237,10 +241,13 @@ private:
     DBuf<int32_t> dbg_cnt_;
     DBuf<long long> dbg_s1_, dbg_s2_;
 
-    // profiling: events before the kick and after kick, bin, sort, list
-    // build and forces (set only while profiling)
-    static constexpr int kProfEvents = 6;
+    // profiling: a start and an end event per timed kernel of a step body
+    // (kick, binning, sort, list build, forces; set only while profiling)
+    static constexpr int kProfKernels = 5;
     hipEvent_t *prof_ev_ = nullptr;
+    void time_next(int k) {
+        if (prof_ev_) arm_kernel_timer(prof_ev_[2 * k], prof_ev_[2 * k + 1]);
+    }
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     int graph_steps_ = 0;

@@ -20,6 +20,8 @@
 // must agree bit for bit with the reference's real=float arithmetic (wrap,
 // cell key, minimum image, cutoff predicate) is evaluated in its source
 // order; fused multiply-adds are written explicitly where wanted.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -2620,6 +2622,29 @@ hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_loss, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
+// Kernel timer of the profile pass: an armed pair of events takes the start
+// and end timestamps of the next timed dispatch (hipExtLaunchKernel), the
+// interval a kernel trace reports, without the gaps that event records
+// between launches add.  Per host thread; armed only around eager launches.
+static thread_local hipEvent_t t_start = nullptr, t_stop = nullptr;
+
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop) {
+    t_start = start;
+    t_stop = stop;
+}
+
+// a timed kernel's launch: by the armed events if any (which it disarms)
+#define FM_GO(KER, GRID, BLK, SHM, STR, ...)                                              \
+    do {                                                                                  \
+        if (t_start) {                                                                    \
+            hipExtLaunchKernelGGL(KER, GRID, BLK, (uint32_t)(SHM), STR, t_start, t_stop, 0u, \
+                                  __VA_ARGS__);                                           \
+            t_start = t_stop = nullptr;                                                   \
+        } else {                                                                          \
+            hipLaunchKernelGGL(KER, GRID, BLK, SHM, STR, __VA_ARGS__);                    \
+        }                                                                                 \
+    } while (0)
+
 hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickArgs &a,
                        hipStream_t s) {
     // mode: 0 = bin only, 1 = kick + drift, 2 = final kick, 3 = both,
@@ -2631,8 +2656,8 @@ hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickA
     const bool list = a.dmax != nullptr;
 #define FM_KICK1(K2, D, G, E) \
     do { \
-        if (list) hipLaunchKernelGGL((k_kick<K2, D, G, true, E>), g, b, 0, s, a); \
-        else hipLaunchKernelGGL((k_kick<K2, D, G, false, E>), g, b, 0, s, a); \
+        if (list) FM_GO((k_kick<K2, D, G, true, E>), g, b, 0, s, a); \
+        else FM_GO((k_kick<K2, D, G, false, E>), g, b, 0, s, a); \
     } while (0)
 #define FM_KICK(K2, D, G) \
     do { \
@@ -2643,9 +2668,9 @@ hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickA
             // gated list-mode binning: a smaller grid-stride grid keeps the
             // skipped steps cheap
             if (a.ctl) g = dim3(std::min(nblk, kGatedBinGrid));
-            hipLaunchKernelGGL(k_bin, g, b, 0, s, a);
+            FM_GO(k_bin, g, b, 0, s, a);
             break;
-        case 4: hipLaunchKernelGGL(k_ke, g, b, 0, s, a); break;
+        case 4: FM_GO(k_ke, g, b, 0, s, a); break;
         case 1:
             if (gather) FM_KICK1(false, true, true, false); else FM_KICK1(false, true, false, false);
             break;
@@ -2663,7 +2688,7 @@ hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickA
 
 hipError_t launch_sort(const ScanArgs &a, hipStream_t s) {
     if (a.nsort <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort, dim3(a.nsort), dim3(kSortCells), 0, s, a);
+    FM_GO(k_sort, dim3(a.nsort), dim3(kSortCells), 0, s, a);
     return hipGetLastError();
 }
 
@@ -2680,7 +2705,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
         if (nblk <= 0) return hipSuccess;
         dim3 g(nblk), b(block);
         const size_t lds = (size_t)(a.lds_cap + 1) * (sizeof(float2) + (sigma ? sizeof(float) : 0));
-#define FM_LIST(S, E) hipLaunchKernelGGL((k_forces<S, false, false, true, E>), g, b, lds, s, a)
+#define FM_LIST(S, E) FM_GO((k_forces<S, false, false, true, E>), g, b, lds, s, a)
         if (sigma) { if (energy) FM_LIST(true, true); else FM_LIST(true, false); }
         else { if (energy) FM_LIST(false, true); else FM_LIST(false, false); }
 #undef FM_LIST
@@ -2690,7 +2715,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
     if (nblk <= 0) return hipSuccess;
     const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
     dim3 g(nblk), b(block);
-#define FM_LAUNCH(S, T, D, E) hipLaunchKernelGGL((k_forces<S, T, D, false, E>), g, b, lds, s, a)
+#define FM_LAUNCH(S, T, D, E) FM_GO((k_forces<S, T, D, false, E>), g, b, lds, s, a)
     if (debug) {  // the dump pass always accumulates energies
         if (sigma) { if (types) FM_LAUNCH(true, true, true, true); else FM_LAUNCH(true, false, true, true); }
         else { if (types) FM_LAUNCH(false, true, true, true); else FM_LAUNCH(false, false, true, true); }
@@ -2732,9 +2757,9 @@ hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStrea
     const int gated = a.gated_grid > 0 ? a.gated_grid : kGatedGrid;
     const dim3 g(a.ctl ? std::min(a.ntiles, gated) : a.ntiles);
     if (sigma)
-        hipLaunchKernelGGL((k_list_build<true>), g, dim3(block), lds, s, a);
+        FM_GO((k_list_build<true>), g, dim3(block), lds, s, a);
     else
-        hipLaunchKernelGGL((k_list_build<false>), g, dim3(block), lds, s, a);
+        FM_GO((k_list_build<false>), g, dim3(block), lds, s, a);
     return hipGetLastError();
 }
 

@@ -187,10 +187,12 @@ int fm_get_cells(fm_system *h, int32_t *ncx, int32_t *ncy, int32_t *key,
 int fm_pair_sets(fm_system *h, int32_t *cnt, int64_t *s1, int64_t *s2);
 /* Time one force pass on the device (ms per pass, averaged over reps). */
 int fm_time_forces(fm_system *h, int32_t reps, double *ms);
-/* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel with HIP
- * events on the engine's stream (no graph).  ms[6] = per step: kick (with
- * the binning when cells are walked every step), binning (list mode),
- * counting sort, list build, force pass, whole step. */
+/* Advance nsteps + 1 steps, timing nsteps of them kernel by kernel (no
+ * graph): each kernel's dispatch start and end timestamps, taken by HIP
+ * events armed on its launch (hipExtLaunchKernel).  ms[6] = per step: kick
+ * (with the binning when cells are walked every step), binning (list mode),
+ * counting sort, list build, force pass (0 for a kernel the mode does not
+ * launch), whole step from the kick's start to the force pass's end. */
 int fm_profile_step(fm_system *h, int32_t nsteps, double *ms);
 /* Elastomer clusters (one per fm_add_elastomer call, in call order). */
 int fm_cluster_count(fm_system *h, int32_t *nc);
