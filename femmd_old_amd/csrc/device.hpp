@@ -103,6 +103,7 @@ struct KickArgs {
     int32_t nspans;            // spans (grid-stride binning)
     const int32_t *fspan;      // gated batches: the rebuilding replicas' spans
     const int32_t *nfs;        // (decided by k_decide), and their count
+    int32_t one_rep;           // gated binning of a single replica (decided first)
 };
 
 // Rebuild decision of a batch (one workgroup): per replica the flag, and

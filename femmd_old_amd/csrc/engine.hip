@@ -640,6 +640,7 @@ int Engine::launch_bin(bool gated) {
         a.ctl = ctl_.p;
         a.any = any_.p;
         a.dmax = dmax_.p;
+        a.one_rep = R_ == 1;
         if (R_ > kDecideMinReps) {
             DecideArgs d{};
             d.reps = reps_.p;

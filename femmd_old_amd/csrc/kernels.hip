@@ -420,6 +420,22 @@ __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
 
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
   int last_rep = -1, flag = 1;
+  if (a.ctl && !a.fspan && a.one_rep) {
+      // a single replica: the decision comes first, from the replica's
+      // descriptor and displacement slots loaded together (no span read
+      // before them), so that a step without a rebuild leaves at once
+      const RepDesc &R0 = a.reps[0];
+      flag = list_disp(a.dmax, a.ctl, 0, R0) > 0.5f * R0.skin;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {  // as list_decide records it
+          a.ctl->flag = flag;
+          if (flag) {
+              a.ctl->nbuild += 1u;
+              *a.any = 1;
+          }
+      }
+      if (!flag) return;
+      last_rep = 0;
+  }
   // decided (batches): only the rebuilding replicas' spans, listed
   const int nsp = a.fspan ? *a.nfs : a.nspans;
   for (int b = blockIdx.x; b < nsp; b += gridDim.x) {
