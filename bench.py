@@ -38,9 +38,11 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 
-def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
+def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None, uniform_mass=False):
     """Bytes each kernel must move per executed launch (each input array
     read once, each output written once; FP32 values, int32 indices).
+    uniform_mass: every particle has the same (finite) mass, which the kick
+    takes from the replica's descriptor: 4/particle less (no inv_m).
 
     Per-step cell walk (list_len None):
     kick     r x,y,vx,vy,fx,fy,inv_m,goff; w x,y,vx,vy,slot: 52/particle;
@@ -81,6 +83,8 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None):
             "list_build": int((20 + slots) * n) + 4 * ncells,
             "forces": int((24 + slots) * n) + 4 * ncells + elems,
         }
+    if uniform_mass:
+        b["kick"] -= 4 * n
     return b
 
 
@@ -328,8 +332,11 @@ def main():
           flush=True)
 
     prof = s.profile(args.profile_steps)
+    import numpy as np
+    m = w.global_arrays()["mass"]
     ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds,
-                           ls1["mean"] if list_mode else None)
+                           ls1["mean"] if list_mode else None,
+                           uniform_mass=bool(np.isfinite(m[0]) and m[0] > 0 and np.all(m == m[0])))
     kern = "forces"
     achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
     step_bytes = ab["kick"] + ab["forces"] + rebuild_frac * sum(ab[k] for k in REBUILD_KERNELS)

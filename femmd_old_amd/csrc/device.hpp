@@ -45,6 +45,8 @@ struct alignas(16) RepDesc {
     float rl_max;      // list mode: farthest reach of a list entry,
                        // rc*sigma_max + 2 skin (only a particle that close
                        // to an edge has pairs needing the minimum image)
+    float im_uniform;  // 1/m shared by every particle of the replica (the
+                       // kick then loads no inv_m); 0: masses differ
 };
 
 // Displacement maxima of a kick: each workgroup takes an atomic max into one

@@ -259,6 +259,12 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             ht.push_back(S.type[u]);
             huid.push_back(u);
         }
+        {  // one inverse mass for the whole replica (none pinned)?
+            const float f = n > 0 ? him[p0] : 0.0f;
+            bool uni = f > 0.0f;
+            for (int k = 1; uni && k < n; ++k) uni = him[p0 + k] == f;
+            D.im_uniform = uni ? f : 0.0f;
+        }
         D.ks0 = (int32_t)ks.size();
         for (int b = p0; b < p0 + n; b += kKickThreads)
             ks.push_back(Span{r, b, std::min(b + kKickThreads, p0 + n), 0});

@@ -176,7 +176,9 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
                 fy += c.y;
             }
         }
-        const float im = a.inv_m[p];
+        float im;  // (the same value either way: no load when uniform)
+        if (R.im_uniform > 0.0f) im = R.im_uniform;
+        else im = a.inv_m[p];
         const float ax = fx * im, ay = fy * im;
         float vx = a.vx[p], vy = a.vy[p];
         if (kKick2) {

@@ -36,3 +36,8 @@ def test_algorithmic_bytes_model():
     assert lists["list_build"] == int((20 + 2 * 21.5) * n) + 4 * ncells
     assert lists["forces"] == int((24 + 2 * 21.5) * n) + 4 * ncells + elems
     assert set(bench.REBUILD_KERNELS) == {"bin", "sort", "list_build"}
+    # one mass for every particle: the kick reads no inverse masses
+    uni = bench.algorithmic_bytes(n, ncells, ntri, nb, 20.0, uniform_mass=True)
+    assert uni["kick"] == lists["kick"] - 4 * n
+    assert {k: v for k, v in uni.items() if k != "kick"} == \
+        {k: v for k, v in lists.items() if k != "kick"}
