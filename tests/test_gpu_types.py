@@ -9,6 +9,12 @@ row-major ntypes x ntypes tables, femmd2D/src/interactions.cpp:65-79).
     sigma (the list radius from the largest sigma pair) and the cell walk.
 Interacting sets are bit-exact with the oracle's FP32 predicate; forces
 meet the 1e-5 bar of oracle.force_error; energies and virials 1e-5.
+
+  * particle masses: two masses (the kick loads each particle's 1/m) and
+    one mass other than 1 for all (the kick takes 1/m from the replica's
+    descriptor): 100 steps against the FP64 oracle's KDK rollout
+    (femmd2D/src/integrator.cpp:100-162), with the trajectory bars of the
+    C1 parity test.
 """
 import numpy as np
 import pytest
@@ -88,3 +94,33 @@ def test_single_type_mixed_sigma(skin):
                                             ushift=[0.0]))
     if skin > 0:
         assert s.list_stats()["rebuilds"] >= 1
+
+
+@pytest.mark.parametrize("masses", ["mixed", "uniform"])
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_masses_trajectory(masses, skin):
+    L, n, dt = 60.0, 2400, 0.005
+    x, y, vx, vy, rng = _liquid(n, L, 13)
+    m = (rng.choice([1.0, 2.5], n) if masses == "mixed" else np.full(n, 2.5)).astype(np.float32)
+    s = System(L, L, dt=dt)
+    s.set_skin(skin)
+    s.set_pair_potential([1.0], [2.5], [0.0])
+    s.add_particles(x, y, vx, vy, m)
+    s.allocate_cuda()
+    x0, y0, vx0, vy0 = s.get_state()
+    Lf = O.box_f32(L)
+    osys = O.OracleSystem(Lf, Lf, x0, y0, eps=[1.0], rc=[2.5], ushift=[0.0])
+    s.run(100)
+    x1, y1, vx1, vy1 = s.get_state()
+    ox, oy, ovx, ovy, _ = O.run(osys, x0, y0, vx0, vy0, m, 100, dt)
+    dx = x1 - ox
+    dx -= Lf * np.round(dx / Lf)
+    dy = y1 - oy
+    dy -= Lf * np.round(dy / Lf)
+    dpos = np.hypot(dx, dy)
+    dvel = np.hypot(vx1 - ovx, vy1 - ovy)
+    print(f"  {masses} skin {skin}: max |dx| {dpos.max():.3e} rms "
+          f"{np.sqrt((dpos ** 2).mean()):.3e} max |dv| {dvel.max():.3e}")
+    assert dpos.max() < 2e-3
+    assert np.sqrt((dpos ** 2).mean()) < 1e-4
+    assert s.device_errors() == 0
