@@ -16,7 +16,9 @@ resident in HBM when the timed region starts.
 
 Prints one JSON line: value = particle-steps/s; roofline for the dominant
 kernel (its dispatches' start and end timestamps, taken live by HIP events
-armed on each launch, and its algorithmic bytes),
+armed on each launch, and its algorithmic bytes; valu_frac = the kernel's
+pure vector-issue time, from the committed SQ_INSTS_VALU count, over its
+launch time),
 with the step's fused lower bound (bmin) and a measured 1 GiB device copy
 beside the 8 TB/s peak; cpu_baseline = the reference CPU path restated in
 C++ (oracle/), timed on a bounded sample of the same workload on this host
@@ -101,6 +103,20 @@ def measured_traffic(kernel: str, mode: str):
         for line in p.read_text().splitlines():
             if line.startswith(f"traffic_bytes_per_launch {kernel}[{mode}]:"):
                 return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def measured_valu_us(kernel: str, mode: str):
+    """Pure VALU issue time (us) of one launch of `kernel` on the full c2
+    workload in `mode`, from the committed SQ_INSTS_VALU summary
+    (instructions x 4 cycles / (1024 SIMDs x 2.4 GHz)); None when absent."""
+    p = ROOT / "profiles" / "k_forces_c2_sq_counters.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith(f"valu_us_per_launch {kernel}[{mode}]:"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
@@ -354,6 +370,12 @@ def main():
     # bond record read and written once (36 B/particle, 36 B/element,
     # 16 B/bond); its fraction of the roofline bounds any fusion of the passes
     bmin = 36 * w.n_total + 36 * w.n_tri + 16 * w.n_bonds
+    valu = measured_valu_us(f"k_{kern}", mode) if args.scale == 1.0 else None
+    if valu is not None:
+        # the kernel's other bound: vector issue (its HBM fraction is low
+        # because it computes ~17 instructions per 2-byte list entry)
+        roof["valu_us"] = valu
+        roof["valu_frac"] = round(valu / (prof[kern] * 1e3), 4)
     roof["bmin_bytes"] = bmin
     roof["bmin_frac"] = round(bmin / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
     roof["measured_copy_GBps"], roof["device"] = copy_bw_child()

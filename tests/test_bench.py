@@ -41,3 +41,16 @@ def test_algorithmic_bytes_model():
     assert uni["kick"] == lists["kick"] - 4 * n
     assert {k: v for k, v in uni.items() if k != "kick"} == \
         {k: v for k, v in lists.items() if k != "kick"}
+
+
+def test_committed_counter_summaries_are_read():
+    """roofline.traffic and roofline.valu_us come from the committed
+    rocprofv3 summaries: both k_forces lines parse, and the measured HBM
+    bytes sit within 1.5 x of the algorithmic model at the c2 size."""
+    t = bench.measured_traffic("k_forces", "list")
+    v = bench.measured_valu_us("k_forces", "list")
+    assert t is not None and v is not None
+    ab = bench.algorithmic_bytes(1141760, 380 * 380, 153600, 0, 21.66)
+    assert 0.9 <= t / ab["forces"] <= 1.5
+    assert 5.0 < v < 100.0
+    assert bench.measured_valu_us("k_none", "list") is None
