@@ -108,7 +108,7 @@ struct KickArgs {
     int32_t one_rep;           // gated binning of a single replica (decided first)
 };
 
-// Rebuild decision of a batch (one workgroup): per replica the flag, and
+// Rebuild decision of a batch (k_flags, k_decide): per replica the flag, and
 // the compact lists of the rebuilding replicas' kick spans and tiles that
 // the gated binning and list build walk instead of every span and tile.
 // It also lists the sort blocks that hold a cell of a rebuilding replica (in

@@ -33,6 +33,9 @@ namespace fm {
 // the work, so a skipped step costs little (the binning workgroups also
 // each make the rebuild decision, so that grid is the smallest).
 constexpr int kGatedBinGrid = 1024;
+// Workgroups of the batch rebuild decision (each writes a share of the
+// compact span and tile lists).
+constexpr int kDecideGrid = 16;
 constexpr int kGatedGrid = 2048;
 
 // ---------------------------------------------------------------------------
@@ -314,9 +317,9 @@ __device__ int list_decide(const KickArgs &a, const Span &sp, const RepDesc &R) 
     return flag;
 }
 
-// The rebuild decision of a batch, in one workgroup: each lane takes a
-// replica (its bound as list_disp forms it), then the rebuilding replicas'
-// kick spans and tiles are written as two compact lists (inclusive scans
+// The rebuild decision of a batch: each lane takes a replica (its bound as
+// list_disp forms it), then the rebuilding replicas' kick spans and tiles
+// are written as two compact lists (inclusive scans
 // over the replicas, in replica order; each entry found by a binary search
 // of those scans in LDS), which the gated binning and list build walk
 // instead of every span and tile.  The sort blocks holding a cell of a
@@ -361,13 +364,18 @@ __global__ __launch_bounds__(256) void k_flags(DecideArgs d) {
     }
 }
 
-// The work lists of a batch, in one workgroup: the rebuilding replicas' kick
-// spans and tiles (inclusive scans over the replicas, in replica order; each
-// entry found by a binary search of those scans in LDS) and the marked sort
-// blocks in increasing order (the marks are cleared for the next step).
+// The work lists of a batch: the rebuilding replicas' kick spans and tiles
+// (inclusive scans over the replicas, in replica order; each entry found by
+// a binary search of those scans in LDS), written by kDecideGrid workgroups,
+// and the marked sort blocks in increasing order (by the first workgroup,
+// which clears the marks for the next step).
 __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
     __shared__ int inS[1024], inT[1024], fS[1024], fT[1024], wsum[16];
     const int t = threadIdx.x;
+    // every workgroup forms the same scans and writes its share of the
+    // span and tile lists; workgroup 0 alone lists the sort blocks (it
+    // clears their marks) and writes the counts
+    const int g = blockIdx.x, G = gridDim.x;
     int baseS = 0, baseT = 0;
     for (int c0 = 0; c0 < d.nrep; c0 += 1024) {
         const int r = c0 + t;
@@ -381,7 +389,7 @@ __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
         // entry v of this chunk's list belongs to the first replica whose
         // inclusive count exceeds v
         auto fill = [&](const int *in, const int *first, int tot, int base, int32_t *out) {
-            for (int v = t; v < tot; v += 1024) {
+            for (int v = g * 1024 + t; v < tot; v += G * 1024) {
                 int lo = 0, hi = 1023;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
@@ -397,6 +405,7 @@ __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
         baseT += totT;
         __syncthreads();
     }
+    if (g != 0) return;  // (uniform per workgroup)
     int nfb = 0;
     if (d.fblock) {
         for (int c0 = 0; c0 < d.nsort; c0 += 1024) {
@@ -2949,7 +2958,7 @@ __global__ __launch_bounds__(256) void k_ro_copy(ReorderArgs a) {
 
 hipError_t launch_decide(const DecideArgs &d, hipStream_t s) {
     hipLaunchKernelGGL(k_flags, dim3((d.nrep + 3) / 4), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1024), 0, s, d);
+    hipLaunchKernelGGL(k_decide, dim3(kDecideGrid), dim3(1024), 0, s, d);
     return hipGetLastError();
 }
 
