@@ -18,7 +18,7 @@ from typing import Iterable, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (FM_LEGACY_GPU, FM_NEO_HOOKEAN, FM_SVK, FmEnergies, FmParams)
+from ._lib import (FM_LEGACY_GPU, FM_NEO_HOOKEAN, FM_SVK, FmAverages, FmEnergies, FmParams)
 
 __all__ = ["System", "Batch", "FemmdError", "FM_NEO_HOOKEAN", "FM_SVK", "FM_LEGACY_GPU"]
 
@@ -171,7 +171,20 @@ class System:
         return np.concatenate(self._rec_tri) if self._rec_tri else np.zeros((0, 3), np.int32)
 
     def bond_pairs(self) -> np.ndarray:
-        return np.concatenate(self._rec_bond) if self._rec_bond else np.zeros((0, 2), np.int32)
+        """(nb, 2) caller ids of every bond (c_getBondsPtr,
+        femmd2D/src/interface.cpp:45)."""
+        n = C.c_int32()
+        self._check(self._lib.fm_num_bonds(self._h, C.byref(n)))
+        out = np.empty((n.value, 2), np.int32)
+        if n.value:
+            self._check(self._lib.fm_get_bonds(self._h, _ip(out)))
+        return out
+
+    def n_bonds(self) -> int:
+        """c_getNumberOfBonds (femmd2D/src/interface.cpp:42)."""
+        n = C.c_int32()
+        self._check(self._lib.fm_num_bonds(self._h, C.byref(n)))
+        return n.value
 
     def set_kBond(self, k: float):
         """System.set_kBond (femmd2D/src/interface.cpp:455-465): the stiffness
@@ -189,7 +202,6 @@ class System:
         k = _f32(np.broadcast_to(np.asarray(k, np.float32), (n,)))
         r0 = None if r0 is None else _f32(np.broadcast_to(np.asarray(r0, np.float32), (n,)))
         self._check(self._lib.fm_add_bonds(self._h, n, _ip(i), _ip(j), _fp(k), _fp(r0)))
-        self._rec_bond.append(np.stack([i, j], 1))
 
     def set_pair_potential(self, eps, rc, us):
         """System.set_pair_potential(eps, rc, us) (femmd2D/src/interface.cpp:493-531):
@@ -259,6 +271,23 @@ class System:
         e = FmEnergies()
         self._check(self._lib.fm_get_energies(self._h, C.byref(e)))
         return e.as_dict()
+
+    def set_observe_every(self, every: int):
+        """1 (default): every step forms and accumulates its observables on
+        the device, as Integrator::doStep -> Properties::compute does
+        (femmd2D/src/integrator.cpp:18); 0: only the last step of each run."""
+        self._check(self._lib.fm_set_observe_every(self._h, int(every)))
+
+    def averages(self) -> dict:
+        """Running averages (avg, stdv) of the per-step observables since the
+        last reset (femmd2D/src/properties.cpp:39-86), with their count."""
+        a = FmAverages()
+        self._check(self._lib.fm_get_averages(self._h, C.byref(a)))
+        return a.as_dict()
+
+    def reset_averages(self):
+        """Properties::accumulate(0) (femmd2D/src/properties.cpp:41-55)."""
+        self._check(self._lib.fm_reset_averages(self._h))
 
     def get_props(self) -> dict:
         """System.get_props (femmd2D/src/interface.cpp:627-644): current values
@@ -521,6 +550,11 @@ class Batch:
         e = FmEnergies()
         self._check(self._lib.fm_batch_get_energies(self._h, int(r), C.byref(e)))
         return e.as_dict()
+
+    def averages(self, r: int) -> dict:
+        a = FmAverages()
+        self._check(self._lib.fm_batch_get_averages(self._h, int(r), C.byref(a)))
+        return a.as_dict()
 
     def set_tuning(self, tile_cells: int = 0, block: int = 0, use_graph: int = -1):
         self._check(self._lib.fm_batch_set_tuning(self._h, int(tile_cells), int(block),

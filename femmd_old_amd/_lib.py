@@ -61,6 +61,24 @@ class FmEnergies(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+AVG_NAMES = ("uKinetic", "uPotential", "uTotal", "pressure", "virial", "feaVirial", "uBonded",
+             "uFea")
+
+
+class FmAverages(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64),
+        ("mean", C.c_double * 8),
+        ("stdv", C.c_double * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {"count": self.count}
+        for k, name in enumerate(AVG_NAMES):
+            d[name] = {"avg": self.mean[k], "stdv": self.stdv[k]}
+        return d
+
+
 _f = C.POINTER(C.c_float)
 _i = C.POINTER(C.c_int32)
 _l = C.POINTER(C.c_int64)
@@ -97,10 +115,16 @@ PROTOTYPES = {
     "fm_step": (C.c_int, [_h, C.c_int32]),
     "fm_compute_forces": (C.c_int, [_h]),
     "fm_get_energies": (C.c_int, [_h, C.POINTER(FmEnergies)]),
+    "fm_set_observe_every": (C.c_int, [_h, C.c_int32]),
+    "fm_get_averages": (C.c_int, [_h, C.POINTER(FmAverages)]),
+    "fm_reset_averages": (C.c_int, [_h]),
     "fm_get_state": (C.c_int, [_h, _f, _f, _f, _f]),
     "fm_set_state": (C.c_int, [_h, _f, _f, _f, _f]),
     "fm_get_forces": (C.c_int, [_h, _f, _f]),
     "fm_num_particles": (C.c_int, [_h, _i]),
+    "fm_num_bonds": (C.c_int, [_h, _i]),
+    "fm_get_bonds": (C.c_int, [_h, _i]),
+    "fm_num_elements": (C.c_int, [_h, _i]),
     "fm_sync": (C.c_int, [_h]),
     "fm_set_tuning": (C.c_int, [_h, C.c_int32, C.c_int32, C.c_int32]),
     "fm_device_errors": (C.c_uint32, [_h]),
@@ -118,6 +142,7 @@ PROTOTYPES = {
     "fm_batch_step": (C.c_int, [_h, C.c_int32]),
     "fm_batch_get_state": (C.c_int, [_h, C.c_int32, _f, _f, _f, _f]),
     "fm_batch_get_energies": (C.c_int, [_h, C.c_int32, C.POINTER(FmEnergies)]),
+    "fm_batch_get_averages": (C.c_int, [_h, C.c_int32, C.POINTER(FmAverages)]),
     "fm_batch_set_tuning": (C.c_int, [_h, C.c_int32, C.c_int32, C.c_int32]),
     "fm_batch_tune": (C.c_int, [_h, C.c_char_p]),
     "fm_batch_set_reorder_interval": (C.c_int, [_h, C.c_int32]),

@@ -150,6 +150,7 @@ int Engine::load_state(const float *src) {
             hipSuccess)
             return -1;
     primed_ = false;
+    obs_valid_ = false;
     return 0;
 }
 
@@ -269,6 +270,7 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
     };
     // ---- forward, checkpoint at every segment start and at the end; the
     // last segment records its positions as it goes (no energies needed) ----
+    obs_valid_ = false;
     energy_ = false;
     for (int s = 0; s < S; ++s) {
         if (list_ && S > 1) {

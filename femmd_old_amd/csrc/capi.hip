@@ -3,7 +3,9 @@
 // Argument checking, caller-order bookkeeping before fm_commit and status
 // codes; all device work is delegated to fm::Engine.  Errors never exit
 // the process (the reference's HANDLE_ERROR exits, femmd2D/src/CudaUtil.h:4-14).
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -82,9 +84,18 @@ int fm_add_particles(fm_system *h, int32_t n, const float *x, const float *y, co
     if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
     if (n < 0 || (n > 0 && (!x || !y || !mass || !sigma))) return fail(h, FM_ERR_ARG, "bad particle arrays");
     fm::HostSystem &S = h->hs;
-    if (first_id) *first_id = (int32_t)S.x.size();
+    // validate every input before anything is appended (all or nothing)
     for (int32_t k = 0; k < n; ++k) {
         if (!std::isfinite(x[k]) || !std::isfinite(y[k])) return fail(h, FM_ERR_ARG, "non-finite position");
+        if ((vx && !std::isfinite(vx[k])) || (vy && !std::isfinite(vy[k])))
+            return fail(h, FM_ERR_ARG, "non-finite velocity");
+        if (!(sigma[k] > 0) || !std::isfinite(sigma[k])) return fail(h, FM_ERR_ARG, "sigma must be positive");
+        if (std::isnan(mass[k])) return fail(h, FM_ERR_ARG, "mass is NaN");
+        if (type && type[k] < 0) return fail(h, FM_ERR_ARG, "negative particle type");
+    }
+    if ((int64_t)S.x.size() + n > INT32_MAX) return fail(h, FM_ERR_ARG, "too many particles");
+    if (first_id) *first_id = (int32_t)S.x.size();
+    for (int32_t k = 0; k < n; ++k) {
         S.x.push_back(x[k]);
         S.y.push_back(y[k]);
         S.vx.push_back(vx ? vx[k] : 0.0f);
@@ -112,12 +123,26 @@ int fm_add_elastomer(fm_system *h, int32_t nv, const float *verts, int32_t nt, c
     if (!(sigma > 0)) return fail(h, FM_ERR_ARG, "sigma must be positive");
     fm::HostSystem &S = h->hs;
     const int32_t off = (int32_t)S.x.size();
+    // validate every input before anything is appended or a group is taken
+    // (all or nothing)
     for (int32_t k = 0; k < 3 * nt; ++k) {
         const int32_t v = tris[k] - index_base;
         if (v < 0 || v >= nv) return fail(h, FM_ERR_ARG, "triangle index out of range");
     }
-    const int32_t group = exclude ? S.next_group++ : -1;
-    if (first_node) *first_node = off;
+    for (int32_t k = 0; k < nv; ++k) {
+        const float px = verts[2 * k] + cx, py = verts[2 * k + 1] + cy;
+        if (!std::isfinite(px) || !std::isfinite(py)) return fail(h, FM_ERR_ARG, "non-finite vertex");
+    }
+    if (type < 0) return fail(h, FM_ERR_ARG, "negative particle type");
+    for (int32_t e = 0; e < nt; ++e) {
+        // setElasticConstants (femmd2D/src/interactions.cpp:43-50): lambda and
+        // mu are finite only for nu in (-1, 1/2)
+        if (E_per_tri && !(E_per_tri[e] >= 0 && std::isfinite(E_per_tri[e])))
+            return fail(h, FM_ERR_ARG, "per-element E must be finite and >= 0");
+        if (nu_per_tri && !(nu_per_tri[e] > -1.0f && nu_per_tri[e] < 0.5f))
+            return fail(h, FM_ERR_ARG, "per-element nu must lie in (-1, 0.5)");
+    }
+    if ((int64_t)off + nv > INT32_MAX) return fail(h, FM_ERR_ARG, "too many particles");
     std::vector<double> m(nv, 0.0);
     if (node_mass > 0) {
         std::fill(m.begin(), m.end(), (double)node_mass);
@@ -139,9 +164,12 @@ int fm_add_elastomer(fm_system *h, int32_t nv, const float *verts, int32_t nt, c
     } else {
         return fail(h, FM_ERR_ARG, "node_mass or total_mass must be positive");
     }
+    for (int32_t k = 0; k < nv; ++k)
+        if (!std::isfinite(m[k])) return fail(h, FM_ERR_ARG, "non-finite node mass");
+    const int32_t group = exclude ? S.next_group++ : -1;
+    if (first_node) *first_node = off;
     for (int32_t k = 0; k < nv; ++k) {
         const float px = verts[2 * k] + cx, py = verts[2 * k + 1] + cy;
-        if (!std::isfinite(px) || !std::isfinite(py)) return fail(h, FM_ERR_ARG, "non-finite vertex");
         S.x.push_back(px);
         S.y.push_back(py);
         S.vx.push_back(0.0f);
@@ -171,9 +199,15 @@ int fm_add_bonds(fm_system *h, int32_t nb, const int32_t *i, const int32_t *j, c
     if (nb < 0 || (nb > 0 && (!i || !j || !k))) return fail(h, FM_ERR_ARG, "bad bond arrays");
     fm::HostSystem &S = h->hs;
     const int32_t n = (int32_t)S.x.size();
+    // validate every bond before any is appended (all or nothing)
     for (int32_t l = 0; l < nb; ++l) {
         if (i[l] < 0 || i[l] >= n || j[l] < 0 || j[l] >= n || i[l] == j[l])
             return fail(h, FM_ERR_ARG, "bond end out of range");
+        if (!std::isfinite(k[l])) return fail(h, FM_ERR_ARG, "bond stiffness must be finite");
+        if (r0 && !(std::isfinite(r0[l]) || (std::isinf(r0[l]) && r0[l] < 0)))
+            return fail(h, FM_ERR_ARG, "bond rest length is NaN or +inf");
+    }
+    for (int32_t l = 0; l < nb; ++l) {
         float len = r0 ? r0[l] : -1.0f;
         if (len < 0) {
             // Bonds::createBond (femmd2D/src/bonds.cpp:17-20): current length
@@ -329,6 +363,30 @@ int fm_get_energies(fm_system *h, fm_energies *e) {
     FM_END(h)
 }
 
+int fm_set_observe_every(fm_system *h, int32_t every) {
+    FM_GUARD(h)
+    if (every < 0 || every > 1) return fail(h, FM_ERR_ARG, "every must be 0 or 1");
+    h->hs.observe_every = every;
+    if (h->eng) return h->eng->set_observe_every(every, h->err);
+    return FM_OK;
+    FM_END(h)
+}
+
+int fm_get_averages(fm_system *h, fm_averages *a) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    if (!a) return fail(h, FM_ERR_ARG, "null output");
+    return h->eng->averages(0, a, h->err);
+    FM_END(h)
+}
+
+int fm_reset_averages(fm_system *h) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->reset_averages(h->err);
+    FM_END(h)
+}
+
 int fm_get_state(fm_system *h, float *x, float *y, float *vx, float *vy) {
     FM_GUARD(h)
     if (!h->eng) {
@@ -374,6 +432,27 @@ int fm_get_forces(fm_system *h, float *fx, float *fy) {
 int fm_num_particles(const fm_system *h, int32_t *n) {
     if (!h || !n) return FM_ERR_ARG;
     *n = (int32_t)h->hs.x.size();
+    return FM_OK;
+}
+
+int fm_num_bonds(const fm_system *h, int32_t *nb) {
+    if (!h || !nb) return FM_ERR_ARG;
+    *nb = (int32_t)h->hs.bi.size();
+    return FM_OK;
+}
+
+int fm_get_bonds(const fm_system *h, int32_t *pairs) {
+    if (!h || !pairs) return FM_ERR_ARG;
+    for (size_t l = 0; l < h->hs.bi.size(); ++l) {
+        pairs[2 * l] = h->hs.bi[l];
+        pairs[2 * l + 1] = h->hs.bj[l];
+    }
+    return FM_OK;
+}
+
+int fm_num_elements(const fm_system *h, int32_t *nt) {
+    if (!h || !nt) return FM_ERR_ARG;
+    *nt = (int32_t)(h->hs.tri.size() / 3);
     return FM_OK;
 }
 
@@ -558,6 +637,12 @@ int fm_batch_get_energies(fm_batch *b, int32_t r, fm_energies *e) {
     if (!b || !e) return FM_ERR_ARG;
     FM_NEED_BENGINE(b);
     return b->eng->energies(r, e, b->err);
+}
+
+int fm_batch_get_averages(fm_batch *b, int32_t r, fm_averages *a) {
+    if (!b || !a) return FM_ERR_ARG;
+    FM_NEED_BENGINE(b);
+    return b->eng->averages(r, a, b->err);
 }
 
 int fm_batch_set_tuning(fm_batch *b, int32_t tile_cells, int32_t block, int32_t use_graph) {

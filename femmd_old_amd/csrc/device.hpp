@@ -49,6 +49,32 @@ struct alignas(16) RepDesc {
                        // kick then loads no inv_m); 0: masses differ
 };
 
+// Per-step observables (femmd2D/src/properties.cpp:15-86, evaluated and
+// accumulated after every step as Integrator::doStep does, integrator.cpp:18).
+// Per replica, kObsStride doubles:
+//   [0, 8)   the last formed step's totals {KE2 = sum m v^2, max v^2, U_lj,
+//            W_lj, U_fea, W_fea, U_bond, W_bond}
+//   [8]      steps accumulated since the last reset
+//   [9, 17)  sums and [17, 25) sums of squares of the averaged quantities
+//            {uKinetic, uPotential, uTotal, pressure, virial, feaVirial,
+//            uBonded, uFea} (Prop::accum, femmd2D/src/properties.h:6-13)
+// A step's totals are formed in two levels with a fixed order (no float
+// atomics): kObsChunks chunks per replica, each the fixed-order sum of one
+// slice of the per-workgroup partials (ke, lj, fea, bond), written by one
+// workgroup of 256 threads; then the chunks in chunk order.
+constexpr int kObsRaw = 8, kObsAvg = 8, kObsStride = 32, kObsChunks = 16;
+struct ObsArgs {
+    const RepDesc *reps;
+    const int32_t *rng;        // 8 per replica: kick, tile, element, bond ranges
+    const double *ke_part;     // 2 per kick span: sum m v^2, max v^2
+    const double *lj_part;     // 2 per tile: U, W
+    const double *fea_part;    // 2 per element span: U, W
+    const double *bond_part;   // 2 per bond span: U, W
+    double *chunk;             // kObsChunks * kObsRaw per replica (level 1)
+    double *obs;               // kObsStride per replica
+    int32_t nrep;
+};
+
 // Displacement maxima of a kick: each workgroup takes an atomic max into one
 // of kDispSlots slots of its replica (spread to limit contention, one slot
 // per 128-byte line); the binning reads the slots, the force pass clears them.
@@ -106,6 +132,8 @@ struct KickArgs {
     const int32_t *fspan;      // gated batches: the rebuilding replicas' spans
     const int32_t *nfs;        // (decided by k_decide), and their count
     int32_t one_rep;           // gated binning of a single replica (decided first)
+    ObsArgs obs;               // per-step observables
+    int32_t obs_l1;            // the binning forms the previous step's level-1 chunks
 };
 
 // Rebuild decision of a batch (k_flags, k_decide): per replica the flag, and
@@ -169,6 +197,8 @@ struct ScanArgs {
     float2 *xys;               // list mode: positions in sorted order
     float *sigs;               // list mode: sigma in sorted order (may be null)
     uint32_t *err;
+    ObsArgs obs;               // per-step observables
+    int32_t obs_l1;            // (cell walk) form the previous step's level-1 chunks
 };
 
 struct ForceArgs {
@@ -230,6 +260,8 @@ struct ForceArgs {
     int32_t *dbg_cnt;
     long long *dbg_s1, *dbg_s2;
     uint32_t *err;
+    ObsArgs obs;               // per-step observables
+    int32_t obs_l2;            // combine the previous step's chunks and accumulate
 };
 
 // Adjoint pass: J(x) w (J = df/dx, the negative Hessian of the potential)

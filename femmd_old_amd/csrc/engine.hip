@@ -17,7 +17,7 @@ namespace fm {
 
 namespace {
 
-constexpr int kGraphSteps = 16;  // step bodies per captured graph
+constexpr int kGraphSteps = 16;  // step bodies per captured graph of a run
 
 float wrap_host(float r, float L) {  // femmd2D/src/def.h:112, FP32
     float q = r / L;
@@ -74,11 +74,8 @@ Engine::~Engine() {
 }
 
 int Engine::drop_graph() {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
-    graph_steps_ = 0;
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
     return FM_OK;
 }
 
@@ -109,6 +106,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->block_ = reps[0]->block;
     E->use_graph_ = reps[0]->use_graph;
     E->reorder_every_ = reps[0]->reorder_every;
+    E->obs_every_ = reps[0]->observe_every;
 
     // neighbour-list mode: one skin for the batch, single-type pair tables
     const float skin = reps[0]->skin;
@@ -466,7 +464,8 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(ke_part_, 2 * std::max<size_t>(ks.size(), 1));
     FM_AL(fea_part_, 2 * std::max<size_t>(es.size(), 1));
     FM_AL(bond_part_, 2 * std::max<size_t>(bs.size(), 1));
-    FM_AL(ene_, 8 * R);
+    FM_AL(obs_chunk_, (size_t)R * kObsChunks * kObsRaw);
+    FM_AL(obs_, (size_t)R * kObsStride);
     FM_AL(err_, 1);
     if (E->list_) {
         FM_AL(nbr_, (size_t)E->nbr_cap_ * N / 2);  // uint16 entries
@@ -509,6 +508,9 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     if ((he = hipMemsetAsync(E->fea_part_.p, 0, E->fea_part_.n * sizeof(double), st)) != hipSuccess)
         return E->check(he, "memset", err);
     if ((he = hipMemsetAsync(E->bond_part_.p, 0, E->bond_part_.n * sizeof(double), st)) != hipSuccess)
+        return E->check(he, "memset", err);
+    if ((he = hipMemsetAsync(E->obs_chunk_.p, 0, E->obs_chunk_.n * sizeof(double), st)) != hipSuccess ||
+        (he = hipMemsetAsync(E->obs_.p, 0, E->obs_.n * sizeof(double), st)) != hipSuccess)
         return E->check(he, "memset", err);
 
     rc = E->build_clusters(reps, err);
@@ -642,6 +644,8 @@ int Engine::launch_bin(bool gated) {
     a.cap = cap_;
     a.err = err_.p;
     a.nspans = (int)kspans_h_.size();
+    a.obs = obs_args();
+    a.obs_l1 = (gated && l1_) ? 1 : 0;
     if (gated) {  // list mode: decide and bin only the replicas that rebuild
         a.ctl = ctl_.p;
         a.any = any_.p;
@@ -671,7 +675,15 @@ int Engine::launch_bin(bool gated) {
 }
 
 // kick2 / drift mode of one step body followed by binning, sort and forces
-int Engine::launch_step_body(bool kick2, bool drift) {
+int Engine::launch_step_body(bool kick2, bool drift, bool energy, bool l1, bool l2) {
+    struct Restore {  // the flags of this body, restored on every return
+        Engine *e;
+        bool en, a, b;
+        ~Restore() { e->energy_ = en; e->l1_ = a; e->l2_ = b; }
+    } restore{this, energy_, l1_, l2_};
+    energy_ = energy;
+    l1_ = l1;
+    l2_ = l2;
     hipError_t e;
     KickArgs k{};
     k.reps = reps_.p;
@@ -739,6 +751,8 @@ int Engine::launch_sort(bool gated) {
     s.sorted = sorted_.p;
     s.sorted_typ = types_ ? sorted_typ_.p : nullptr;
     s.err = err_.p;
+    s.obs = obs_args();
+    s.obs_l1 = (!list_ && l1_) ? 1 : 0;  // the cell walk's sort forms level 1
     if (list_) {
         s.spos = spos_.p;
         s.xys = xys_.p;
@@ -849,7 +863,23 @@ ForceArgs Engine::force_args() {
     f.dbg_s1 = dbg_s1_.p;
     f.dbg_s2 = dbg_s2_.p;
     f.err = err_.p;
+    f.obs = obs_args();
+    f.obs_l2 = l2_ ? 1 : 0;
     return f;
+}
+
+ObsArgs Engine::obs_args() {
+    ObsArgs o{};
+    o.reps = reps_.p;
+    o.rng = rng_.p;
+    o.ke_part = ke_part_.p;
+    o.lj_part = lj_part_.p;
+    o.fea_part = fea_part_.p;
+    o.bond_part = bond_part_.p;
+    o.chunk = obs_chunk_.p;
+    o.obs = obs_.p;
+    o.nrep = R_;
+    return o;
 }
 
 // Per-kernel device time of nsteps step bodies (no graph): HIP events on
@@ -868,19 +898,22 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
         if (rc) return rc;
     }
     // the first half-kick of this batch of steps (velocities stay a
-    // consistent KDK state: a final half-kick closes the loop below)
-    energy_ = false;  // as in the timed steps of a run
-    if (launch_step_body(false, true)) { energy_ = true; return check(hipGetLastError(), "step", err); }
+    // consistent KDK state: a final half-kick closes the loop below); the
+    // bodies carry the observables as the timed steps of a run do
+    const bool every = obs_every_ > 0;
+    obs_valid_ = false;
+    if (launch_step_body(false, true, every, false, false)) return check(hipGetLastError(), "step", err);
     for (int s = 0; s < nsteps; ++s) {
         prof_ev_ = &ev[K * s];
-        int bad = launch_step_body(true, true);
+        int bad = launch_step_body(true, true, every || s == nsteps - 1, every, every);
         prof_ev_ = nullptr;
-        if (bad) { energy_ = true; return check(hipGetLastError(), "step", err); }
+        if (bad) return check(hipGetLastError(), "step", err);
     }
-    energy_ = true;
-    if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
+    if (launch_bodies_eager(BodyKind{1, true, false, true, false, false, true}))
+        return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps + 1;
     rc = sync(err);
+    if (!rc) obs_valid_ = true;
     // out[k] = dispatch duration of kernel k (0 where the mode launches no
     // such kernel: its events were never recorded), out[5] = whole body,
     // from the kick's start to the force pass's end
@@ -978,6 +1011,67 @@ int Engine::ensure_order(std::string &err) {
     return FM_OK;
 }
 
+// A run of identical bodies (or the closing half-kick and observables):
+// each distinct kind is captured into a graph the first time it is launched
+// and replayed afterwards, so that steady-state runs of any length launch
+// only graphs.
+int Engine::launch_bodies_eager(const BodyKind &k) {
+    if (k.close) {
+        // femmd2D/src/integrator.cpp:18-19: second half-kick (kinetic-energy
+        // partials) and the step's observables (Properties::compute)
+        if (launch_step_body(true, false, true, false, false)) return -1;
+        return launch_observe(obs_args(), true, stream_) == hipSuccess ? 0 : -1;
+    }
+    for (int q = 0; q < k.count; ++q)
+        if (launch_step_body(k.kick2, k.drift, k.energy, k.l1, k.l2)) return -1;
+    return 0;
+}
+
+hipGraphExec_t Engine::graph_of(const BodyKind &k) {
+    const int key = k.key();
+    for (auto &g : graphs_)
+        if (g.first == key) return g.second;
+    if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const int bad = launch_bodies_eager(k);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(stream_, &g);
+    if (bad || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return nullptr;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return nullptr;
+    graphs_.emplace_back(key, ex);
+    return ex;
+}
+
+int Engine::launch_bodies(const BodyKind &k) {
+    if (!use_graph_ || rec_ || prof_ev_) return launch_bodies_eager(k);
+    hipGraphExec_t ex = graph_of(k);
+    if (!ex) return -1;
+    return hipGraphLaunch(ex, stream_) == hipSuccess ? 0 : -1;
+}
+
+int Engine::prepare_graphs() {
+    if (!use_graph_) return 0;
+    const bool every = obs_every_ > 0;
+    const BodyKind kinds[] = {
+        {1, false, true, every, false, false, false},             // first body
+        {kGraphSteps, true, true, every, every, every, false},    // runs of 16
+        {1, true, true, every, every, every, false},              // single bodies
+        {1, true, true, true, false, false, false},               // last body (per-run mode)
+        {1, false, true, true, false, false, false},              // first = last body
+        {1, true, false, true, false, false, true},               // closing half-kick
+    };
+    for (const BodyKind &k : kinds) {
+        if (every && (&k == &kinds[3] || &k == &kinds[4])) continue;
+        if (!graph_of(k)) return -1;
+    }
+    return 0;
+}
+
 int Engine::step(int nsteps, std::string &err) {
     if (nsteps < 0) { err = "nsteps must be >= 0"; return FM_ERR_ARG; }
     if (nsteps == 0) return FM_OK;
@@ -986,57 +1080,56 @@ int Engine::step(int nsteps, std::string &err) {
     rc = ensure_primed(err);
     if (rc) return rc;
     // femmd2D/src/integrator.cpp:8-29 per step; the second half-kick of
-    // step s is fused with the first half-kick of step s+1.  Pair energies
-    // and virials are accumulated only in the last step (the one whose
-    // energies can be queried).
-    energy_ = nsteps == 1;
-    int bad0 = launch_step_body(false, true);
-    energy_ = false;
-    if (bad0) { energy_ = true; return check(hipGetLastError(), "step", err); }
+    // step s is fused with the first half-kick of step s+1.  Observed every
+    // step (obs_every_): every body forms the pair energies and virials and
+    // the kinetic-energy partials, and accumulates the previous step's
+    // observables; the closing body forms and accumulates the last step's.
+    // Otherwise only the last step's are formed and accumulated.
+    const bool every = obs_every_ > 0;
+    if (graphs_.empty() && prepare_graphs()) {
+        drop_graph();
+        return check(hipErrorUnknown, "graph capture", err);
+    }
     // the internal order is re-sorted between step bodies every
-    // reorder_every_ steps (asynchronously, outside the step graph)
+    // reorder_every_ steps (asynchronously, outside the step graphs)
     auto advance = [&](int done) -> int {
         since_reorder_ += done;
         if (reorder_every_ <= 0 || since_reorder_ < reorder_every_) return 0;
         since_reorder_ = 0;
         return launch_reorder_now();
     };
-    if (advance(1)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
+    auto fail = [&](const char *what) {
+        obs_valid_ = false;
+        const hipError_t e = hipGetLastError();
+        return check(e != hipSuccess ? e : hipErrorUnknown, what, err);
+    };
+    // the first body: half-kick and drift only (the velocities are a full
+    // step's); nothing to accumulate yet
+    if (launch_bodies(BodyKind{1, false, true, every || nsteps == 1, false, false, false}))
+        return fail("step");
+    if (advance(1)) return fail("reorder");
     int rem = nsteps - 1;
-    if (use_graph_ && rem >= kGraphSteps + 1) {
-        if (!graph_exec_) {
-            rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture", err);
-            if (rc) return rc;
-            int bad = 0;
-            for (int s = 0; s < kGraphSteps; ++s) bad |= launch_step_body(true, true);
-            hipGraph_t g = nullptr;
-            hipError_t ce = hipStreamEndCapture(stream_, &g);
-            if (bad || ce != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                energy_ = true;
-                return check(ce != hipSuccess ? ce : hipErrorLaunchFailure, "capture step", err);
-            }
-            graph_ = g;
-            rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate", err);
-            if (rc) { drop_graph(); energy_ = true; return rc; }
-            graph_steps_ = kGraphSteps;
-        }
-        // the last body stays outside the graph (it accumulates energies)
-        while (rem > graph_steps_) {
-            rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch", err);
-            if (rc) { energy_ = true; return rc; }
-            rem -= graph_steps_;
-            if (advance(graph_steps_)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
-        }
+    const BodyKind body16{kGraphSteps, true, true, every, every, every, false};
+    const BodyKind body1{1, true, true, every, every, every, false};
+    // (when not every step is observed, the last body forms the energies)
+    const int tail = every ? 0 : 1;
+    while (rem - tail >= kGraphSteps) {
+        if (launch_bodies(body16)) return fail("step");
+        rem -= kGraphSteps;
+        if (advance(kGraphSteps)) return fail("reorder");
     }
-    for (; rem > 0; --rem) {
-        energy_ = rem == 1;
-        if (launch_step_body(true, true)) { energy_ = true; return check(hipGetLastError(), "step", err); }
-        if (rem > 1 && advance(1)) { energy_ = true; return check(hipGetLastError(), "reorder", err); }
+    for (; rem > tail; --rem) {
+        if (launch_bodies(body1)) return fail("step");
+        if (advance(1)) return fail("reorder");
     }
-    energy_ = true;
-    if (launch_step_body(true, false)) return check(hipGetLastError(), "final kick", err);
+    if (rem > 0) {  // the last body forms the energies
+        if (launch_bodies(BodyKind{1, true, true, true, false, false, false})) return fail("step");
+        if (advance(1)) return fail("reorder");
+    }
+    if (launch_bodies(BodyKind{1, true, false, true, false, false, true})) return fail("final kick");
     steps_ += nsteps;
+    ene_valid_ = true;  // the last force pass formed the energies
+    obs_valid_ = true;
     return sync(err);
 }
 
@@ -1068,8 +1161,22 @@ int Engine::compute_forces(std::string &err) {
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
     if (rc) return rc;
     primed_ = false;
-    rc = ensure_primed(err);
+    obs_valid_ = false;
+    return form_observables(err);
+}
+
+// Forces, energies and virials at the current positions (the same bits as
+// a step's force pass: same lists, or the same deterministic sort), the
+// kinetic-energy partials of the current velocities, and the observables of
+// this state (not accumulated).
+int Engine::form_observables(std::string &err) {
+    if (obs_valid_) return FM_OK;
+    int rc = ensure_primed(err);
     if (rc) return rc;
+    if (!ene_valid_) {
+        rc = energy_pass(err);
+        if (rc) return rc;
+    }
     KickArgs k{};
     k.spans = kspans_.p;
     k.vx = vx_.p;
@@ -1078,30 +1185,27 @@ int Engine::compute_forces(std::string &err) {
     k.ke_part = ke_part_.p;
     if (launch_kick(4, false, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
         return check(hipGetLastError(), "ke", err);
-    return sync(err);
+    if (launch_observe(obs_args(), false, stream_) != hipSuccess)
+        return check(hipGetLastError(), "observe", err);
+    rc = sync(err);
+    if (rc) return rc;
+    obs_valid_ = true;
+    return FM_OK;
 }
 
 int Engine::energies(int r, fm_energies *out, std::string &err) {
     if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
     if (rc) return rc;
-    if (!primed_) {
-        rc = compute_forces(err);
-        if (rc) return rc;
-    } else if (!ene_valid_) {
-        rc = energy_pass(err);
-        if (rc) return rc;
-    }
-    if (launch_energy(R_, rng_.p, ke_part_.p, lj_part_.p, fea_part_.p, bond_part_.p, ene_.p,
-                      stream_) != hipSuccess)
-        return check(hipGetLastError(), "energy", err);
-    ene_h_.resize(8 * R_);
-    rc = check(hipMemcpyAsync(ene_h_.data(), ene_.p, 8 * R_ * sizeof(double), hipMemcpyDeviceToHost,
-                              stream_), "energy copy", err);
+    rc = form_observables(err);
+    if (rc) return rc;
+    obs_h_.resize((size_t)kObsStride * R_);
+    rc = check(hipMemcpyAsync(obs_h_.data(), obs_.p, obs_h_.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, stream_), "energy copy", err);
     if (rc) return rc;
     rc = sync(err);
     if (rc) return rc;
-    const double *v = &ene_h_[8 * r];
+    const double *v = &obs_h_[(size_t)kObsStride * r];
     const RepDesc &D = reps_h_[r];
     const double n = D.p1 - D.p0, V = (double)D.Lx * D.Ly;
     std::memset(out, 0, sizeof(*out));
@@ -1118,6 +1222,50 @@ int Engine::energies(int r, fm_energies *out, std::string &err) {
     out->pressure = 0.5 * v[0] / V + 0.5 * (out->w_lj + out->w_fea) / V;
     out->steps = steps_;
     out->n = (int64_t)n;
+    return FM_OK;
+}
+
+// Running averages since the last reset (Prop::avg, femmd2D/src/properties.h:11).
+int Engine::averages(int r, fm_averages *out, std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    obs_h_.resize((size_t)kObsStride * R_);
+    rc = check(hipMemcpyAsync(obs_h_.data(), obs_.p, obs_h_.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, stream_), "averages copy", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    const double *v = &obs_h_[(size_t)kObsStride * r];
+    std::memset(out, 0, sizeof(*out));
+    const double cnt = v[kObsRaw];
+    out->count = (int64_t)cnt;
+    for (int k = 0; k < kObsAvg; ++k) {
+        const double s = v[kObsRaw + 1 + k], s2 = v[kObsRaw + 1 + kObsAvg + k];
+        const double m = cnt > 0 ? s / cnt : 0.0;
+        out->mean[k] = m;
+        out->stdv[k] = cnt > 0 ? std::sqrt(std::max(s2 / cnt - m * m, 0.0)) : 0.0;
+    }
+    return FM_OK;
+}
+
+int Engine::reset_averages(std::string &err) {
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    // keep each replica's last totals, zero its count and sums
+    for (int r = 0; r < R_; ++r) {
+        rc = check(hipMemsetAsync(obs_.p + (size_t)kObsStride * r + kObsRaw, 0,
+                                  (kObsStride - kObsRaw) * sizeof(double), stream_),
+                   "reset", err);
+        if (rc) return rc;
+    }
+    return sync(err);
+}
+
+int Engine::set_observe_every(int every, std::string &err) {
+    if (every < 0 || every > 1) { err = "observe_every must be 0 or 1"; return FM_ERR_ARG; }
+    if (every != obs_every_) drop_graph();  // the bodies of a run change
+    obs_every_ = every;
     return FM_OK;
 }
 
@@ -1173,6 +1321,9 @@ int Engine::set_state(int r, const float *x, const float *y, const float *vx, co
         if (rc) return rc;
     }
     if (x || y) primed_ = false;
+    // the kinetic-energy partials (velocities) and the energies (positions)
+    // of the last formed observables no longer describe the state
+    if (x || y || vx || vy) obs_valid_ = false;
     return FM_OK;
 }
 
@@ -1260,9 +1411,18 @@ int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string
     if ((he = dbg_cnt_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
     if ((he = dbg_s1_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
     if ((he = dbg_s2_.alloc(N_)) != hipSuccess) return check(he, "alloc", err);
-    // rebin (the debug pass must see the binning of the current positions)
-    if (launch_bin()) return check(hipGetLastError(), "bin", err);
-    if (launch_forces_only(true)) return check(hipGetLastError(), "pair sets", err);
+    if (list_) {
+        // the interacting sets the neighbour lists give at the current
+        // positions (lists as last built, possibly steps ago): the list pass
+        // with the dump, which leaves the rebuild state alone
+        const ForceArgs f = force_args();
+        if (launch_forces(sigma_, types_, true, true, true, block_, f, stream_) != hipSuccess)
+            return check(hipGetLastError(), "pair sets", err);
+    } else {
+        // rebin (the debug pass must see the binning of the current positions)
+        if (launch_bin()) return check(hipGetLastError(), "bin", err);
+        if (launch_forces_only(true)) return check(hipGetLastError(), "pair sets", err);
+    }
     std::vector<long long> a(N_), b(N_);
     rc = check(hipMemcpyAsync(cnt, dbg_cnt_.p, N_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
                "copy", err);
@@ -1282,7 +1442,9 @@ int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string
     dbg_cnt_.release();
     dbg_s1_.release();
     dbg_s2_.release();
-    primed_ = false;  // the dump pass overwrote the pair partials
+    // the cell walk's dump re-sorted the particles (list mode: the dump
+    // wrote the forces and energies of the current positions again, same bits)
+    if (!list_) primed_ = false;
     return FM_OK;
 }
 

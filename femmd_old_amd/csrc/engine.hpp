@@ -53,6 +53,8 @@ struct HostSystem {
     int32_t tile_w = 0, block = 256, use_graph = 1;
     // steps between re-orderings of the internal order (0: never)
     int32_t reorder_every = 1000;
+    // observables formed and accumulated every step (1) or per run (0)
+    int32_t observe_every = 1;
 };
 
 template <class T>
@@ -134,6 +136,12 @@ public:
     // velocity rescale to temperature temp (the reference's setTemperature)
     int set_temperature(int r, double temp, int n, const int32_t *ids, std::string &err);
     int tile_width(int r) const { return tile_w_[r]; }
+    // running averages of the per-step observables (Properties::accumulate,
+    // femmd2D/src/properties.cpp:39-86)
+    int averages(int r, fm_averages *out, std::string &err);
+    int reset_averages(std::string &err);
+    int set_observe_every(int every, std::string &err);
+    int observe_every() const { return obs_every_; }
     int block() const { return block_; }
     double density_of(int r) const;
 
@@ -144,12 +152,42 @@ private:
     // every sort writes the sorted-order state; a sort that no list build
     // follows must leave primed_ false.
     int launch_bin(bool gated = false);
-    int launch_step_body(bool kick2, bool drift);
+    // One step body: kick2 (second half-kick of the previous step, with the
+    // kinetic-energy partials when energy is set), drift (first half-kick,
+    // drift, wrap), then the neighbour update and the force pass (pair
+    // energies and virials when energy is set).  l1 / l2: the previous step's
+    // observables are formed (level 1 in the binning or sort, level 2 in the
+    // force pass) and accumulated.
+    int launch_step_body(bool kick2, bool drift, bool energy = false, bool l1 = false,
+                         bool l2 = false);
+    // A run of `count` identical bodies, or the closing half-kick plus the
+    // observables of the last step (close): replayed from a graph captured
+    // on first use (use_graph_), else launched one by one.
+    struct BodyKind {
+        int count;
+        bool kick2, drift, energy, l1, l2, close;
+        int key() const {
+            return count | (kick2 << 8) | (drift << 9) | (energy << 10) | (l1 << 11) | (l2 << 12) |
+                   (close << 13);
+        }
+    };
+    int launch_bodies(const BodyKind &k);
+    int launch_bodies_eager(const BodyKind &k);
+    // the graph of kind k, captured (not launched) if missing; null when
+    // graphs are off or the capture failed
+    hipGraphExec_t graph_of(const BodyKind &k);
+    // captures every kind of body a run launches in the current mode, so
+    // that the first run of any length already replays graphs only
+    int prepare_graphs();
     int launch_forces_only(bool debug);
     int launch_sort(bool gated = false);
     int launch_list_build(bool gated);
     int launch_pair_pass();  // list mode: list pair pass + elements + bonds
     ForceArgs force_args();
+    ObsArgs obs_args();
+    // the observables of the current state (no accumulation), when the last
+    // formed step is not the current state
+    int form_observables(std::string &err);
     // positions the binning / force kernels read: the live state, or a
     // stored configuration during the adjoint sweep
     float *xcur() const { return xalt_ ? xalt_ : x_.p; }
@@ -187,6 +225,11 @@ private:
     // ene_valid_: the partials belong to the current positions
     bool energy_ = true, ene_valid_ = false;
     int energy_pass(std::string &err);
+    // per-step observables: formed and accumulated every step (1, the
+    // reference's doStep) or only for the last step of each run (0)
+    int obs_every_ = 1;
+    bool obs_valid_ = false;  // obs_ holds the totals of the current state
+    bool l1_ = false, l2_ = false;  // flags of the body being launched
     std::vector<char> lj_on_;
     int64_t steps_ = 0;
     int reorder_every_ = 1000;
@@ -203,7 +246,6 @@ private:
     std::vector<Span> kspans_h_, espans_h_, bspans_h_;
     std::vector<Tile> tiles_h_;
     std::vector<int32_t> rng_h_;  // per replica 8 ints (energy ranges)
-    std::vector<double> ene_h_;
 
     DBuf<RepDesc> reps_;
     DBuf<PairEntry> ptab_;
@@ -234,7 +276,9 @@ private:
     DBuf<float4> dmbm_;
     DBuf<float2> alm_, cf_;
     DBuf<float> bk_, br0_;
-    DBuf<double> ke_part_, lj_part_, fea_part_, bond_part_, ene_;
+    DBuf<double> ke_part_, lj_part_, fea_part_, bond_part_;
+    DBuf<double> obs_chunk_, obs_;  // observables: level-1 chunks, per-replica records
+    std::vector<double> obs_h_;
     DBuf<int32_t> rng_;
     DBuf<uint32_t> err_;
     DBuf<float> outx_, outy_;
@@ -248,9 +292,8 @@ private:
     void time_next(int k) {
         if (prof_ev_) arm_kernel_timer(prof_ev_[2 * k], prof_ev_[2 * k + 1]);
     }
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
-    int graph_steps_ = 0;
+    // captured runs of bodies, keyed by BodyKind::key()
+    std::vector<std::pair<int, hipGraphExec_t>> graphs_;
 
     // adjoint state
     float *xalt_ = nullptr, *yalt_ = nullptr;
@@ -285,8 +328,7 @@ size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
                               const int32_t *goff, const int32_t *gent, float *ox, float *oy,
                               hipStream_t s);
-hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke, const double *lj,
-                         const double *fea, const double *bond, double *out, hipStream_t s);
+hipError_t launch_observe(const ObsArgs &o, bool accumulate, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
 hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s);
 hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s);

@@ -120,6 +120,104 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-step observables (femmd2D/src/properties.cpp:15-86).  Level 1: chunk c
+// of replica r is the fixed-order sum of slice c of each of its partial
+// ranges (kick spans, tiles, element spans, bond spans), formed by one
+// workgroup of exactly 256 threads (the binning, sort and observe kernels all
+// run 256), so its bits do not depend on which kernel forms it.  Level 2:
+// one thread sums the replica's chunks in chunk order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void obs_slice(int b, int e, int c, int &lo, int &hi) {
+    const long long len = e - b;
+    lo = b + (int)(len * c / kObsChunks);
+    hi = b + (int)(len * (c + 1) / kObsChunks);
+}
+
+// Level 1 for chunk c of replica r: out[0, 8) of the chunk (see kObsRaw).
+// Called by every thread of a 256-thread workgroup (uniformly).
+__device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
+    const int32_t *g = o.rng + 8 * r;
+    const int tid = threadIdx.x;
+    double v[kObsRaw] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int lo, hi;
+    obs_slice(g[0], g[1], c, lo, hi);
+    for (int k = lo + tid; k < hi; k += 256) {
+        v[0] += o.ke_part[2 * k];
+        v[1] = fmax(v[1], o.ke_part[2 * k + 1]);
+    }
+    obs_slice(g[2], g[3], c, lo, hi);
+    for (int k = lo + tid; k < hi; k += 256) {
+        v[2] += o.lj_part[2 * k];
+        v[3] += o.lj_part[2 * k + 1];
+    }
+    obs_slice(g[4], g[5], c, lo, hi);
+    for (int k = lo + tid; k < hi; k += 256) {
+        v[4] += o.fea_part[2 * k];
+        v[5] += o.fea_part[2 * k + 1];
+    }
+    obs_slice(g[6], g[7], c, lo, hi);
+    for (int k = lo + tid; k < hi; k += 256) {
+        v[6] += o.bond_part[2 * k];
+        v[7] += o.bond_part[2 * k + 1];
+    }
+    // max v^2: order-independent maximum over the workgroup
+    double m = v[1];
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s));
+    double *out = o.chunk + ((size_t)r * kObsChunks + c) * kObsRaw;
+    double z = 0.0;
+    block_sum2(v[0], z, red);
+    if (tid == 0) out[0] = v[0];
+    block_sum2(v[2], v[3], red);
+    if (tid == 0) { out[2] = v[2]; out[3] = v[3]; }
+    block_sum2(v[4], v[5], red);
+    if (tid == 0) { out[4] = v[4]; out[5] = v[5]; }
+    block_sum2(v[6], v[7], red);
+    if (tid == 0) { out[6] = v[6]; out[7] = v[7]; }
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < 4; ++k) m = fmax(m, red[k]);
+        out[1] = m;
+    }
+    __syncthreads();  // red is free again
+}
+
+// The level-1 jobs (replica, chunk) of one step, spread over a kernel's
+// workgroups (256 threads each; uniform per workgroup).
+__device__ void obs_level1(const ObsArgs &o, double *red) {
+    const int njob = o.nrep * kObsChunks;
+    for (int q = blockIdx.x; q < njob; q += gridDim.x) obs_chunk(o, q / kObsChunks, q % kObsChunks, red);
+}
+
+// Level 2 for replica r (one thread): the step's totals from its chunks in
+// chunk order, the averaged quantities of femmd2D/src/properties.cpp:15-36
+// and, when accumulate is set, Prop::accum of each (properties.h:10).
+__device__ void obs_finish(const ObsArgs &o, int r, bool accumulate) {
+    const double *ch = o.chunk + (size_t)r * kObsChunks * kObsRaw;
+    double t[kObsRaw] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < kObsChunks; ++c) {
+        const double *q = ch + c * kObsRaw;
+        for (int k = 0; k < kObsRaw; ++k) t[k] = (k == 1) ? fmax(t[1], q[1]) : t[k] + q[k];
+    }
+    double *ob = o.obs + (size_t)r * kObsStride;
+    for (int k = 0; k < kObsRaw; ++k) ob[k] = t[k];
+    if (!accumulate) return;
+    const RepDesc &R = o.reps[r];
+    const double n = (double)(R.p1 - R.p0), V = (double)R.Lx * (double)R.Ly;
+    const double uk = 0.5 * t[0] / n;
+    const double up = (t[2] + t[4] + t[6]) / n;
+    const double a[kObsAvg] = {uk, up, uk + up, 0.5 * t[0] / V + 0.5 * (t[3] + t[5]) / V,
+                               t[3], t[5], t[6] / n, t[4] / n};
+    ob[kObsRaw] += 1.0;
+    for (int k = 0; k < kObsAvg; ++k) {
+        ob[kObsRaw + 1 + k] += a[k];
+        ob[kObsRaw + 1 + kObsAvg + k] += a[k] * a[k];
+    }
+}
+
 // Bin one particle per lane into its cell's slot list (the reference's
 // atomic cell fill, femmd2D/src/nnCuda.cu:69-82, re-designed): runs of
 // equal cells among consecutive lanes -- the common case, since particles
@@ -430,6 +528,9 @@ __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
 }
 
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
+  __shared__ double ored[8];
+  // the previous step's observables, level 1 (before any early exit)
+  if (a.obs_l1) obs_level1(a.obs, ored);
   int last_rep = -1, flag = 1;
   if (a.ctl && !a.fspan && a.one_rep) {
       // a single replica: the decision comes first, from the replica's
@@ -531,6 +632,10 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ int s_carry, s_head;
     __shared__ int s_ids[kSortIds];       // the block's slot entries, cell order
     __shared__ uint8_t s_cl[kSortIds];    // their cell (local index)
+    __shared__ double s_ored[8];
+    // (cell walk) the previous step's observables, level 1, before the
+    // ticket: no workgroup waits on one that has not taken its ticket
+    if (a.obs_l1) obs_level1(a.obs, s_ored);
     if (a.any && !*a.any) return;  // list mode, no replica rebuilds (uniform)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // listed (gated batches): workgroups beyond the listed blocks leave at
@@ -1356,9 +1461,10 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // neighbours from LDS (the pair maths and accumulation of pair_tile_body;
 // the FP32 cutoff predicate on the current positions, so the interacting
 // set is the cell walk's whenever the list is complete).
-template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit>
+template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit, bool kDebug>
 __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W, int tile,
-                                    const PairLds &L, const float2 *lp, const float *ls) {
+                                    const PairLds &L, const float2 *lp, const float *ls,
+                                    const int *lid) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -1384,10 +1490,11 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
         const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the sentinel slot
         float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
         int nint = 0;
+        long long d1 = 0, d2 = 0;  // diagnostic dump: sums of (id + 1), (id + 1)^2
         // one list entry, without branches: the force of a non-interacting
         // entry (or of the padding entry) is computed and discarded by a
         // select
-        auto term = [&](float2 pj, float sj) {
+        auto term = [&](float2 pj, float sj, int jo) {
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
@@ -1430,10 +1537,15 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             const float fc = hit ? tv * inv : 0.0f;
             fx = __builtin_fmaf(fc, dxa, fx);
             fy = __builtin_fmaf(fc, dya, fy);
+            if (kEnergy || kDebug) nint += hit ? 1 : 0;
             if (kEnergy) {
                 accA += hit ? s12 : 0.0f;
                 accB += hit ? sr6 : 0.0f;
-                nint += hit ? 1 : 0;
+            }
+            if (kDebug && hit) {
+                const long long v = (long long)lid[jo >> 3] + 1;
+                d1 += v;
+                d2 += v * v;
             }
         };
         auto group = [&](uint64_t cur) {
@@ -1452,10 +1564,10 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
                 s2 = ls[j2 >> 3];
                 s3 = ls[j3 >> 3];
             }
-            term(q0, s0);
-            term(q1, s1);
-            term(q2, s2);
-            term(q3, s3);
+            term(q0, s0, j0);
+            term(q1, s1, j1);
+            term(q2, s2, j2);
+            term(q3, s3, j3);
         };
         // the next group is loaded while this one is evaluated (the group
         // index is clamped inside the array; a group past the list is loaded
@@ -1476,6 +1588,12 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             Ublk += 0.5 * ((double)pe.e4 * (A - B) + (double)nint * pe.shift);
             Wblk += 0.5 * ((double)pe.e48 * (A - 0.5 * B));
         }
+        if (kDebug) {
+            const int u = a.uid[id];
+            a.dbg_cnt[u] = nint;
+            a.dbg_s1[u] = d1;
+            a.dbg_s2[u] = d2;
+        }
     }
     if (kEnergy) {
         block_sum2(Ublk, Wblk, L.red);
@@ -1486,9 +1604,9 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
     }
 }
 
-template <bool kSigma, bool kEnergy>
+template <bool kSigma, bool kEnergy, bool kDebug>
 __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 *lp,
-                               float *ls) {
+                               float *ls, int *lid) {
     const Tile T = a.tiles[tile];
     const RepDesc R = a.reps[T.rep];
     const int W = T.w_flags & 0xffff;
@@ -1506,25 +1624,30 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
         T, R, W, L,
         [&](int p) {
             const float2 q = a.xys[p];
-            return make_float3(q.x, q.y, kSigma ? a.sigs[p] : 0.0f);
+            return make_float4(q.x, q.y, kSigma ? a.sigs[p] : 0.0f,
+                               kDebug ? __int_as_float(a.uid[a.perm[p]]) : 0.0f);
         },
-        [&](int e, const float3 &u) {
+        [&](int e, const float4 &u) {
             lp[e] = make_float2(u.x, u.y);
             if (kSigma) ls[e] = u.z;
+            if (kDebug) lid[e] = __float_as_int(u.w);  // caller id (diagnostic dump)
         });
     if (threadIdx.x == 0) {  // the padding entries' slot, far from any particle
         lp[a.lds_cap] = make_float2(kSentinel, kSentinel);
         if (kSigma) ls[a.lds_cap] = 1.0f;
+        if (kDebug) lid[a.lds_cap] = -1;
     }
     __syncthreads();
     const bool wrapT = (T.w_flags >> 16) & 1;
+#define FM_LPB(WR, U) list_tile_pass_body<WR, kSigma, kEnergy, U, kDebug>(a, R, W, tile, L, lp, ls, lid)
     if (!kSigma && R.s2_uniform == 1.0f) {
-        if (wrapT) list_tile_pass_body<true, kSigma, kEnergy, true>(a, R, W, tile, L, lp, ls);
-        else list_tile_pass_body<false, kSigma, kEnergy, true>(a, R, W, tile, L, lp, ls);
+        if (wrapT) FM_LPB(true, true);
+        else FM_LPB(false, true);
     } else {
-        if (wrapT) list_tile_pass_body<true, kSigma, kEnergy, false>(a, R, W, tile, L, lp, ls);
-        else list_tile_pass_body<false, kSigma, kEnergy, false>(a, R, W, tile, L, lp, ls);
+        if (wrapT) FM_LPB(true, false);
+        else FM_LPB(false, false);
     }
+#undef FM_LPB
 }
 
 // CST element forces (femmd2D/src/interactions.cpp:532-689), re-derived in
@@ -1667,10 +1790,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
     __shared__ double s_red[2 * 4];
     int b = blockIdx.x;
+    // the previous step's observables, level 2 (one thread per replica)
+    if (a.obs_l2 && threadIdx.x == 0)
+        for (int r = blockIdx.x; r < a.obs.nrep; r += gridDim.x) obs_finish(a.obs, r, true);
     // the rebuild of this step (if any) is complete: clear its summary,
     // fold this step into each replica's displacement bound (zero after a
     // rebuild) and clear the displacement slots for the next kick
-    if (kList) {
+    // (the diagnostic dump pass leaves the rebuild state alone)
+    if (kList && !kDebug) {
         if (b == 0 && a.any && threadIdx.x == 0) *a.any = 0;
         // replica r by workgroup r (the first wavefront): bound, then slots
         if (threadIdx.x < 64) {
@@ -1707,7 +1834,8 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             L.red = s_red;
             float2 *lp = reinterpret_cast<float2 *>(dyn);
             float *ls = reinterpret_cast<float *>(lp + a.lds_cap + 1);
-            list_tile_pass<kSigma, kEnergy>(a, xcd_remap(b, a.ntiles), L, lp, ls);
+            int *lid = reinterpret_cast<int *>(ls + (kSigma ? a.lds_cap + 1 : 0));
+            list_tile_pass<kSigma, kEnergy, kDebug>(a, xcd_remap(b, a.ntiles), L, lp, ls, lid);
         }
     } else if (b < a.ntiles) {
         PairLds L;
@@ -1751,55 +1879,22 @@ __global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
     oy[p] = y;
 }
 
-// Per-replica energy totals in a fixed order: one workgroup per replica.
-// out[r * 8 + ...] = {KE2 (sum m v^2), vvmax, U_lj, W_lj, U_fea, W_fea,
-//                     U_bond, W_bond}
-struct EnergyArgs {
-    const int32_t *rng;  // per replica 8 ints: kick, tile, elem, bond ranges
-    const double *ke_part, *lj_part, *fea_part, *bond_part;
-    double *out;
-};
-__global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
+// Observables of the current step for every replica (workgroup r): both
+// levels in one workgroup, with the same chunks and order as the per-step
+// path.  accumulate: add them to the running sums.
+__global__ __launch_bounds__(256) void k_observe(ObsArgs o, int accumulate) {
     __shared__ double red[8];
-    const int r = blockIdx.x;
-    const int32_t *g = a.rng + 8 * r;
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double vmax = 0.0;
-    for (int k = g[0] + (int)threadIdx.x; k < g[1]; k += blockDim.x) {
-        v[0] += a.ke_part[2 * k];
-        vmax = fmax(vmax, a.ke_part[2 * k + 1]);
+    for (int r = blockIdx.x; r < o.nrep; r += gridDim.x) {
+        for (int c = 0; c < kObsChunks; ++c) obs_chunk(o, r, c, red);
+        if (threadIdx.x == 0) obs_finish(o, r, accumulate != 0);
     }
-    for (int k = g[2] + (int)threadIdx.x; k < g[3]; k += blockDim.x) {
-        v[2] += a.lj_part[2 * k];
-        v[3] += a.lj_part[2 * k + 1];
-    }
-    for (int k = g[4] + (int)threadIdx.x; k < g[5]; k += blockDim.x) {
-        v[4] += a.fea_part[2 * k];
-        v[5] += a.fea_part[2 * k + 1];
-    }
-    for (int k = g[6] + (int)threadIdx.x; k < g[7]; k += blockDim.x) {
-        v[6] += a.bond_part[2 * k];
-        v[7] += a.bond_part[2 * k + 1];
-    }
-    double z = 0.0;
-    block_sum2(v[0], z, red);
-    if (threadIdx.x == 0) a.out[8 * r] = v[0];
-    block_sum2(v[2], v[3], red);
-    if (threadIdx.x == 0) { a.out[8 * r + 2] = v[2]; a.out[8 * r + 3] = v[3]; }
-    block_sum2(v[4], v[5], red);
-    if (threadIdx.x == 0) { a.out[8 * r + 4] = v[4]; a.out[8 * r + 5] = v[5]; }
-    block_sum2(v[6], v[7], red);
-    if (threadIdx.x == 0) { a.out[8 * r + 6] = v[6]; a.out[8 * r + 7] = v[7]; }
-    // max over the block
-    for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
-        a.out[8 * r + 1] = m;
-    }
+}
+
+hipError_t launch_observe(const ObsArgs &o, bool accumulate, hipStream_t s) {
+    if (o.nrep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_observe, dim3(std::min(o.nrep, 4096)), dim3(256), 0, s, o,
+                       accumulate ? 1 : 0);
+    return hipGetLastError();
 }
 
 // Start a new list epoch (after a forced rebuild), which is counted.
@@ -2727,14 +2822,17 @@ size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
 hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
                          const ForceArgs &a, hipStream_t s) {
     if (list) {
-        // neighbour lists: single type, no dump (the caller checks)
+        // neighbour lists: single type (the caller checks); debug: the
+        // interacting-set dump over the lists (with energies)
         const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
         if (nblk <= 0) return hipSuccess;
         dim3 g(nblk), b(block);
-        const size_t lds = (size_t)(a.lds_cap + 1) * (sizeof(float2) + (sigma ? sizeof(float) : 0));
-#define FM_LIST(S, E) FM_GO((k_forces<S, false, false, true, E>), g, b, lds, s, a)
-        if (sigma) { if (energy) FM_LIST(true, true); else FM_LIST(true, false); }
-        else { if (energy) FM_LIST(false, true); else FM_LIST(false, false); }
+        const size_t lds = (size_t)(a.lds_cap + 1) *
+                           (sizeof(float2) + (sigma ? sizeof(float) : 0) + (debug ? sizeof(int) : 0));
+#define FM_LIST(S, D, E) FM_GO((k_forces<S, false, D, true, E>), g, b, lds, s, a)
+        if (debug) { if (sigma) FM_LIST(true, true, true); else FM_LIST(false, true, true); }
+        else if (sigma) { if (energy) FM_LIST(true, false, true); else FM_LIST(true, false, false); }
+        else { if (energy) FM_LIST(false, false, true); else FM_LIST(false, false, false); }
 #undef FM_LIST
         return hipGetLastError();
     }
@@ -2797,14 +2895,6 @@ hipError_t launch_total_force(int n, const float *fx, const float *fy,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_total_force, dim3((n + 255) / 256), dim3(256), 0, s, n,
                        fx, fy, cf, goff, gent, ox, oy);
-    return hipGetLastError();
-}
-
-hipError_t launch_energy(int nrep, const int32_t *rng, const double *ke,
-                         const double *lj, const double *fea,
-                         const double *bond, double *out, hipStream_t s) {
-    EnergyArgs a{rng, ke, lj, fea, bond, out};
-    hipLaunchKernelGGL(k_energy, dim3(nrep), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

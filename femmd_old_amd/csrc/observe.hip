@@ -207,15 +207,9 @@ int Engine::set_temperature(int r, double temp, int n, const int32_t *ids, std::
     const float f = (float)std::sqrt(temp / cur);
     if (launch_vel_scale(sel_.p, m, vx_.p, vy_.p, f, stream_) != hipSuccess)
         return check(hipGetLastError(), "rescale", err);
-    // the kinetic-energy partials of the energies follow the new velocities
-    KickArgs k{};
-    k.spans = kspans_.p;
-    k.vx = vx_.p;
-    k.vy = vy_.p;
-    k.mass = mass_.p;
-    k.ke_part = ke_part_.p;
-    if (launch_kick(4, false, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
-        return check(hipGetLastError(), "ke", err);
+    // the kinetic energy of the last formed observables is stale: the next
+    // query forms them again from the new velocities
+    obs_valid_ = false;
     return sync(err);
 }
 

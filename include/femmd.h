@@ -80,6 +80,28 @@ typedef struct fm_energies {
     int64_t n;
 } fm_energies;
 
+/* Running averages of the per-step observables since the last reset
+ * (Properties::accumulate / Prop::avg, femmd2D/src/properties.cpp:39-86,
+ * femmd2D/src/properties.h:6-13).  Quantity k of mean[] / stdv[]:
+ *   0 uKinetic = KE / N          1 uPotential = (U_lj + U_fea + U_bond) / N
+ *   2 uTotal                     3 pressure (as fm_energies.pressure)
+ *   4 virial = W_lj              5 feaVirial = W_fea
+ *   6 uBonded = U_bond / N       7 uFea = U_fea / N
+ * (temperature = uKinetic in 2D with kB = 1, properties.cpp:21). */
+#define FM_AVG_UKINETIC 0
+#define FM_AVG_UPOTENTIAL 1
+#define FM_AVG_UTOTAL 2
+#define FM_AVG_PRESSURE 3
+#define FM_AVG_VIRIAL 4
+#define FM_AVG_FEAVIRIAL 5
+#define FM_AVG_UBONDED 6
+#define FM_AVG_UFEA 7
+typedef struct fm_averages {
+    int64_t count;  /* steps accumulated                                   */
+    double mean[8]; /* Prop::avgval                                        */
+    double stdv[8]; /* Prop::stdv = sqrt(max(sum2 / n - avg^2, 0))         */
+} fm_averages;
+
 /* Number of HIP devices visible (md_device_init, femmd2D/src/CudaUtil.cu:45-102). */
 int fm_device_count(int32_t *n);
 
@@ -162,12 +184,31 @@ int fm_commit(fm_system *h);
 /* ---- stepping (Integrator::run / doStep / calcForces) ---- */
 int fm_step(fm_system *h, int32_t nsteps);     /* c_run, System.run       */
 int fm_compute_forces(fm_system *h);           /* c_calcForces            */
+/* Energies and observables of the current state: after fm_step, those of
+ * its last step (formed on the device by the step itself). */
 int fm_get_energies(fm_system *h, fm_energies *e);
+/* Per-step observables (Integrator::doStep -> Properties::compute,
+ * femmd2D/src/integrator.cpp:18, femmd2D/src/properties.cpp:88-98): with
+ * every = 1 (the default, the reference's cadence) each step of fm_step
+ * forms its energies, virials, kinetic energy and pressure on the device and
+ * adds them to running sums; with every = 0 only the last step of each
+ * fm_step call is formed and accumulated (the others skip the energy and
+ * virial sums; forces and trajectories are the same bits either way). */
+int fm_set_observe_every(fm_system *h, int32_t every);
+int fm_get_averages(fm_system *h, fm_averages *a);
+/* Properties::accumulate(0): zero the running sums. */
+int fm_reset_averages(fm_system *h);
 int fm_get_state(fm_system *h, float *x, float *y, float *vx, float *vy);
 int fm_set_state(fm_system *h, const float *x, const float *y, const float *vx,
                  const float *vy);
 int fm_get_forces(fm_system *h, float *fx, float *fy);
-int fm_num_particles(const fm_system *h, int32_t *n);
+int fm_num_particles(const fm_system *h, int32_t *n); /* c_getNumberOfParticles */
+/* c_getNumberOfBonds / c_getBondsPtr (femmd2D/src/interface.cpp:42,45): the
+ * bonds as (i, j) pairs of caller ids, 2 * nb ints. */
+int fm_num_bonds(const fm_system *h, int32_t *nb);
+int fm_get_bonds(const fm_system *h, int32_t *pairs);
+/* Number of CST elements (triangles added by fm_add_elastomer). */
+int fm_num_elements(const fm_system *h, int32_t *nt);
 int fm_sync(fm_system *h);
 
 /* Launch tuning: cells per tile row and threads per workgroup of the force
@@ -251,6 +292,7 @@ int fm_batch_step(fm_batch *b, int32_t nsteps);
 int fm_batch_get_state(fm_batch *b, int32_t r, float *x, float *y, float *vx,
                        float *vy);
 int fm_batch_get_energies(fm_batch *b, int32_t r, fm_energies *e);
+int fm_batch_get_averages(fm_batch *b, int32_t r, fm_averages *a);
 int fm_batch_set_tuning(fm_batch *b, int32_t tile_cells, int32_t block,
                         int32_t use_graph);
 /* Autotune tile width and workgroup size per replica-size bucket; results

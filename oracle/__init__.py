@@ -16,6 +16,10 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 OR_U_LJ, OR_W_LJ, OR_U_BOND, OR_W_BOND, OR_U_FEA, OR_W_FEA, OR_NPAIRS, OR_NINV = range(8)
+# sums of the magnitudes of the bond / element virial terms (oracle.h)
+OR_WS_BOND, OR_WS_FEA, OR_NEN = 8, 9, 10
+# or_run history columns beyond the energies
+OR_H_KE, OR_H_VVMAX = OR_NEN, OR_NEN + 1
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int32)
@@ -202,13 +206,13 @@ def pair_sets(osys: OracleSystem, mode: int = 1):
 
 def forces(osys: OracleSystem, pred_f32: bool = True, all_pairs: bool = False,
            scale: bool = False):
-    """FP64 forces (fx, fy), energies[8] and, with scale=True, an (2, n) array:
+    """FP64 forces (fx, fy), energies[OR_NEN] and, with scale=True, an (2, n) array:
     row 0 the per-particle sum of force-term magnitudes, row 1 the sum of
     r |df/dr| over those terms (see oracle.h)."""
     n = osys.n
     fx = np.empty(n)
     fy = np.empty(n)
-    en = np.empty(8)
+    en = np.empty(OR_NEN)
     fs = np.empty(2 * n) if scale else None
     rc = lib().or_forces(C.byref(osys.s), int(pred_f32), int(all_pairs), _dp(fx), _dp(fy), _dp(en),
                          _dp(fs))
@@ -236,7 +240,7 @@ def run(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt, pred_f32=False, hist
     mass = np.asarray(mass, np.float64)
     inv_m = np.where(np.isfinite(mass) & (mass > 0), 1.0 / np.where(mass > 0, mass, 1.0), 0.0)
     m_ke = np.where(np.isfinite(mass), mass, 0.0)
-    hist = np.empty((nsteps, 10)) if history else None
+    hist = np.empty((nsteps, OR_NEN + 2)) if history else None
     rc = lib().or_run(C.byref(osys.s), _dp(x), _dp(y), _dp(vx), _dp(vy), _dp(m_ke), _dp(inv_m),
                       int(nsteps), float(dt), int(pred_f32), _dp(hist))
     assert rc == 0, rc

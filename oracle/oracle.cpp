@@ -246,6 +246,7 @@ void bonds64(const or_system *s, const double *x, const double *y, double *fx,
         }
         en[OR_U_BOND] += 0.5 * s->bk[b] * dr * dr;
         en[OR_W_BOND] += kdr * (dx * dx + dy * dy);
+        en[OR_WS_BOND] += std::fabs(kdr * (dx * dx + dy * dy));
     }
 }
 
@@ -348,6 +349,8 @@ void cst64(const or_system *s, const double *x, const double *y, double *fx,
         en[OR_U_FEA] += psi * vol0;
         en[OR_W_FEA] += f0x * r0x + f0y * r0y + f1x * r1x + f1y * r1y +
                         f2x * r2x + f2y * r2y;
+        en[OR_WS_FEA] += std::fabs(f0x * (r0x - r2x)) + std::fabs(f0y * (r0y - r2y)) +
+                         std::fabs(f1x * (r1x - r2x)) + std::fabs(f1y * (r1y - r2y));
     }
 }
 
@@ -356,7 +359,7 @@ int32_t forces_impl(const or_system *s, int pred_f32, int all_pairs,
     if (!s || s->n < 0 || !(s->Lx > 0) || !(s->Ly > 0)) return -1;
     std::fill(fx, fx + s->n, 0.0);
     std::fill(fy, fy + s->n, 0.0);
-    std::fill(en, en + 8, 0.0);
+    std::fill(en, en + OR_NEN, 0.0);
     if (fs) std::fill(fs, fs + 2 * (size_t)s->n, 0.0);
     // force order of femmd2D/src/interactions.cpp:96-102: bonds, FEA, pairs.
     if (s->nbond > 0) bonds64(s, s->x, s->y, fx, fy, en, fs);
@@ -441,7 +444,7 @@ int32_t or_run(const or_system *s0, double *x, double *y, double *vx,
     s.y = y;
     int n = s.n;
     std::vector<double> fx(n), fy(n);
-    double en[8];
+    double en[OR_NEN];
     int rc = forces_impl(&s, pred_f32, 0, fx.data(), fy.data(), en);
     if (rc) return rc;
     double h = 0.5 * dt;
@@ -465,9 +468,10 @@ int32_t or_run(const or_system *s0, double *x, double *y, double *vx,
             vvmax = std::max(vvmax, vv);
         }
         if (en_hist) {
-            std::memcpy(en_hist + 10 * (size_t)st, en, 8 * sizeof(double));
-            en_hist[10 * (size_t)st + 8] = 0.5 * ke;
-            en_hist[10 * (size_t)st + 9] = vvmax;
+            double *hs = en_hist + (OR_NEN + 2) * (size_t)st;
+            std::memcpy(hs, en, OR_NEN * sizeof(double));
+            hs[OR_NEN] = 0.5 * ke;
+            hs[OR_NEN + 1] = vvmax;
         }
     }
     return 0;

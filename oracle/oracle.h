@@ -64,10 +64,14 @@ typedef struct or_system {
     int32_t model;              /* 0 Neo-Hookean, 1 SVK, 2 legacy 2xSVK  */
 } or_system;
 
-/* energies[] layout shared by every entry point */
+/* energies[OR_NEN] layout shared by every entry point.  OR_WS_BOND and
+ * OR_WS_FEA are the sums of the magnitudes of the virial terms (per bond
+ * |f . r|, per element corner |f_k . (r_k - r_2)|): the scale of the
+ * cancellation in W_bond and W_fea, against which an FP32 sum is judged. */
 enum {
     OR_U_LJ = 0, OR_W_LJ = 1, OR_U_BOND = 2, OR_W_BOND = 3,
-    OR_U_FEA = 4, OR_W_FEA = 5, OR_NPAIRS = 6, OR_NINVERTED = 7
+    OR_U_FEA = 4, OR_W_FEA = 5, OR_NPAIRS = 6, OR_NINVERTED = 7,
+    OR_WS_BOND = 8, OR_WS_FEA = 9, OR_NEN = 10
 };
 
 /* def.h:112 wrap, FP32, in place. */
@@ -105,7 +109,7 @@ int32_t or_forces(const or_system *s, int32_t pred_f32, int32_t all_pairs,
 /* FP64 KDK rollout (integrator.cpp:8-29,100-135) from (x, y, vx, vy).
  * inv_m = 0 marks a clamped node.  The cutoff predicate is FP64 unless
  * pred_f32.  State arrays are updated in place.  If en_hist is not NULL it
- * receives 10 doubles per step: the 8 energies of or_forces plus
+ * receives OR_NEN + 2 doubles per step: the energies of or_forces plus
  * KE = 1/2 sum m v^2 and max v^2 at the end of the step. */
 int32_t or_run(const or_system *s, double *x, double *y, double *vx,
                double *vy, const double *mass, const double *inv_m,

@@ -82,7 +82,7 @@ int main() {
     std::vector<int64_t> s1(n), s2(n);
     for (int mode = 0; mode < 2; ++mode)
         if (or_pair_sets_f32(&s, mode, cnt.data(), s1.data(), s2.data()) != 0) return 1;
-    std::vector<double> fx(n), fy(n), en(8), fs(2 * n);
+    std::vector<double> fx(n), fy(n), en(OR_NEN), fs(2 * n);
     for (int model = 0; model < 3; ++model) {
         s.model = model;
         for (int p32 = 0; p32 < 2; ++p32)
@@ -91,7 +91,7 @@ int main() {
                     return 2;
     }
     s.model = 0;
-    std::vector<double> vx(n), vy(n), mass(n, 1.0), inv_m(n, 1.0), hist(10 * 20);
+    std::vector<double> vx(n), vy(n), mass(n, 1.0), inv_m(n, 1.0), hist((OR_NEN + 2) * 20);
     for (int i = 0; i < n; ++i) {
         vx[i] = 0.3 * (u(rng) - 0.5);
         vy[i] = 0.3 * (u(rng) - 0.5);

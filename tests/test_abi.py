@@ -103,3 +103,52 @@ def test_set_mass_and_tuning_validation():
     s.set_reorder_interval(0)
     assert s.reorder_info() == {"interval": 0, "count": 0}
     s.close()
+
+
+def test_ingestion_is_all_or_nothing():
+    """A rejected fm_add_particles / fm_add_elastomer / fm_add_bonds call
+    leaves the system as it was: every input is validated before anything is
+    appended, an exclusion group is taken or an id is written."""
+    lib = _lib.load()
+    s = System(30.0, 30.0)
+    s.add_particles([1.0, 2.0, 3.0], [1.0, 1.0, 1.0])
+    first = C.c_int32(-7)
+    bad = np.array([4.0, 5.0, np.nan], np.float32)
+    ones = np.ones(3, np.float32)
+    rc = lib.fm_add_particles(s.handle, 3, bad.ctypes.data_as(_lib._f), ones.ctypes.data_as(_lib._f),
+                              None, None, ones.ctypes.data_as(_lib._f),
+                              ones.ctypes.data_as(_lib._f), None, C.byref(first))
+    assert rc == _lib.FM_ERR_ARG and first.value == -7 and s.N == 3
+    # a bad sigma, velocity or type is refused before any particle is appended
+    for kw in ({"sig": [1.0, 0.0]}, {"vx": [0.0, np.inf]}, {"typ": [0, -1]}):
+        with pytest.raises(FemmdError):
+            s.add_particles([4.0, 5.0], [1.0, 1.0], **kw)
+        assert s.N == 3
+    v = np.array([0, 0, 1, 0, 0, 1, 1, 1], np.float32)
+    t = np.array([1, 2, 3, 2, 4, 3], np.int32)
+    # per-element nu outside (-1, 1/2), E negative or NaN: nothing appended
+    for kw in ({"nu": [0.3, 0.6]}, {"E": [10.0, -1.0]}, {"E": [np.nan, 1.0]}):
+        with pytest.raises(FemmdError):
+            s.add_elastomer(v, t, 10.0, 10.0, mass=4.0, **kw)
+        assert s.N == 3
+        assert s.n_elements == 0
+    with pytest.raises(FemmdError):
+        s.add_elastomer(np.array([0, 0, 1, 0, 0, np.inf], np.float32), t[:3], 10.0, 10.0, mass=1.0)
+    assert s.N == 3
+    node0 = s.add_elastomer(v, t, 10.0, 10.0, mass=4.0, E=[20.0, 30.0], nu=[0.3, 0.45])
+    assert node0 == 3 and s.N == 7
+    ne = C.c_int32()
+    assert lib.fm_num_elements(s.handle, C.byref(ne)) == _lib.FM_OK and ne.value == 2
+    # bonds: the second bond is bad, so neither is added
+    with pytest.raises(FemmdError):
+        s.add_bonds([0, 1], [1, 99], [10.0, 10.0])
+    assert s.n_bonds() == 0
+    with pytest.raises(FemmdError):
+        s.add_bonds([0, 1], [1, 2], [10.0, np.nan])
+    assert s.n_bonds() == 0
+    with pytest.raises(FemmdError):
+        s.add_bonds([0, 1], [1, 2], [10.0, 10.0], [1.0, np.nan])
+    assert s.n_bonds() == 0
+    s.add_bonds([0, 1], [1, 2], [10.0, 10.0])
+    np.testing.assert_array_equal(s.bond_pairs(), [[0, 1], [1, 2]])
+    s.close()

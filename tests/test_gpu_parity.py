@@ -23,6 +23,7 @@ import pytest
 
 import oracle as O
 from femmd_old_amd import Batch, FemmdError, System, _lib, configs
+from parity_helpers import check_energies
 
 pytestmark = pytest.mark.gpu
 
@@ -44,16 +45,6 @@ def force_err(s, osys):
     return O.force_error(fx, fy, rx, ry, fs), en
 
 
-def check_energies(s, en, rtol=1e-5):
-    e = s.energies()
-    scale_u = abs(en[O.OR_U_LJ]) + abs(en[O.OR_U_FEA]) + abs(en[O.OR_U_BOND]) + 1.0
-    assert abs(e["u_lj"] - en[O.OR_U_LJ]) <= rtol * scale_u, (e["u_lj"], en[O.OR_U_LJ])
-    assert abs(e["w_lj"] - en[O.OR_W_LJ]) <= rtol * (abs(en[O.OR_W_LJ]) + 1.0)
-    assert abs(e["u_fea"] - en[O.OR_U_FEA]) <= rtol * scale_u
-    assert abs(e["u_bond"] - en[O.OR_U_BOND]) <= rtol * scale_u
-    return e
-
-
 # ---------------------------------------------------------------------------
 def test_cells_and_counting_sort_bit_exact():
     w = configs.c1()
@@ -72,19 +63,49 @@ def test_cells_and_counting_sort_bit_exact():
         x, y, _, _ = s.get_state()
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2s", "c3s"])
-def test_pair_sets_bit_exact(cfg):
+@pytest.mark.parametrize("skin", SKINS)
+@pytest.mark.parametrize("cfg", ["c1", "c2s", "c3s", "c5s"])
+def test_pair_sets_bit_exact(cfg, skin):
+    """The interacting sets the force pass uses, at the current positions:
+    with neighbour lists (skin > 0) the sets the lists give, lists built
+    steps earlier (femmd2D/src/nn.cpp:22-62 walk, interactionsCuda.cu:88-90
+    predicate on the current positions); with skin 0 the per-step cell walk.
+    c5s is WCA (f -> 0 at the cutoff), where a pair missed near the cutoff
+    would not show in the force bar."""
     w = {"c1": lambda: configs.c1(), "c2s": lambda: configs.c2(scale=0.01),
-         "c3s": lambda: configs.c3(scale=0.005)}[cfg]()
-    s, osys, _ = engine_and_oracle(w)
-    for rep in range(2):
+         "c3s": lambda: configs.c3(scale=0.005), "c5s": lambda: configs.c5(scale=0.01)}[cfg]()
+    s, osys, _ = engine_and_oracle(w, skin)
+    for rep in range(4):
         cnt, s1, s2 = s.pair_sets()
         x, y, _, _ = s.get_state()
         ocnt, os1, os2 = O.pair_sets(osys.with_positions(x, y), 1)
         np.testing.assert_array_equal(cnt, ocnt)
         np.testing.assert_array_equal(s1, os1)
         np.testing.assert_array_equal(s2, os2)
-        s.run(40)
+        s.run(7 + 5 * rep)
+    if skin > 0:
+        assert s.list_stats()["rebuilds"] >= 2
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.4])
+def test_pair_sets_from_stale_lists_hot_system(skin):
+    # three times the thermal speed: lists are rebuilt often and used up to
+    # the skin / 2 displacement bound; every step's sets are checked
+    w = configs.c2(scale=0.01)
+    s = configs.build_system(w, commit=False, skin=skin)
+    x, y, vx, vy = s.get_state()
+    s.set_state(None, None, 3.0 * vx, 3.0 * vy)
+    s.allocate_cuda()
+    osys = O.from_workload(w)
+    for step in range(24):
+        cnt, s1, s2 = s.pair_sets()
+        x, y, _, _ = s.get_state()
+        ocnt, os1, os2 = O.pair_sets(osys.with_positions(x, y), 1)
+        np.testing.assert_array_equal(cnt, ocnt)
+        np.testing.assert_array_equal(s1, os1)
+        np.testing.assert_array_equal(s2, os2)
+        s.run(1)
+    assert s.list_stats()["rebuilds"] >= 4
 
 
 @pytest.mark.parametrize("skin", SKINS)
@@ -98,7 +119,7 @@ def test_forces_within_1e5_of_fp64(cfg, skin):
         err, en = force_err(s, osys.with_positions(x, y))
         print(f"[{cfg} skin {skin} step {step * 30}] max rel force err {err.max():.3e}")
         assert err.max() <= FORCE_TOL
-        check_energies(s, en)
+        check_energies(s, en, w)
         s.run(30)
 
 
@@ -122,7 +143,7 @@ def test_material_models(model):
     print(f"[model {model}] max rel force err {err.max():.3e}, U_fea {en[O.OR_U_FEA]:.6f}")
     assert en[O.OR_U_FEA] > 0
     assert err.max() <= FORCE_TOL
-    check_energies(s, en)
+    check_energies(s, en, w)
 
 
 @pytest.mark.parametrize("skin", SKINS)
@@ -143,7 +164,7 @@ def test_trajectory_100_steps_and_energy_drift(skin):
     e1 = s.energies()
     E0 = e0["ke"] + e0["u_lj"] + e0["u_fea"]
     E1 = e1["ke"] + e1["u_lj"] + e1["u_fea"]
-    oE1 = hist[-1, 8] + hist[-1, O.OR_U_LJ] + hist[-1, O.OR_U_FEA]
+    oE1 = hist[-1, O.OR_H_KE] + hist[-1, O.OR_U_LJ] + hist[-1, O.OR_U_FEA]
     print(f"[traj] max |dx| {dpos.max():.3e} rms {np.sqrt((dpos ** 2).mean()):.3e} "
           f"max |dv| {dvel.max():.3e}; E drift gpu {(E1 - E0) / abs(E0):.3e} "
           f"oracle {(oE1 - E0) / abs(E0):.3e}")

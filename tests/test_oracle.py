@@ -267,6 +267,20 @@ def test_bond_rest_length_and_fd():
         assert -(up - um) / (2 * h) == pytest.approx(fx[i], rel=1e-6, abs=1e-7)
 
 
+@pytest.mark.parametrize("r,r0,k", [(1.3, 1.0, 100.0), (0.8, 1.0, 30.0), (2.0, 2.0, 5.0)])
+def test_bond_virial_closed_form(r, r0, k):
+    # W_bond = f_ij . r_ij = -k (r - r0) / r * r^2 = -k (r - r0) r, whatever
+    # the bond's direction and image
+    L = 20.0
+    for th, x0 in ((0.3, 5.0), (2.1, 19.6)):
+        x = np.array([x0, (x0 + r * math.cos(th)) % L])
+        y = np.array([7.0, 7.0 + r * math.sin(th)])
+        s = O.OracleSystem(L, L, x, y, lj_on=False, bonds=([0], [1], [k], [r0]))
+        _, _, en = O.forces(s, pred_f32=False)
+        assert en[O.OR_W_BOND] == pytest.approx(-k * (r - r0) * r, rel=1e-9, abs=1e-12)
+        assert en[O.OR_U_BOND] == pytest.approx(0.5 * k * (r - r0) ** 2, rel=1e-9, abs=1e-12)
+
+
 # ---------------------------------------------------------------------------
 # CST (femmd2D/src/interactions.cpp:532-689)
 # ---------------------------------------------------------------------------
@@ -345,6 +359,54 @@ def test_cst_rotation_invariance_and_small_strain_limit():
     eps_ = 0.5 * (F + F.T) - np.eye(2)
     psi_lin = mu * (eps_ * eps_).sum() + 0.5 * lam * np.trace(eps_) ** 2
     assert en[O.OR_U_FEA] / A == pytest.approx(psi_lin, rel=5e-3)
+
+
+def p_dot_f(F, model, lam, mu):
+    """P:F of the material law, in closed form from the invariants: Neo-Hookean
+    P = mu (F - F^-T) + lam lnJ F^-T gives mu (I1 - 2) + 2 lam lnJ; SVK
+    P = F S with S = 2 mu E + lam tr(E) I, E = (C - I)/2 gives S:C; the
+    legacy GPU energy is twice SVK."""
+    F = np.asarray(F, float)
+    if model == 0:
+        J = np.linalg.det(F)
+        return mu * ((F * F).sum() - 2.0) + 2.0 * lam * math.log(J)
+    C_ = F.T @ F
+    E_ = 0.5 * (C_ - np.eye(2))
+    S_ = 2 * mu * E_ + lam * np.trace(E_) * np.eye(2)
+    return (2.0 if model == 2 else 1.0) * (S_ * C_).sum()
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+@pytest.mark.parametrize("F", [np.diag([1.2, 1.0]), np.array([[1.08, 0.05], [-0.03, 0.95]]),
+                               np.array([[0.9, 0.2], [0.1, 1.1]])])
+def test_cst_virial_closed_form(model, F):
+    # sum_k f_k . x_k = tr(H^T Ds) = -A0 tr(Bm P^T F Dm)... = -A0 P:F for a
+    # homogeneous deformation (the reference sums f . unfoldedPos,
+    # femmd2D/src/interactions.cpp:681; sum_k f_k = 0 makes it independent of
+    # the origin)
+    E, nu = 50.0, 0.3
+    lam, mu = lame(E, nu)
+    _, _, _, _, en, A = one_element(F, model, E, nu)
+    want = -A * p_dot_f(F, model, lam, mu)
+    assert en[O.OR_W_FEA] == pytest.approx(want, rel=1e-9, abs=1e-12)
+
+
+def test_cst_virial_independent_of_position_and_image():
+    F = np.array([[1.05, 0.02], [0.01, 0.97]])
+    L = 100.0
+    ref = np.array([[0.0, 0.0], [1.0, 0.0], [0.3, 0.9]])
+    cur = ref @ F.T
+    w = []
+    for shift in (40.0, -0.5, 99.6):
+        c = cur + np.array([shift, 40.0])
+        c[:, 0] %= L
+        s = O.OracleSystem(L, L, c[:, 0], c[:, 1], lj_on=False, tri=[0, 1, 2], refx=ref[:, 0],
+                           refy=ref[:, 1], E=[50.0], nu=[0.3], model=0)
+        w.append(O.forces(s, pred_f32=False)[2][O.OR_W_FEA])
+    lam, mu = lame(50.0, 0.3)
+    A = 0.45
+    for v in w:
+        assert v == pytest.approx(-A * p_dot_f(F, 0, lam, mu), rel=1e-9)
 
 
 def test_cst_orientation_uses_signed_area():
@@ -428,7 +490,7 @@ def test_kdk_energy_conservation_c1():
     vx, vy = configs.node_velocities(w)
     _, _, _, _, hist = O.run(s, s.keep["x"], s.keep["y"], vx, vy, ga["mass"], 200, 0.005,
                              history=True)
-    etot = hist[:, O.OR_U_LJ] + hist[:, O.OR_U_FEA] + hist[:, 8]
+    etot = hist[:, O.OR_U_LJ] + hist[:, O.OR_U_FEA] + hist[:, O.OR_H_KE]
     drift = (etot.max() - etot.min()) / abs(etot[0])
     assert drift < 5e-3
 
