@@ -382,6 +382,15 @@ class System:
             a = _i32(ids)
             self._check(self._lib.fm_set_temperature(self._h, float(temp), len(a), _ip(a)))
 
+    def set_temperature_noalloc(self, temp: float, ids=None):
+        """set_temperature without device allocation (fm_set_temperature_noalloc:
+        scratch in a fixed device array; at most 2^20 ids)."""
+        if ids is None:
+            self._check(self._lib.fm_set_temperature_noalloc(self._h, float(temp), 0, None))
+        else:
+            a = _i32(ids)
+            self._check(self._lib.fm_set_temperature_noalloc(self._h, float(temp), len(a), _ip(a)))
+
     def get_state(self):
         n = self.N
         out = [np.empty(n, np.float32) for _ in range(4)]

@@ -110,6 +110,7 @@ PROTOTYPES = {
     "fm_cluster_props": (C.c_int, [_h, _d, _f, _f]),
     "fm_cluster_virial": (C.c_int, [_h, _d]),
     "fm_set_temperature": (C.c_int, [_h, C.c_double, C.c_int32, _i]),
+    "fm_set_temperature_noalloc": (C.c_int, [_h, C.c_double, C.c_int32, _i]),
     "fm_set_mass": (C.c_int, [_h, C.c_int32, _i, _f]),
     "fm_commit": (C.c_int, [_h]),
     "fm_step": (C.c_int, [_h, C.c_int32]),

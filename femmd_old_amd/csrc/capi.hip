@@ -558,6 +558,14 @@ int fm_set_temperature(fm_system *h, double temp, int32_t n, const int32_t *ids)
     FM_END(h)
 }
 
+int fm_set_temperature_noalloc(fm_system *h, double temp, int32_t n, const int32_t *ids) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    if (n < 0 || (n > 0 && !ids)) return fail(h, FM_ERR_ARG, "n >= 0 and ids for n > 0");
+    return h->eng->set_temperature_noalloc(0, temp, n, ids, h->err);
+    FM_END(h)
+}
+
 int fm_force_jvp(fm_system *h, const float *wx, const float *wy, float *jx, float *jy,
                  double *dE, double *deps, double *dsig) {
     FM_GUARD(h)

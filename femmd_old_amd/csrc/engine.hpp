@@ -135,6 +135,9 @@ public:
     int cluster_virial(int r, double *vir, std::string &err);
     // velocity rescale to temperature temp (the reference's setTemperature)
     int set_temperature(int r, double temp, int n, const int32_t *ids, std::string &err);
+    // the same rescale with no device allocation (scratch in a file-scope
+    // device array; see observe.hip)
+    int set_temperature_noalloc(int r, double temp, int n, const int32_t *ids, std::string &err);
     int tile_width(int r) const { return tile_w_[r]; }
     // running averages of the per-step observables (Properties::accumulate,
     // femmd2D/src/properties.cpp:39-86)

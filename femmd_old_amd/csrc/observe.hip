@@ -4,6 +4,7 @@
 // temperature (femmd2D/src/md3dsystem.cpp:92-136).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "engine.hpp"
@@ -209,6 +210,121 @@ int Engine::set_temperature(int r, double temp, int n, const int32_t *ids, std::
         return check(hipGetLastError(), "rescale", err);
     // the kinetic energy of the last formed observables is stale: the next
     // query forms them again from the new velocities
+    obs_valid_ = false;
+    return sync(err);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free velocity rescale.  The same kernels and the same order of
+// sums as set_temperature above, with the selected internal ids, the
+// kinetic-energy partials and their total in one file-scope device array
+// (`scratch`, kNoallocBytes per device, held until the process ends) instead
+// of buffers allocated per call.  Every call on one GPU shares that GPU's
+// array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoallocIds = 1 << 20;    // selected particles per call
+constexpr int kNoallocParts = 1 << 16;  // kinetic-energy partials (256 particles each)
+constexpr size_t kNoallocPartOff = (size_t)kNoallocIds * sizeof(int32_t);
+constexpr size_t kNoallocTotalOff = kNoallocPartOff + (size_t)kNoallocParts * sizeof(double);
+constexpr size_t kNoallocBytes = kNoallocTotalOff + 256;
+constexpr int kNoallocDevices = 64;
+
+// The host-side name of this array is a stand-in, not a device address:
+// host code uses the address hipGetSymbolAddress returns for each device.
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoallocDevices] = {};
+
+// This device's copy of `scratch`, looked up once per device with that
+// device current (the caller has made it current).
+hipError_t scratch_of(int device, unsigned char **out) {
+    if (device < 0 || device >= kNoallocDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[device]);
+    return hipSuccess;
+}
+
+// out = part[0] + part[1] + ... in index order (one thread: the order of the
+// host loop of set_temperature, so the total has the same bits)
+__global__ void k_sum_in_order(const double *part, int n, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += part[k];
+    *out = s;
+}
+}  // namespace
+
+int Engine::set_temperature_noalloc(int r, double temp, int n, const int32_t *ids,
+                                    std::string &err) {
+    if (r < 0 || r >= R_) { err = "replica out of range"; return FM_ERR_ARG; }
+    if (!(temp >= 0.0) || !std::isfinite(temp)) { err = "temperature must be >= 0"; return FM_ERR_ARG; }
+    const RepDesc &D = reps_h_[r];
+    const int np = D.p1 - D.p0;
+    const int m = n > 0 ? n : np;
+    const int nb = (m + 255) / 256;
+    if (n > kNoallocIds) {
+        err = "fm_set_temperature_noalloc: at most 1048576 ids per call";
+        return FM_ERR_ARG;
+    }
+    if (nb > kNoallocParts) {
+        err = "fm_set_temperature_noalloc: at most 16777216 particles per call";
+        return FM_ERR_ARG;
+    }
+    if (n > 0 && !ids) { err = "ids required"; return FM_ERR_ARG; }
+    for (int k = 0; k < n; ++k)
+        if (ids[k] < 0 || ids[k] >= np) { err = "particle id out of range"; return FM_ERR_ARG; }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_order(err);
+    if (rc) return rc;
+    unsigned char *base = nullptr;
+    rc = check(scratch_of(device_, &base), "hipGetSymbolAddress(scratch)", err);
+    if (rc) return rc;
+    int32_t *sel = reinterpret_cast<int32_t *>(base);
+    double *part = reinterpret_cast<double *>(base + kNoallocPartOff);
+    double *total = reinterpret_cast<double *>(base + kNoallocTotalOff);
+    // the selected caller ids as internal ids, staged through a buffer on
+    // the stack; all particles of a single system need no ids (internal id
+    // k, in the order set_temperature lists them)
+    const int32_t *dsel = nullptr;
+    if (n > 0 || D.p0 != 0) {
+        int32_t chunk[2048];
+        for (int k0 = 0; k0 < m; k0 += 2048) {
+            const int c = std::min(2048, m - k0);
+            for (int k = 0; k < c; ++k)
+                chunk[k] = n > 0 ? iid_h_[r][ids[k0 + k]] : D.p0 + k0 + k;
+            rc = check(hipMemcpyAsync(sel + k0, chunk, c * sizeof(int32_t), hipMemcpyHostToDevice,
+                                      stream_), "ids", err);
+            if (rc) return rc;
+            rc = check(hipStreamSynchronize(stream_), "ids", err);  // chunk is reused
+            if (rc) return rc;
+        }
+        dsel = sel;
+    }
+    if (launch_vel_ke(dsel, m, vx_.p, vy_.p, mass_.p, part, stream_) != hipSuccess)
+        return check(hipGetLastError(), "kinetic energy", err);
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, stream_, part, nb, total);
+    rc = check(hipGetLastError(), "kinetic energy sum", err);
+    if (rc) return rc;
+    double ke = 0.0;
+    rc = check(hipMemcpyAsync(&ke, total, sizeof(double), hipMemcpyDeviceToHost, stream_),
+               "kinetic energy", err);
+    if (rc) return rc;
+    rc = sync(err);
+    if (rc) return rc;
+    const double cur = (n > 0) ? 2.0 * ke / (3.0 * np) : ke / np;
+    if (!(cur > 0.0)) { err = "the selected particles are at rest"; return FM_ERR_STATE; }
+    const float f = (float)std::sqrt(temp / cur);
+    if (launch_vel_scale(dsel, m, vx_.p, vy_.p, f, stream_) != hipSuccess)
+        return check(hipGetLastError(), "rescale", err);
     obs_valid_ = false;
     return sync(err);
 }

@@ -259,6 +259,16 @@ int fm_cluster_virial(fm_system *h, double *virial);
  * reference's group formula T = 2 sum(m v^2 / 2) / (3 N), N the whole
  * system's count.  v *= sqrt(temp / T). */
 int fm_set_temperature(fm_system *h, double temp, int32_t n, const int32_t *ids);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (a thermostat inside a tight serving loop, harnesses that forbid
+ * hipMalloc): fm_set_temperature with the same arguments and the same result
+ * bits, calling no hipMalloc / hipFree.  Its scratch (selected ids, kinetic
+ * energy partials) lives in one file-scope device array of 4.5 MiB that each
+ * GPU the library runs on holds until the process ends.  Limits: n <= 2^20
+ * ids; with n = 0, N <= 2^24 particles; larger requests fail with FM_ERR_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several threads or handles) race. */
+int fm_set_temperature_noalloc(fm_system *h, double temp, int32_t n, const int32_t *ids);
 /* Neighbour-list statistics: mean and max list length at the last build,
  * rebuilds so far (all zero when the cells are walked every step). */
 int fm_list_stats(fm_system *h, double *mean_len, int32_t *max_len,
