@@ -63,7 +63,7 @@ int Engine::adj_pass_list(bool fresh, std::string &err) {
         // the positions of this build
         if (launch_bin() || launch_sort() || launch_list_build(false))
             return check(hipGetLastError(), "adjoint lists", err);
-        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, stream_) != hipSuccess)
+        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, 2 * R_ * kDispSlots, stream_) != hipSuccess)
             return check(hipGetLastError(), "ctl", err);
     } else {
         KickArgs k{};
@@ -72,8 +72,8 @@ int Engine::adj_pass_list(bool fresh, std::string &err) {
         k.x = xcur();
         k.y = ycur();
         k.spos = spos_.p;
-        k.xys = xys_.p;
-        k.dmax = dmax_.p;
+        k.xys = xys_cur();  // the positions of the last build
+        k.dmax = dmax_cur();
         if (launch_adj_disp((int)kspans_h_.size(), k, stream_) != hipSuccess)
             return check(hipGetLastError(), "adjoint displacement", err);
         if (launch_bin(true) || launch_sort(true) || launch_list_build(true))

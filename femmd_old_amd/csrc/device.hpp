@@ -47,6 +47,10 @@ struct alignas(16) RepDesc {
                        // to an edge has pairs needing the minimum image)
     float im_uniform;  // 1/m shared by every particle of the replica (the
                        // kick then loads no inv_m); 0: masses differ
+    int32_t pg;        // particles [pg, p1) have element or bond corners (or
+                       // the replica has no pair forces): the node kick
+                       // integrates them, the force pass the others
+    int32_t pad_[3];
 };
 
 // Per-step observables (femmd2D/src/properties.cpp:15-86, evaluated and
@@ -66,7 +70,13 @@ constexpr int kObsRaw = 8, kObsAvg = 8, kObsStride = 32, kObsChunks = 16;
 struct ObsArgs {
     const RepDesc *reps;
     const int32_t *rng;        // 8 per replica: kick, tile, element, bond ranges
+    const int32_t *nrng;       // 2 per replica: node-kick span range
+    int32_t ke_kick;           // kinetic energy from ke_part over the kick spans
+                               // (a query); else from ke_tile over the tiles
+                               // and ke_node over the node-kick spans (steps)
     const double *ke_part;     // 2 per kick span: sum m v^2, max v^2
+    const double *ke_tile;     // 2 per tile: the same, particles kicked by the pair pass
+    const double *ke_node;     // 2 per node-kick span: the same, element / bond particles
     const double *lj_part;     // 2 per tile: U, W
     const double *fea_part;    // 2 per element span: U, W
     const double *bond_part;   // 2 per bond span: U, W
@@ -262,6 +272,23 @@ struct ForceArgs {
     uint32_t *err;
     ObsArgs obs;               // per-step observables
     int32_t obs_l2;            // combine the previous step's chunks and accumulate
+    // The kick of the pass (kick modes of k_forces): each tile integrates its
+    // centre particles that have no element or bond corners (the others are
+    // left to the node kick, which needs their gathered corner forces):
+    // second half-kick of the previous step with the kinetic energy and, in
+    // the step mode, first half-kick, drift and wrap.
+    float *xw, *yw;            // internal positions written (the same arrays as x, y)
+    float *vx, *vy;
+    const float *inv_m, *mass;
+    const int32_t *goff;       // gather CSR offsets (null: no particle has corners)
+    float2 *xys_out;           // list mode: the drifted positions, sorted order
+                               // (the other buffer of the pair: tiles still
+                               // stage their neighbours from xys)
+    uint32_t *dmax_out;        // list mode: slots of this drift's maximum squared
+                               // displacement (dmax: the previous drift's)
+    float *rec;                // trajectory recording (as KickArgs)
+    double *ke_tile;           // 2 per tile: sum m v^2, max v^2 of the kicked particles
+    int32_t *count, *tmp;      // cell walk: binning of the drifted positions
 };
 
 // Adjoint pass: J(x) w (J = df/dx, the negative Hessian of the potential)

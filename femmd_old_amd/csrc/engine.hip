@@ -229,16 +229,27 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             pe.rc = S.rc[k];
             hpt.push_back(pe);
         }
-        // wrap, key, stable order by cell (internal order of this replica)
+        // wrap, key, internal order of this replica: the particles without
+        // element or bond corners in cell order, then those with corners (and
+        // every particle of a replica without pair forces) in cell order.
+        // The force pass kicks the first group itself; the node kick takes
+        // the second, [pg, p1), as one contiguous range.
         std::vector<float> xw(n), yw(n);
         std::vector<int> key(n), order(n);
+        std::vector<char> gat(n, S.lj_on ? 0 : 1);
+        for (int32_t u : S.tri) gat[u] = 1;
+        for (int32_t u : S.bi) gat[u] = 1;
+        for (int32_t u : S.bj) gat[u] = 1;
         for (int u = 0; u < n; ++u) {
             xw[u] = wrap_host(S.x[u], D.Lx);
             yw[u] = wrap_host(S.y[u], D.Ly);
             key[u] = cell_key_host(xw[u], yw[u], D);
         }
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return gat[a] != gat[b] ? gat[a] < gat[b] : key[a] < key[b];
+        });
+        D.pg = p0 + (int32_t)std::count(gat.begin(), gat.end(), (char)0);
         std::vector<int32_t> &iid = E->iid_h_[r];
         iid.assign(n, 0);
         for (int k = 0; k < n; ++k) iid[order[k]] = p0 + k;
@@ -462,6 +473,26 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(sorted_typ_, N);
     FM_AL(cf_, std::max(1, 3 * E->ntri_ + 2 * E->nbond_));
     FM_AL(ke_part_, 2 * std::max<size_t>(ks.size(), 1));
+    {
+        // the node kick: per replica its particles [pg, p1) (those with
+        // element or bond corners, and every particle of a replica without
+        // pair forces: no tile kicks them), spans of kKickThreads
+        std::vector<Span> hs;
+        std::vector<int32_t> hr(2 * R, 0);
+        E->nnode_ = 0;
+        for (int r = 0; r < R; ++r) {
+            hr[2 * r] = (int)hs.size();
+            for (int b = rd[r].pg; b < rd[r].p1; b += kKickThreads)
+                hs.push_back(Span{r, b, std::min(b + kKickThreads, rd[r].p1), 0});
+            hr[2 * r + 1] = (int)hs.size();
+            E->nnode_ += rd[r].p1 - rd[r].pg;
+        }
+        E->nspans_h_ = hs;
+        if ((he = upload(E->nspans_, hs, st)) != hipSuccess ||
+            (he = upload(E->nrng_, hr, st)) != hipSuccess)
+            return E->check(he, "upload node kick", err);
+        FM_AL(ke_node_, 2 * std::max<size_t>(hs.size(), 1));
+    }
     FM_AL(fea_part_, 2 * std::max<size_t>(es.size(), 1));
     FM_AL(bond_part_, 2 * std::max<size_t>(bs.size(), 1));
     FM_AL(obs_chunk_, (size_t)R * kObsChunks * kObsRaw);
@@ -472,9 +503,11 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         FM_AL(nnum_, N);
         FM_AL(ctl_, R);
         FM_AL(any_, 1);
-        FM_AL(dmax_, (size_t)R * kDispSlots * kDispStride);
+        // two sets of displacement slots and two sorted-order position
+        // buffers: a drift writes the set its force pass does not read
+        FM_AL(dmax_, (size_t)2 * R * kDispSlots * kDispStride);
         FM_AL(spos_, N);
-        FM_AL(xys_, N);
+        FM_AL(xys_, (size_t)2 * N);
         if (E->sigma_) FM_AL(sigs_, N);
     }
 #undef FM_AL
@@ -583,9 +616,11 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
     if (list_ && R_ > kDecideMinReps &&
         (he = ftile_.alloc(std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
         return check(he, "tile list", err);
-    if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
-        return check(he, "lj_part", err);
-    if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess)
+    if ((he = lj_part_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess ||
+        (he = ke_tile_.alloc(2 * std::max<size_t>(tiles_h_.size(), 1))) != hipSuccess)
+        return check(he, "tile partials", err);
+    if ((he = hipMemsetAsync(lj_part_.p, 0, lj_part_.n * sizeof(double), stream_)) != hipSuccess ||
+        (he = hipMemsetAsync(ke_tile_.p, 0, ke_tile_.n * sizeof(double), stream_)) != hipSuccess)
         return check(he, "memset", err);
     primed_ = false;
     return FM_OK;
@@ -634,6 +669,12 @@ int Engine::set_tuning(int tile_w, int block, int use_graph, std::string &err) {
 // Binning of the positions the force pass reads (xcur); gated = list mode,
 // only the replicas whose rebuild flag is set.
 int Engine::launch_bin(bool gated) {
+    if (!gated && bins_pending_) {
+        // a drift's binning no sort consumed (a run broken off after a body):
+        // start from empty cells
+        if (launch_fill_int(count_.p, 0, (size_t)ncells_, stream_) != hipSuccess) return -1;
+        bins_pending_ = false;
+    }
     KickArgs a{};
     a.reps = reps_.p;
     a.spans = kspans_.p;
@@ -649,13 +690,13 @@ int Engine::launch_bin(bool gated) {
     if (gated) {  // list mode: decide and bin only the replicas that rebuild
         a.ctl = ctl_.p;
         a.any = any_.p;
-        a.dmax = dmax_.p;
+        a.dmax = dmax_cur();  // the drift that produced the current positions
         a.one_rep = R_ == 1;
         if (R_ > kDecideMinReps) {
             DecideArgs d{};
             d.reps = reps_.p;
             d.ctl = ctl_.p;
-            d.dmax = dmax_.p;
+            d.dmax = dmax_cur();
             d.rng = rng_.p;
             d.nrep = R_;
             d.any = any_.p;
@@ -674,17 +715,9 @@ int Engine::launch_bin(bool gated) {
     return launch_kick(0, false, false, (int)kspans_h_.size(), a, stream_) == hipSuccess ? 0 : -1;
 }
 
-// kick2 / drift mode of one step body followed by binning, sort and forces
-int Engine::launch_step_body(bool kick2, bool drift, bool energy, bool l1, bool l2) {
-    struct Restore {  // the flags of this body, restored on every return
-        Engine *e;
-        bool en, a, b;
-        ~Restore() { e->energy_ = en; e->l1_ = a; e->l2_ = b; }
-    } restore{this, energy_, l1_, l2_};
-    energy_ = energy;
-    l1_ = l1;
-    l2_ = l2;
-    hipError_t e;
+// The kick arguments shared by the first half-kick of a run (every
+// particle) and the node kick (the particles with element or bond corners).
+KickArgs Engine::kick_args(bool drift) {
     KickArgs k{};
     k.reps = reps_.p;
     k.spans = kspans_.p;
@@ -706,26 +739,80 @@ int Engine::launch_step_body(bool kick2, bool drift, bool energy, bool l1, bool 
     k.rec = drift ? rec_ : nullptr;
     k.n = N_;
     if (list_) {
-        k.dmax = dmax_.p;
+        // the drift writes the positions of the next buffer of the pair
+        k.dmax = dmax_nxt();
         k.spos = spos_.p;
-        k.xys = xys_.p;
+        k.xys = xys_nxt();
     }
-    const int mode = (kick2 ? 2 : 0) | (drift ? 1 : 0);
-    time_next(0);
-    e = launch_kick(mode, gather_, energy_, (int)kspans_h_.size(), k, stream_);
-    arm_kernel_timer(nullptr, nullptr);
-    if (e != hipSuccess) return -1;
-    if (!drift) return 0;
-    if (!list_) return launch_forces_only(false);
-    // list mode: binning, sort and list build run only for the replicas
-    // where some particle moved more than skin / 2 since the last build
-    time_next(1);
-    const int bad = launch_bin(true);
-    arm_kernel_timer(nullptr, nullptr);
-    if (bad) return -1;
-    if (launch_sort(true)) return -1;
-    if (launch_list_build(true)) return -1;
-    return launch_pair_pass();
+    return k;
+}
+
+// One piece of a run (femmd2D/src/integrator.cpp:8-29 per step):
+//   open  (drift, !kick2): first half-kick, drift and wrap of every particle
+//         from the stored forces (k_kick);
+//   body  (drift, kick2):  the neighbour update of the current positions
+//         (list mode: binning, counting sort and list build gated on the
+//         rebuild decision; cell walk: the sort of the last binning), then
+//         the force pass, which also gives every particle without element or
+//         bond corners its second half-kick of this step and the first
+//         half-kick, drift and wrap of the next, then the node kick of the
+//         others (their gathered corner forces added);
+//   close (!drift, kick2): as a body with the second half-kick only; the
+//         forces are stored for the next run's first half-kick.
+// In list mode the drift writes the other buffer of the sorted-order
+// positions and displacement slots (par_ flips).
+int Engine::launch_step_body(bool kick2, bool drift, bool energy, bool l1, bool l2) {
+    struct Restore {  // the flags of this body, restored on every return
+        Engine *e;
+        bool en, a, b;
+        ~Restore() { e->energy_ = en; e->l1_ = a; e->l2_ = b; }
+    } restore{this, energy_, l1_, l2_};
+    energy_ = energy;
+    l1_ = l1;
+    l2_ = l2;
+    if (!kick2) {
+        if (!drift) return 0;
+        const KickArgs k = kick_args(true);
+        const hipError_t e = launch_kick(1, gather_, false, (int)kspans_h_.size(), k, stream_);
+        if (e != hipSuccess) return -1;
+        if (list_) par_ ^= 1;
+        else bins_pending_ = true;
+        return 0;
+    }
+    if (list_) {
+        // binning, sort and list build run only for the replicas where some
+        // particle moved more than skin / 2 since the last build
+        time_next(1);
+        const int bad = launch_bin(true);
+        arm_kernel_timer(nullptr, nullptr);
+        if (bad) return -1;
+        if (launch_sort(true)) return -1;
+        if (launch_list_build(true)) return -1;
+    } else {
+        if (launch_sort()) return -1;  // the binning of the last drift
+    }
+    {
+        const ForceArgs f = force_args();
+        time_next(4);
+        const hipError_t e =
+            launch_forces(sigma_, types_, false, list_, energy_, drift ? 2 : 1, block_, f, stream_);
+        arm_kernel_timer(nullptr, nullptr);
+        if (e != hipSuccess) return -1;
+        ene_valid_ = energy_ || !drift;
+    }
+    if (nnode_ > 0) {
+        KickArgs k = kick_args(drift);
+        k.spans = nspans_.p;
+        k.ke_part = ke_node_.p;
+        time_next(0);
+        const hipError_t e =
+            launch_kick(drift ? 3 : 2, gather_, energy_ || !drift, (int)nspans_h_.size(), k, stream_);
+        arm_kernel_timer(nullptr, nullptr);
+        if (e != hipSuccess) return -1;
+    }
+    if (drift && list_) par_ ^= 1;
+    if (drift && !list_) bins_pending_ = true;  // the next body's sort consumes it
+    return 0;
 }
 
 // Counting sort of the last binning (single pass: scan and scatter).
@@ -755,7 +842,7 @@ int Engine::launch_sort(bool gated) {
     s.obs_l1 = (!list_ && l1_) ? 1 : 0;  // the cell walk's sort forms level 1
     if (list_) {
         s.spos = spos_.p;
-        s.xys = xys_.p;
+        s.xys = xys_cur();
         s.sigs = sigma_ ? sigs_.p : nullptr;
     }
     if (gated) {
@@ -770,6 +857,7 @@ int Engine::launch_sort(bool gated) {
     time_next(2);
     const hipError_t e = ::fm::launch_sort(s, stream_);
     arm_kernel_timer(nullptr, nullptr);
+    if (!gated) bins_pending_ = false;
     return e == hipSuccess ? 0 : -1;
 }
 
@@ -790,7 +878,7 @@ int Engine::launch_list_build(bool gated) {
 int Engine::launch_pair_pass() {
     const ForceArgs f = force_args();
     time_next(4);
-    const hipError_t e = launch_forces(sigma_, types_, false, true, energy_, block_, f, stream_);
+    const hipError_t e = launch_forces(sigma_, types_, false, true, energy_, 0, block_, f, stream_);
     arm_kernel_timer(nullptr, nullptr);
     ene_valid_ = energy_;
     return e == hipSuccess ? 0 : -1;
@@ -802,7 +890,7 @@ int Engine::launch_forces_only(bool debug) {
     const ForceArgs f = force_args();
     time_next(4);
     const hipError_t e =
-        launch_forces(sigma_, types_, debug, false, energy_ || debug, block_, f, stream_);
+        launch_forces(sigma_, types_, debug, false, energy_ || debug, 0, block_, f, stream_);
     arm_kernel_timer(nullptr, nullptr);
     ene_valid_ = energy_ || debug;
     return e == hipSuccess ? 0 : -1;
@@ -851,10 +939,8 @@ ForceArgs Engine::force_args() {
     f.lbuf = list_build_lds_bytes(lds_cap_, cap_, nbr_cap_, block_, true) <= 62 * 1024;
     f.cell_cap = cap_;
     f.gated_grid = lb_grid_;
-    f.xys = xys_.p;
     f.sigs = sigs_.p;
     f.any = list_ ? any_.p : nullptr;
-    f.dmax = dmax_.p;
     f.ctl_w = ctl_.p;
     f.nrep = R_;
     f.n = N_;
@@ -865,6 +951,22 @@ ForceArgs Engine::force_args() {
     f.err = err_.p;
     f.obs = obs_args();
     f.obs_l2 = l2_ ? 1 : 0;
+    f.xys = list_ ? xys_cur() : nullptr;
+    f.dmax = list_ ? dmax_cur() : nullptr;
+    // the kick of the force pass
+    f.xw = x_.p;
+    f.yw = y_.p;
+    f.vx = vx_.p;
+    f.vy = vy_.p;
+    f.inv_m = inv_m_.p;
+    f.mass = mass_.p;
+    f.goff = gather_ ? goff_.p : nullptr;
+    f.xys_out = list_ ? xys_nxt() : nullptr;
+    f.dmax_out = list_ ? dmax_nxt() : nullptr;
+    f.rec = rec_;
+    f.ke_tile = ke_tile_.p;
+    f.count = count_.p;
+    f.tmp = tmp_.p;
     return f;
 }
 
@@ -872,7 +974,11 @@ ObsArgs Engine::obs_args() {
     ObsArgs o{};
     o.reps = reps_.p;
     o.rng = rng_.p;
+    o.nrng = nrng_.p;
+    o.ke_kick = 0;
     o.ke_part = ke_part_.p;
+    o.ke_tile = ke_tile_.p;
+    o.ke_node = ke_node_.p;
     o.lj_part = lj_part_.p;
     o.fea_part = fea_part_.p;
     o.bond_part = bond_part_.p;
@@ -902,28 +1008,33 @@ int Engine::profile(int nsteps, double *out, std::string &err) {
     // bodies carry the observables as the timed steps of a run do
     const bool every = obs_every_ > 0;
     obs_valid_ = false;
-    if (launch_step_body(false, true, every, false, false)) return check(hipGetLastError(), "step", err);
+    (void)every;
+    if (launch_bodies_eager(kind_open())) return check(hipGetLastError(), "step", err);
     for (int s = 0; s < nsteps; ++s) {
         prof_ev_ = &ev[K * s];
-        int bad = launch_step_body(true, true, every || s == nsteps - 1, every, every);
+        const BodyKind k = s == 0 ? kind_first() : kind_body(1);
+        int bad = launch_step_body(k.kick2, k.drift, k.energy, k.l1, k.l2);
         prof_ev_ = nullptr;
         if (bad) return check(hipGetLastError(), "step", err);
     }
-    if (launch_bodies_eager(BodyKind{1, true, false, true, false, false, true}))
+    if (launch_bodies_eager(kind_close(nsteps >= 1)))
         return check(hipGetLastError(), "final kick", err);
     steps_ += nsteps + 1;
     rc = sync(err);
     if (!rc) obs_valid_ = true;
     // out[k] = dispatch duration of kernel k (0 where the mode launches no
     // such kernel: its events were never recorded), out[5] = whole body,
-    // from the kick's start to the force pass's end
+    // from its first kernel's start (binning in list mode, else the sort) to
+    // its last kernel's end (the node kick, else the force pass)
     double acc[kProfKernels + 1] = {0};
+    const int k_first = list_ ? 1 : 2, k_last = nnode_ > 0 ? 0 : 4;
     for (int s = 0; s < nsteps && !rc; ++s) {
         float t;
         hipEvent_t *e = &ev[K * s];
         for (int k = 0; k < kProfKernels; ++k)
             if (hipEventElapsedTime(&t, e[2 * k], e[2 * k + 1]) == hipSuccess) acc[k] += t;
-        if (hipEventElapsedTime(&t, e[0], e[K - 1]) == hipSuccess) acc[kProfKernels] += t;
+        if (hipEventElapsedTime(&t, e[2 * k_first], e[2 * k_last + 1]) == hipSuccess)
+            acc[kProfKernels] += t;
     }
     (void)hipGetLastError();  // the elapsed times of unrecorded events failed
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -938,7 +1049,8 @@ int Engine::ensure_primed(std::string &err) {
         // a fresh list epoch at the current positions
         if (launch_sort()) return check(hipGetLastError(), "sort", err);
         if (launch_list_build(false)) return check(hipGetLastError(), "list build", err);
-        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, stream_) != hipSuccess) return check(hipGetLastError(), "ctl", err);
+        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, 2 * R_ * kDispSlots, stream_) != hipSuccess)
+            return check(hipGetLastError(), "ctl", err);
         if (launch_pair_pass()) return check(hipGetLastError(), "forces", err);
     } else {
         if (launch_forces_only(false)) return check(hipGetLastError(), "forces", err);
@@ -1011,30 +1123,35 @@ int Engine::ensure_order(std::string &err) {
     return FM_OK;
 }
 
-// A run of identical bodies (or the closing half-kick and observables):
-// each distinct kind is captured into a graph the first time it is launched
-// and replayed afterwards, so that steady-state runs of any length launch
-// only graphs.
+// A run of identical bodies, or the opening / closing piece of a run: each
+// distinct kind (with the buffer parity it starts from) is captured into a
+// graph the first time it is launched and replayed afterwards, so that
+// steady-state runs of any length launch only graphs.
 int Engine::launch_bodies_eager(const BodyKind &k) {
-    if (k.close) {
-        // femmd2D/src/integrator.cpp:18-19: second half-kick (kinetic-energy
-        // partials) and the step's observables (Properties::compute)
-        if (launch_step_body(true, false, true, false, false)) return -1;
-        return launch_observe(obs_args(), true, stream_) == hipSuccess ? 0 : -1;
-    }
     for (int q = 0; q < k.count; ++q)
         if (launch_step_body(k.kick2, k.drift, k.energy, k.l1, k.l2)) return -1;
+    if (k.close) {
+        // femmd2D/src/integrator.cpp:18-19: the observables of the last step
+        // (Properties::compute)
+        return launch_observe(obs_args(), true, stream_) == hipSuccess ? 0 : -1;
+    }
     return 0;
 }
 
 hipGraphExec_t Engine::graph_of(const BodyKind &k) {
-    const int key = k.key();
+    const int key = k.key() | (par_ << 14);
     for (auto &g : graphs_)
         if (g.first == key) return g.second;
+    // capturing records the launches of the kind from the current parity;
+    // nothing runs, so the parity is restored afterwards
+    const int par0 = par_;
+    const bool pend0 = bins_pending_;
     if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     const int bad = launch_bodies_eager(k);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(stream_, &g);
+    par_ = par0;
+    bins_pending_ = pend0;
     if (bad || ce != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return nullptr;
@@ -1051,24 +1168,44 @@ int Engine::launch_bodies(const BodyKind &k) {
     if (!use_graph_ || rec_ || prof_ev_) return launch_bodies_eager(k);
     hipGraphExec_t ex = graph_of(k);
     if (!ex) return -1;
-    return hipGraphLaunch(ex, stream_) == hipSuccess ? 0 : -1;
+    if (hipGraphLaunch(ex, stream_) != hipSuccess) return -1;
+    if (list_ && k.drift) par_ ^= (k.count & 1);  // each drift flips the parity
+    if (!list_) bins_pending_ = k.drift;          // a body or the opening kick bins
+    return 0;
+}
+
+// The kinds of a run: the first half-kick; the first body (its step's
+// observables are formed, nothing earlier is accumulated); further bodies
+// (the previous step's observables accumulated); the close.  n >= 2 closes
+// with the previous step accumulated.
+Engine::BodyKind Engine::kind_open() const { return BodyKind{1, false, true, false, false, false, false}; }
+Engine::BodyKind Engine::kind_first() const {
+    const bool every = obs_every_ > 0;
+    return BodyKind{1, true, true, every, false, false, false};
+}
+Engine::BodyKind Engine::kind_body(int count) const {
+    const bool every = obs_every_ > 0;
+    return BodyKind{count, true, true, every, every, every, false};
+}
+Engine::BodyKind Engine::kind_close(bool after_body) const {
+    const bool acc = obs_every_ > 0 && after_body;
+    return BodyKind{1, true, false, true, acc, acc, true};
 }
 
 int Engine::prepare_graphs() {
     if (!use_graph_) return 0;
-    const bool every = obs_every_ > 0;
-    const BodyKind kinds[] = {
-        {1, false, true, every, false, false, false},             // first body
-        {kGraphSteps, true, true, every, every, every, false},    // runs of 16
-        {1, true, true, every, every, every, false},              // single bodies
-        {1, true, true, true, false, false, false},               // last body (per-run mode)
-        {1, false, true, true, false, false, false},              // first = last body
-        {1, true, false, true, false, false, true},               // closing half-kick
-    };
-    for (const BodyKind &k : kinds) {
-        if (every && (&k == &kinds[3] || &k == &kinds[4])) continue;
-        if (!graph_of(k)) return -1;
+    const int par0 = par_;
+    const BodyKind kinds[] = {kind_open(), kind_first(), kind_body(kGraphSteps), kind_body(1),
+                              kind_close(true), kind_close(false)};
+    for (int p = 0; p < (list_ ? 2 : 1); ++p) {
+        par_ = p;
+        for (const BodyKind &k : kinds)
+            if (!graph_of(k)) {
+                par_ = par0;
+                return -1;
+            }
     }
+    par_ = par0;
     return 0;
 }
 
@@ -1079,19 +1216,18 @@ int Engine::step(int nsteps, std::string &err) {
     if (rc) return rc;
     rc = ensure_primed(err);
     if (rc) return rc;
-    // femmd2D/src/integrator.cpp:8-29 per step; the second half-kick of
-    // step s is fused with the first half-kick of step s+1.  Observed every
-    // step (obs_every_): every body forms the pair energies and virials and
-    // the kinetic-energy partials, and accumulates the previous step's
-    // observables; the closing body forms and accumulates the last step's.
-    // Otherwise only the last step's are formed and accumulated.
-    const bool every = obs_every_ > 0;
     if (graphs_.empty() && prepare_graphs()) {
         drop_graph();
         return check(hipErrorUnknown, "graph capture", err);
     }
-    // the internal order is re-sorted between step bodies every
-    // reorder_every_ steps (asynchronously, outside the step graphs)
+    // femmd2D/src/integrator.cpp:8-29 per step; the second half-kick of
+    // step s is fused with the first half-kick of step s+1 in the force pass
+    // of step s.  Observed every step (obs_every_): every step's energies,
+    // virials and kinetic energy are formed by its force pass and node kick,
+    // and accumulated by the next body (or the close); otherwise only the
+    // last step's, by the close.
+    // The internal order is re-sorted between pieces of a run every
+    // reorder_every_ steps (asynchronously, outside the step graphs).
     auto advance = [&](int done) -> int {
         since_reorder_ += done;
         if (reorder_every_ <= 0 || since_reorder_ < reorder_every_) return 0;
@@ -1103,32 +1239,26 @@ int Engine::step(int nsteps, std::string &err) {
         const hipError_t e = hipGetLastError();
         return check(e != hipSuccess ? e : hipErrorUnknown, what, err);
     };
-    // the first body: half-kick and drift only (the velocities are a full
-    // step's); nothing to accumulate yet
-    if (launch_bodies(BodyKind{1, false, true, every || nsteps == 1, false, false, false}))
-        return fail("step");
+    if (launch_bodies(kind_open())) return fail("step");
     if (advance(1)) return fail("reorder");
-    int rem = nsteps - 1;
-    const BodyKind body16{kGraphSteps, true, true, every, every, every, false};
-    const BodyKind body1{1, true, true, every, every, every, false};
-    // (when not every step is observed, the last body forms the energies)
-    const int tail = every ? 0 : 1;
-    while (rem - tail >= kGraphSteps) {
-        if (launch_bodies(body16)) return fail("step");
+    int rem = nsteps - 1;  // drifts still to do
+    if (rem > 0) {
+        if (launch_bodies(kind_first())) return fail("step");
+        if (advance(1)) return fail("reorder");
+        --rem;
+    }
+    while (rem >= kGraphSteps) {
+        if (launch_bodies(kind_body(kGraphSteps))) return fail("step");
         rem -= kGraphSteps;
         if (advance(kGraphSteps)) return fail("reorder");
     }
-    for (; rem > tail; --rem) {
-        if (launch_bodies(body1)) return fail("step");
+    for (; rem > 0; --rem) {
+        if (launch_bodies(kind_body(1))) return fail("step");
         if (advance(1)) return fail("reorder");
     }
-    if (rem > 0) {  // the last body forms the energies
-        if (launch_bodies(BodyKind{1, true, true, true, false, false, false})) return fail("step");
-        if (advance(1)) return fail("reorder");
-    }
-    if (launch_bodies(BodyKind{1, true, false, true, false, false, true})) return fail("final kick");
+    if (launch_bodies(kind_close(nsteps >= 2))) return fail("final kick");
     steps_ += nsteps;
-    ene_valid_ = true;  // the last force pass formed the energies
+    ene_valid_ = true;  // the closing force pass formed the energies
     obs_valid_ = true;
     return sync(err);
 }
@@ -1185,7 +1315,9 @@ int Engine::form_observables(std::string &err) {
     k.ke_part = ke_part_.p;
     if (launch_kick(4, false, false, (int)kspans_h_.size(), k, stream_) != hipSuccess)
         return check(hipGetLastError(), "ke", err);
-    if (launch_observe(obs_args(), false, stream_) != hipSuccess)
+    ObsArgs o = obs_args();
+    o.ke_kick = 1;  // the partials k_ke just formed
+    if (launch_observe(o, false, stream_) != hipSuccess)
         return check(hipGetLastError(), "observe", err);
     rc = sync(err);
     if (rc) return rc;
@@ -1416,7 +1548,7 @@ int Engine::pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string
         // positions (lists as last built, possibly steps ago): the list pass
         // with the dump, which leaves the rebuild state alone
         const ForceArgs f = force_args();
-        if (launch_forces(sigma_, types_, true, true, true, block_, f, stream_) != hipSuccess)
+        if (launch_forces(sigma_, types_, true, true, true, 0, block_, f, stream_) != hipSuccess)
             return check(hipGetLastError(), "pair sets", err);
     } else {
         // rebin (the debug pass must see the binning of the current positions)

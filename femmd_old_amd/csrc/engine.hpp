@@ -176,6 +176,10 @@ private:
     };
     int launch_bodies(const BodyKind &k);
     int launch_bodies_eager(const BodyKind &k);
+    BodyKind kind_open() const;
+    BodyKind kind_first() const;
+    BodyKind kind_body(int count) const;
+    BodyKind kind_close(bool after_body) const;
     // the graph of kind k, captured (not launched) if missing; null when
     // graphs are off or the capture failed
     hipGraphExec_t graph_of(const BodyKind &k);
@@ -187,7 +191,22 @@ private:
     int launch_list_build(bool gated);
     int launch_pair_pass();  // list mode: list pair pass + elements + bonds
     ForceArgs force_args();
+    KickArgs kick_args(bool drift);
     ObsArgs obs_args();
+    // list mode: the sorted-order positions and displacement slots come in
+    // pairs; par_ names the buffer that holds the current positions (and the
+    // slots of the drift that produced them), the other one receives the
+    // next drift's
+    int par_ = 0;
+    // cell walk: a drift's binning not yet consumed by a sort (a body ends
+    // with the next step's binning pending); a fresh binning clears it first
+    bool bins_pending_ = false;
+    float2 *xys_cur() const { return xys_.p + (size_t)par_ * N_; }
+    float2 *xys_nxt() const { return xys_.p + (size_t)(par_ ^ 1) * N_; }
+    uint32_t *dmax_cur() const { return dmax_.p + (size_t)par_ * R_ * kDispSlots * kDispStride; }
+    uint32_t *dmax_nxt() const {
+        return dmax_.p + (size_t)(par_ ^ 1) * R_ * kDispSlots * kDispStride;
+    }
     // the observables of the current state (no accumulation), when the last
     // formed step is not the current state
     int form_observables(std::string &err);
@@ -280,6 +299,15 @@ private:
     DBuf<float2> alm_, cf_;
     DBuf<float> bk_, br0_;
     DBuf<double> ke_part_, lj_part_, fea_part_, bond_part_;
+    // node kick: the number of particles it integrates (per replica
+    // [pg, p1)), its spans (per replica [nrng[2r], nrng[2r+1])) and their
+    // kinetic-energy partials; the force pass's partials of the particles it
+    // integrates, per tile
+    int nnode_ = 0;
+    DBuf<int32_t> nrng_;
+    DBuf<Span> nspans_;
+    std::vector<Span> nspans_h_;
+    DBuf<double> ke_node_, ke_tile_;
     DBuf<double> obs_chunk_, obs_;  // observables: level-1 chunks, per-replica records
     std::vector<double> obs_h_;
     DBuf<int32_t> rng_;
@@ -321,12 +349,12 @@ private:
 hipError_t launch_kick(int mode, bool gather, bool energy, int nblk, const KickArgs &a,
                        hipStream_t s);
 hipError_t launch_sort(const ScanArgs &a, hipStream_t s);
-hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
-                         const ForceArgs &a, hipStream_t s);
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int kick,
+                         int block, const ForceArgs &a, hipStream_t s);
 hipError_t launch_list_build(bool sigma, int block, const ForceArgs &a, hipStream_t s);
 size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered);
 int list_build_resident(bool sigma, int block, const ForceArgs &a, int device);
-hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, hipStream_t s);
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, int nslots, hipStream_t s);
 size_t forces_lds_bytes(int lds_cap, bool types, bool debug);
 hipError_t launch_total_force(int n, const float *fx, const float *fy, const float2 *cf,
                               const int32_t *goff, const int32_t *gent, float *ox, float *oy,

@@ -1,20 +1,27 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the femmd2D step.
 //
-// Per step (one HIP graph node each):
-//   k_kick      second half-kick of the previous step (+ KE partials),
-//               first half-kick, drift, periodic wrap, cell key, bin
-//               (wavefront-aggregated integer atomics into a capacity-
-//               padded slot list per cell)
-//   k_scan_reduce / k_scan_apply / k_scatter
-//               counting sort: exclusive scan of the cell counts into
-//               cell_start, then one thread per particle places it at
-//               cell_start + its rank by index within the cell (perm) and
-//               copies its position into sorted order
-//   k_forces    one launch with three workgroup roles:
-//               LJ cell tiles (LDS-staged 3x3 cells), CST elements and
-//               harmonic bonds (corner forces into a buffer, no atomics)
-// and the gather of element / bond corner forces happens inside the next
-// k_kick through a per-particle CSR, in a fixed order.
+// A run of steps (one HIP graph node each):
+//   k_kick       first half-kick, drift and wrap of every particle from the
+//                stored forces (the start of a run; cell walk: binning)
+// then per step (femmd2D/src/integrator.cpp:8-29):
+//   k_bin        list mode: rebuild decision (displacement bound > skin/2)
+//                and binning of the replicas that rebuild (gated)
+//   k_sort       single-pass counting sort of the binning (decoupled
+//                look-back): cell starts, sorted order, sorted copies
+//   k_list_build list mode: Verlet lists of the rebuilding replicas' tiles
+//   k_forces     one launch with three workgroup roles: LJ cell tiles
+//                (LDS-staged neighbours, lists or the 3x3 cell walk), CST
+//                elements and harmonic bonds (corner forces into a buffer in
+//                gather order, no atomics); each tile then gives its centre
+//                particles that have no element or bond corners their second
+//                half-kick, and the first half-kick, drift and wrap of the
+//                next step (cell walk: binning of the new positions)
+//   k_kick       the node kick: the same for the particles with corners,
+//                their corner forces gathered through a per-particle CSR in
+//                a fixed order
+// and at the end of a run the second half-kick only, then the observables.
+// Energies, virials and the kinetic energy are per-workgroup FP64 partials
+// summed in a fixed order (k_bin / k_sort level 1, k_forces level 2).
 //
 // The file is compiled with -ffp-contract=off: every FP32 expression that
 // must agree bit for bit with the reference's real=float arithmetic (wrap,
@@ -141,15 +148,27 @@ __device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
     const int tid = threadIdx.x;
     double v[kObsRaw] = {0, 0, 0, 0, 0, 0, 0, 0};
     int lo, hi;
-    obs_slice(g[0], g[1], c, lo, hi);
-    for (int k = lo + tid; k < hi; k += 256) {
-        v[0] += o.ke_part[2 * k];
-        v[1] = fmax(v[1], o.ke_part[2 * k + 1]);
+    if (o.ke_kick) {  // a query: the partials of k_ke over the kick spans
+        obs_slice(g[0], g[1], c, lo, hi);
+        for (int k = lo + tid; k < hi; k += 256) {
+            v[0] += o.ke_part[2 * k];
+            v[1] = fmax(v[1], o.ke_part[2 * k + 1]);
+        }
+    } else {  // a step: the particles the node kick integrated
+        obs_slice(o.nrng[2 * r], o.nrng[2 * r + 1], c, lo, hi);
+        for (int k = lo + tid; k < hi; k += 256) {
+            v[0] += o.ke_node[2 * k];
+            v[1] = fmax(v[1], o.ke_node[2 * k + 1]);
+        }
     }
     obs_slice(g[2], g[3], c, lo, hi);
     for (int k = lo + tid; k < hi; k += 256) {
         v[2] += o.lj_part[2 * k];
         v[3] += o.lj_part[2 * k + 1];
+        if (!o.ke_kick) {  // and those the pair pass integrated, per tile
+            v[0] += o.ke_tile[2 * k];
+            v[1] = fmax(v[1], o.ke_tile[2 * k + 1]);
+        }
     }
     obs_slice(g[4], g[5], c, lo, hi);
     for (int k = lo + tid; k < hi; k += 256) {
@@ -225,7 +244,8 @@ __device__ void obs_finish(const ObsArgs &o, int r, bool accumulate) {
 // run and hand out slots by lane offset.  Slot order within a cell is not
 // deterministic; the scatter below orders each cell by particle index.
 // All 64 lanes of the wavefront must call this (ballot / shuffles).
-__device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, int c) {
+__device__ __forceinline__ void bin_wave_to(int32_t *count, int32_t *tmp, int cap, uint32_t *err,
+                                            bool active, int p, int c) {
     const int lane = threadIdx.x & 63;
     const int ck = active ? c : -1 - lane;  // inactive lanes never join a run
     const int prev = __shfl_up(ck, 1);
@@ -236,22 +256,67 @@ __device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, 
     const unsigned long long after = heads & ~upto;
     const int next = after ? (__ffsll((unsigned long long)after) - 1) : 64;
     int base = 0;
-    if (head && active) base = atomicAdd(&a.count[c], next - lane);
+    if (head && active) base = atomicAdd(&count[c], next - lane);
     base = __shfl(base, h);
     if (active) {
         const int slot = base + (lane - h);
-        if (slot < a.cap) a.tmp[(size_t)c * a.cap + slot] = p;
-        else atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW (the particle is dropped)
+        if (slot < cap) tmp[(size_t)c * cap + slot] = p;
+        else atomicOr(err, 1u);  // FM_DEV_CELL_OVERFLOW (the particle is dropped)
+    }
+}
+__device__ __forceinline__ void bin_wave(const KickArgs &a, bool active, int p, int c) {
+    bin_wave_to(a.count, a.tmp, a.cap, a.err, active, p, c);
+}
+
+// The velocity-Verlet kick of one particle (femmd2D/src/integrator.cpp:
+// 100-135, integratorCuda.cu:110-206), shared by k_kick and the pair pass's
+// kick so both give the same bits.  ax, ay = f * inv_m.
+//   kKick2: v += dt/2 a (second half of the previous step); with kEnergy
+//           m |v|^2 (m from 1/m: one rounding from the stored mass; a pinned
+//           particle has inv_m = 0 and zero velocity) and |v|^2
+//   kDrift: v += dt/2 a, x += dt v, wrap (non-finite coordinates flagged
+//           and reset); dxp, dyp: the step's displacement
+template <bool kKick2, bool kDrift, bool kEnergy>
+__device__ __forceinline__ void kick_particle(const RepDesc &R, float ax, float ay, float im,
+                                              float &vx, float &vy, float &x, float &y,
+                                              double &ke, float &vv, float &dxp, float &dyp,
+                                              uint32_t *err) {
+    if (kKick2) {
+        vx = vx + R.hdt * ax;
+        vy = vy + R.hdt * ay;
+        if (kEnergy) {
+            vv = vx * vx + vy * vy;
+            const float m = im > 0.0f ? 1.0f / im : 0.0f;
+            ke = (double)(m * vv);
+        }
+    }
+    if (kDrift) {
+        vx = vx + R.hdt * ax;
+        vy = vy + R.hdt * ay;
+        dxp = R.dt * vx;
+        dyp = R.dt * vy;
+        x = x + dxp;
+        y = y + dyp;
+        x = wrap1(x, R.Lx);
+        y = wrap1(y, R.Ly);
+        if (!(isfinite(x) && isfinite(y))) {
+            atomicOr(err, 8u);  // FM_DEV_NONFINITE
+            x = 0.0f;
+            y = 0.0f;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_kick: integrator + binning (femmd2D/src/integratorCuda.cu:110-206,
+// k_kick: integrator + binning (femmd2D/src/integrator.cpp:100-135,
 // femmd2D/src/nnCuda.cu:69-82 re-designed).  One workgroup covers one span
-// of one replica.
-//   kKick2: v += dt/2 a (second half of the previous step); with kEnergy the
-//           kinetic-energy and max v^2 partials of the workgroup
-//   kDrift: v += dt/2 a, x += dt v, wrap; bin (cell walk every step) or
+// of one replica: particles [begin, end) (all particles: the first
+// half-kick of a run; the replica's [pg, p1): the node kick, of the
+// particles with element or bond corners, which the force pass does not
+// kick).
+//   kKick2: second half-kick of the previous step (kinetic-energy and max
+//           v^2 partials of the workgroup with kEnergy)
+//   kDrift: first half-kick, drift, wrap; bin (cell walk every step) or
 //           mirror x, y into sorted order and take the step's largest
 //           displacement (neighbour lists)
 // a = (f_pair + sum of the particle's corner forces) * inv_m; the corner
@@ -264,10 +329,11 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     const Span sp = a.spans[blockIdx.x];
     const RepDesc R = a.reps[sp.rep];
     const int p = sp.begin + (int)threadIdx.x;
+    const bool active = p < sp.end;
     double ke = 0.0;
     float vvmax = 0.0f, d2 = 0.0f;
     int cell = 0;
-    if (p < sp.end) {
+    if (active) {
         float fx = a.fx[p], fy = a.fy[p];
         if (kGather) {
             const int e0 = a.goff[p], e1 = a.goff[p + 1];
@@ -282,31 +348,14 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         else im = a.inv_m[p];
         const float ax = fx * im, ay = fy * im;
         float vx = a.vx[p], vy = a.vy[p];
-        if (kKick2) {
-            vx = vx + R.hdt * ax;
-            vy = vy + R.hdt * ay;
-            if (kEnergy) {
-                const float vv = vx * vx + vy * vy;
-                // m from 1/m (one rounding from the stored mass; a pinned
-                // particle has inv_m = 0 and zero velocity)
-                const float m = im > 0.0f ? 1.0f / im : 0.0f;
-                ke = (double)(m * vv);
-                vvmax = vv;
-            }
-        }
+        float x = 0.0f, y = 0.0f, dxp = 0.0f, dyp = 0.0f;
         if (kDrift) {
-            vx = vx + R.hdt * ax;
-            vy = vy + R.hdt * ay;
-            const float dxp = R.dt * vx, dyp = R.dt * vy;
-            float x = a.x[p] + dxp;
-            float y = a.y[p] + dyp;
-            x = wrap1(x, R.Lx);
-            y = wrap1(y, R.Ly);
-            if (!(isfinite(x) && isfinite(y))) {
-                atomicOr(a.err, 8u);  // FM_DEV_NONFINITE
-                x = 0.0f;
-                y = 0.0f;
-            }
+            x = a.x[p];
+            y = a.y[p];
+        }
+        kick_particle<kKick2, kDrift, kEnergy>(R, ax, ay, im, vx, vy, x, y, ke, vvmax, dxp, dyp,
+                                               a.err);
+        if (kDrift) {
             a.x[p] = x;
             a.y[p] = y;
             if (a.rec) {
@@ -322,7 +371,7 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
         a.vx[p] = vx;
         a.vy[p] = vy;
     }
-    if (kDrift && !kList) bin_wave(a, p < sp.end, p, cell);
+    if (kDrift && !kList) bin_wave(a, active, p, cell);
     if (kDrift && kList) {
         // the workgroup's largest displacement, one atomic max into a slot of
         // the replica (relaxed: the binning reads the slots after this
@@ -627,6 +676,7 @@ __device__ __forceinline__ int rep_of_cell(const ScanArgs &a, int b, int c) {
 __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ int s_ex[kSortCells + 1];  // local exclusive prefix of the counts
     __shared__ int s_st[kSortCells];      // cell starts
+    __shared__ int s_lim[kSortCells];     // end of each cell's replica range
     __shared__ int s_w[kSortCells / 64];
     __shared__ unsigned long long s_tk;
     __shared__ int s_carry, s_head;
@@ -673,6 +723,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     }
     const int r = rep_of_cell(a, b, min(c, a.ncells - 1));
     const RepDesc &R = a.reps[r];
+    s_lim[t] = R.p1;
     const int hl = R.cell0 - c0;  // replica's first cell, local (< 0: earlier block)
     // block exclusive scan
     const int incl = wave_incl_scan(n);
@@ -802,6 +853,12 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
                 }
                 ii[u] = i;
                 pos[u] = s_st[cl] + rank;
+                // counts that add up past the replica's particles (a binning
+                // on top of one not yet sorted) are reported, never written
+                if (pos[u] >= s_lim[cl]) {
+                    atomicOr(a.err, 1u);  // FM_DEV_CELL_OVERFLOW
+                    ii[u] = -1;
+                }
             }
         }
         float4 e[kSortBatch];
@@ -855,14 +912,122 @@ struct PairLds {
     int *cstart;   // per slot: first sorted position
     int *off;      // per slot: first LDS entry (S + 1)
     PairEntry *pt; // pair table of the replica
-    double *red;
+    double *red;   // 3 per wavefront
+    float *redf;   // 2 per wavefront
 };
+
+// The kick a tile gives its centre particles (the kick modes of k_forces):
+// accumulators of the kicked particles' kinetic energy, max v^2 and max
+// squared displacement.
+struct TileKick {
+    double ke = 0.0;
+    float vv = 0.0f, d2 = 0.0f;
+};
+
+// Kick of the tile's centre particle id (sorted position p) at its staged
+// position (x, y) (the same bits as x[id]) with pair force (fxs, fys).
+// kKick 1: second half-kick only (the closing kick of a run); 2: second and
+// first half-kick, drift and wrap.  A particle with element or bond corners
+// is left to the node kick, which adds its gathered corner forces: its pair
+// force is stored for it (and every particle's in mode 0 and 1, for the next
+// run's first half-kick).  Returns the cell of the drifted position (cell
+// walk, mode 2), -1 when there is none to bin.
+template <int kKick, bool kList, bool kEnergy>
+__device__ __forceinline__ int tile_kick(const ForceArgs &a, const RepDesc &R, int id, int p,
+                                         float x, float y, float fxs, float fys, TileKick &tk) {
+    const bool node = id >= R.pg;  // (internal order: these come last)
+    if (kKick != 2 || node) {
+        a.fx[id] = fxs;
+        a.fy[id] = fys;
+    }
+    if (kKick == 0 || node) return -1;
+    const float im = R.im_uniform > 0.0f ? R.im_uniform : a.inv_m[id];
+    const float ax = fxs * im, ay = fys * im;
+    float vx = a.vx[id], vy = a.vy[id], dxp = 0.0f, dyp = 0.0f, vv = 0.0f;
+    double ke = 0.0;
+    kick_particle<true, kKick == 2, kEnergy>(R, ax, ay, im, vx, vy, x, y, ke, vv, dxp, dyp, a.err);
+    a.vx[id] = vx;
+    a.vy[id] = vy;
+    if (kEnergy) {
+        tk.ke += ke;
+        tk.vv = fmaxf(tk.vv, vv);
+    }
+    if (kKick != 2) return -1;
+    a.xw[id] = x;
+    a.yw[id] = y;
+    if (a.rec) {
+        a.rec[id] = x;
+        a.rec[a.n + id] = y;
+    }
+    if (kList) {
+        a.xys_out[p] = make_float2(x, y);
+        tk.d2 = fmaxf(tk.d2, dxp * dxp + dyp * dyp);
+        return -1;
+    }
+    return R.cell0 + cell_key(x, y, R);
+}
+
+// End of a tile: fixed-order sums of U, W and the kinetic energy, maxima of
+// v^2 and of the squared displacement (valid in thread 0).
+__device__ void tile_reduce(double &u, double &w, TileKick &tk, const PairLds &L) {
+    u = wave_sum(u);
+    w = wave_sum(w);
+    double ke = wave_sum(tk.ke);
+    float vv = wave_max(tk.vv), d2 = wave_max(tk.d2);
+    const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        L.red[3 * wv] = u;
+        L.red[3 * wv + 1] = w;
+        L.red[3 * wv + 2] = ke;
+        L.redf[2 * wv] = vv;
+        L.redf[2 * wv + 1] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double su = 0.0, sw = 0.0, sk = 0.0;
+        float mv = 0.0f, md = 0.0f;
+        for (int k = 0; k < nw; ++k) {
+            su += L.red[3 * k];
+            sw += L.red[3 * k + 1];
+            sk += L.red[3 * k + 2];
+            mv = fmaxf(mv, L.redf[2 * k]);
+            md = fmaxf(md, L.redf[2 * k + 1]);
+        }
+        u = su;
+        w = sw;
+        tk.ke = sk;
+        tk.vv = mv;
+        tk.d2 = md;
+    }
+}
+
+// Writes a tile's partials: U, W (kEnergy), the kicked particles' kinetic
+// energy and max v^2 (kEnergy with a kick) and its largest displacement
+// into one of the replica's slots for the next rebuild decision (list mode,
+// step kick).  Thread 0.
+template <int kKick, bool kList, bool kEnergy>
+__device__ __forceinline__ void tile_partials(const ForceArgs &a, int tile, int rep, double u,
+                                              double w, const TileKick &tk) {
+    if (threadIdx.x != 0) return;
+    if (kEnergy) {
+        a.lj_part[2 * tile] = u;
+        a.lj_part[2 * tile + 1] = w;
+    }
+    if (kEnergy && kKick != 0) {
+        a.ke_tile[2 * tile] = tk.ke;
+        a.ke_tile[2 * tile + 1] = (double)tk.vv;
+    }
+    if (kList && kKick == 2)
+        atomicMax(a.dmax_out + ((size_t)rep * kDispSlots + tile % kDispSlots) * kDispStride,
+                  __float_as_uint(tk.d2));
+}
 
 // LJ over one tile (femmd2D/src/interactionsCuda.cu:43-100 pair maths,
 // full-list accumulation of femmd2D/src/interactions.cpp:347-389: each
 // ordered pair adds to f_i only, U and W take 1/2).
-template <bool kWrap, bool kSigma, bool kTypes, bool kDebug, bool kEnergy>
-__device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
+template <bool kWrap, bool kSigma, bool kTypes, bool kDebug, bool kEnergy, int kKick>
+__device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int rep, int W,
                                int tile, PairLds &L) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
@@ -871,7 +1036,13 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const int nt = R.ntypes;
     double Ublk = 0.0, Wblk = 0.0;
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    TileKick tk;
+    // every lane of a wavefront takes part in each pass (the binning of the
+    // drifted positions is a wavefront operation)
+    for (int k0 = 0; k0 < nc; k0 += blockDim.x) {
+        const int k = k0 + (int)threadIdx.x;
+        int id = 0, cell = -1;
+        if (k < nc) {
         const int me = c_lo + k;
         // slot of this centre particle within row 1 (1..W)
         int lo = row + 1, hi = row + W + 1;
@@ -999,9 +1170,8 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             Ui = (double)accA + (double)ush;
             Wi = (double)accB;
         }
-        const int id = a.perm[sbase + k];
-        a.fx[id] = fx;
-        a.fy[id] = fy;
+        id = a.perm[sbase + k];
+        cell = tile_kick<kKick, false, kEnergy>(a, R, id, sbase + k, xi, yi, fx, fy, tk);
         if (kEnergy) {
             Ublk += 0.5 * Ui;
             Wblk += 0.5 * Wi;
@@ -1012,13 +1182,12 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int W,
             a.dbg_s1[u] = s1;
             a.dbg_s2[u] = s2;
         }
-    }
-    if (kEnergy) {
-        block_sum2(Ublk, Wblk, L.red);
-        if (threadIdx.x == 0) {
-            a.lj_part[2 * tile] = Ublk;
-            a.lj_part[2 * tile + 1] = Wblk;
         }
+        if (kKick == 2) bin_wave_to(a.count, a.tmp, a.cell_cap, a.err, cell >= 0, id, cell);
+    }
+    if (kEnergy || kKick == 2) {
+        tile_reduce(Ublk, Wblk, tk, L);
+        tile_partials<kKick, false, kEnergy>(a, tile, rep, Ublk, Wblk, tk);
     }
 }
 
@@ -1151,24 +1320,21 @@ __device__ int stage_tile(const ForceArgs &a, const Tile &T, const RepDesc &R, i
     return M;
 }
 
-template <bool kSigma, bool kTypes, bool kDebug, bool kEnergy>
+template <bool kSigma, bool kTypes, bool kDebug, bool kEnergy, int kKick>
 __device__ void pair_tile(const ForceArgs &a, int tile, PairLds &L) {
     const Tile T = a.tiles[tile];
     const RepDesc R = a.reps[T.rep];
     const int W = T.w_flags & 0xffff;
     const bool wrapT = (T.w_flags >> 16) & 1;
     if (stage_tile<kTypes, kDebug>(a, T, R, W, L, nullptr, nullptr) < 0) {
-        if (threadIdx.x == 0) {
-            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
-            a.lj_part[2 * tile] = 0.0;
-            a.lj_part[2 * tile + 1] = 0.0;
-        }
+        atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW (its particles are not moved)
+        tile_partials<kKick, false, kEnergy>(a, tile, T.rep, 0.0, 0.0, TileKick{});
         return;
     }
     if (wrapT)
-        pair_tile_body<true, kSigma, kTypes, kDebug, kEnergy>(a, R, W, tile, L);
+        pair_tile_body<true, kSigma, kTypes, kDebug, kEnergy, kKick>(a, R, T.rep, W, tile, L);
     else
-        pair_tile_body<false, kSigma, kTypes, kDebug, kEnergy>(a, R, W, tile, L);
+        pair_tile_body<false, kSigma, kTypes, kDebug, kEnergy, kKick>(a, R, T.rep, W, tile, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1461,8 +1627,16 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // neighbours from LDS (the pair maths and accumulation of pair_tile_body;
 // the FP32 cutoff predicate on the current positions, so the interacting
 // set is the cell walk's whenever the list is complete).
-template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit, bool kDebug>
-__device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W, int tile,
+// Energies and virials (kEnergy) from two FP32 sums per particle, three
+// vector instructions per list entry: T = sum_hit tv with tv = s12 - s6 / 2
+// (the force's own factor) and Q = sum over all evaluated entries of
+// (hit ? s6 : kq), kq = shift / (2 eps).  With A = sum s12, B = sum s6 and
+// n interacting pairs, T = A - B / 2 and Q = B + kq (L - n) for L evaluated
+// entries, so U_i = 4 eps (A - B) + n shift = 4 eps T - 2 eps (Q - kq L)
+// and W_i = 48 eps (A - B / 2) = 48 eps T (femmd2D/src/interactionsCuda.cu:
+// 91-96).  kCount (eps = 0, or the dump): n is counted instead.
+template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit, bool kDebug, bool kCount, int kKick>
+__device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int rep, int W, int tile,
                                     const PairLds &L, const float2 *lp, const float *ls,
                                     const int *lid) {
     const int row = W + 2;
@@ -1472,10 +1646,13 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
     const float rc2_u = R.rc2_uniform;
+    // (not used when kCount: eps may be 0 there)
+    const float kq = kCount ? 0.0f : 2.0f * pe.shift / pe.e4;
     const uint32_t Nu = (uint32_t)a.n;
     const int capg = a.nbr_cap >> 2;
     const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
     double Ublk = 0.0, Wblk = 0.0;
+    TileKick tk;
     for (int l = threadIdx.x; l < nc; l += blockDim.x) {
         // list slot q holds the list of the tile's particle k (slots of a
         // chunk of blockDim particles are ordered longest list first)
@@ -1488,7 +1665,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
         const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the sentinel slot
-        float fx = 0.f, fy = 0.f, accA = 0.f, accB = 0.f;
+        float fx = 0.f, fy = 0.f, accT = 0.f, accQ = 0.f;
         int nint = 0;
         long long d1 = 0, d2 = 0;  // diagnostic dump: sums of (id + 1), (id + 1)^2
         // one list entry, without branches: the force of a non-interacting
@@ -1534,13 +1711,16 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             const float sr6 = sr4 * sr2;
             const float s12 = sr6 * sr6;
             const float tv = __builtin_fmaf(-0.5f, sr6, s12);
-            const float fc = hit ? tv * inv : 0.0f;
+            // (a non-interacting entry has r2a > 0, so its rcp is finite)
+            const float tvh = hit ? tv : 0.0f;
+            const float fc = tvh * inv;
             fx = __builtin_fmaf(fc, dxa, fx);
             fy = __builtin_fmaf(fc, dya, fy);
-            if (kEnergy || kDebug) nint += hit ? 1 : 0;
+            if (kCount) nint += hit ? 1 : 0;
             if (kEnergy) {
-                accA += hit ? s12 : 0.0f;
-                accB += hit ? sr6 : 0.0f;
+                accT += tvh;
+                if (!kCount) accQ += hit ? sr6 : kq;
+                else accQ += hit ? sr6 : 0.0f;
             }
             if (kDebug && hit) {
                 const long long v = (long long)lid[jo >> 3] + 1;
@@ -1581,12 +1761,14 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             group(cur);
         }
         const int id = a.perm[p];
-        a.fx[id] = fx * pe.e48;
-        a.fy[id] = fy * pe.e48;
+        tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx * pe.e48, fy * pe.e48, tk);
         if (kEnergy) {
-            const double A = accA, B = accB;
-            Ublk += 0.5 * ((double)pe.e4 * (A - B) + (double)nint * pe.shift);
-            Wblk += 0.5 * ((double)pe.e48 * (A - 0.5 * B));
+            const double T = accT, Q = accQ, e4 = pe.e4;
+            // kCount: Q = B exactly; otherwise Q - kq L = B - kq n
+            const double Ui = kCount ? e4 * (T - 0.5 * Q) + (double)nint * pe.shift
+                                     : e4 * T - 0.5 * e4 * (Q - (double)kq * (4.0 * n4));
+            Ublk += 0.5 * Ui;
+            Wblk += 0.5 * ((double)pe.e48 * T);
         }
         if (kDebug) {
             const int u = a.uid[id];
@@ -1595,29 +1777,21 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int W,
             a.dbg_s2[u] = d2;
         }
     }
-    if (kEnergy) {
-        block_sum2(Ublk, Wblk, L.red);
-        if (threadIdx.x == 0) {
-            a.lj_part[2 * tile] = Ublk;
-            a.lj_part[2 * tile + 1] = Wblk;
-        }
+    if (kEnergy || kKick == 2) {
+        tile_reduce(Ublk, Wblk, tk, L);
+        tile_partials<kKick, true, kEnergy>(a, tile, rep, Ublk, Wblk, tk);
     }
 }
 
-template <bool kSigma, bool kEnergy, bool kDebug>
+template <bool kSigma, bool kEnergy, bool kDebug, int kKick>
 __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 *lp,
                                float *ls, int *lid) {
     const Tile T = a.tiles[tile];
     const RepDesc R = a.reps[T.rep];
     const int W = T.w_flags & 0xffff;
     if (stage_layout(a, T, R, W, L) < 0) {
-        if (threadIdx.x == 0) {
-            atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW
-            if (kEnergy) {
-                a.lj_part[2 * tile] = 0.0;
-                a.lj_part[2 * tile + 1] = 0.0;
-            }
-        }
+        atomicOr(a.err, 2u);  // FM_DEV_TILE_OVERFLOW (its particles are not moved)
+        tile_partials<kKick, true, kEnergy>(a, tile, T.rep, 0.0, 0.0, TileKick{});
         return;
     }
     stage_copy(
@@ -1639,13 +1813,18 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
     }
     __syncthreads();
     const bool wrapT = (T.w_flags >> 16) & 1;
-#define FM_LPB(WR, U) list_tile_pass_body<WR, kSigma, kEnergy, U, kDebug>(a, R, W, tile, L, lp, ls, lid)
-    if (!kSigma && R.s2_uniform == 1.0f) {
-        if (wrapT) FM_LPB(true, true);
-        else FM_LPB(false, true);
+#define FM_LPB(WR, U, C) \
+    list_tile_pass_body<WR, kSigma, kEnergy, U, kDebug, C, kKick>(a, R, T.rep, W, tile, L, lp, ls, lid)
+    // hits are counted for the dump, and for the energy when eps = 0
+    if (kDebug || (kEnergy && L.pt[0].e4 == 0.0f)) {
+        if (wrapT) FM_LPB(true, false, true);
+        else FM_LPB(false, false, true);
+    } else if (!kSigma && R.s2_uniform == 1.0f) {
+        if (wrapT) FM_LPB(true, true, false);
+        else FM_LPB(false, true, false);
     } else {
-        if (wrapT) FM_LPB(true, false);
-        else FM_LPB(false, false);
+        if (wrapT) FM_LPB(true, false, false);
+        else FM_LPB(false, false, false);
     }
 #undef FM_LPB
 }
@@ -1782,13 +1961,21 @@ __device__ void bond_block(const ForceArgs &a, int b, double *red) {
     }
 }
 
-template <bool kSigma, bool kTypes, bool kDebug, bool kList, bool kEnergy>
-__global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
+// The force pass (femmd2D/src/interactions.cpp:83-109).  kKick: 0 forces
+// only; 1 forces and the closing second half-kick of a run; 2 forces, the
+// second half-kick of this step and the first half-kick, drift and wrap of
+// the next (femmd2D/src/integrator.cpp:8-29), for every particle without
+// element or bond corners (tile_kick).
+// (8 wavefronts per SIMD: measured faster than the 6 its registers would
+// otherwise allow, at the cost of one spilled register)
+template <bool kSigma, bool kTypes, bool kDebug, bool kList, bool kEnergy, int kKick>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_forces(ForceArgs a) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
     __shared__ int s_off[kMaxSlots + 1];
     __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
-    __shared__ double s_red[2 * 4];
+    __shared__ double s_red[3 * 4];
+    __shared__ float s_redf[2 * 4];
     int b = blockIdx.x;
     // the previous step's observables, level 2 (one thread per replica)
     if (a.obs_l2 && threadIdx.x == 0)
@@ -1832,10 +2019,11 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             L.off = s_off;
             L.pt = s_pt;
             L.red = s_red;
+            L.redf = s_redf;
             float2 *lp = reinterpret_cast<float2 *>(dyn);
             float *ls = reinterpret_cast<float *>(lp + a.lds_cap + 1);
             int *lid = reinterpret_cast<int *>(ls + (kSigma ? a.lds_cap + 1 : 0));
-            list_tile_pass<kSigma, kEnergy, kDebug>(a, xcd_remap(b, a.ntiles), L, lp, ls, lid);
+            list_tile_pass<kSigma, kEnergy, kDebug, kKick>(a, xcd_remap(b, a.ntiles), L, lp, ls, lid);
         }
     } else if (b < a.ntiles) {
         PairLds L;
@@ -1847,7 +2035,8 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         L.off = s_off;
         L.pt = s_pt;
         L.red = s_red;
-        pair_tile<kSigma, kTypes, kDebug, kEnergy>(a, xcd_remap(b, a.ntiles), L);
+        L.redf = s_redf;
+        pair_tile<kSigma, kTypes, kDebug, kEnergy, kKick>(a, xcd_remap(b, a.ntiles), L);
     }
 }
 
@@ -1879,38 +2068,41 @@ __global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
     oy[p] = y;
 }
 
-// Observables of the current step for every replica (workgroup r): both
-// levels in one workgroup, with the same chunks and order as the per-step
-// path.  accumulate: add them to the running sums.
-__global__ __launch_bounds__(256) void k_observe(ObsArgs o, int accumulate) {
+// Observables of the current step for every replica, with the same chunks
+// and order as the per-step path: level 1 (one workgroup per chunk), then
+// level 2 (one thread per replica); accumulate: add them to the running sums.
+__global__ __launch_bounds__(256) void k_observe_l1(ObsArgs o) {
     __shared__ double red[8];
-    for (int r = blockIdx.x; r < o.nrep; r += gridDim.x) {
-        for (int c = 0; c < kObsChunks; ++c) obs_chunk(o, r, c, red);
-        if (threadIdx.x == 0) obs_finish(o, r, accumulate != 0);
-    }
+    obs_level1(o, red);
+}
+__global__ __launch_bounds__(64) void k_observe_l2(ObsArgs o, int accumulate) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < o.nrep) obs_finish(o, r, accumulate != 0);
 }
 
 hipError_t launch_observe(const ObsArgs &o, bool accumulate, hipStream_t s) {
     if (o.nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_observe, dim3(std::min(o.nrep, 4096)), dim3(256), 0, s, o,
+    hipLaunchKernelGGL(k_observe_l1, dim3(std::min(o.nrep * kObsChunks, 4096)), dim3(256), 0, s,
+                       o);
+    hipLaunchKernelGGL(k_observe_l2, dim3((o.nrep + 63) / 64), dim3(64), 0, s, o,
                        accumulate ? 1 : 0);
     return hipGetLastError();
 }
 
 // Start a new list epoch (after a forced rebuild), which is counted.
-__global__ void k_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax) {
+// (nslots displacement slots are cleared: every set of the pair)
+__global__ void k_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, int nslots) {
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
         ctl[r].flag = 0;
         ctl[r].disp = 0.0f;
         ctl[r].nbuild += 1u;
     }
-    for (int k = threadIdx.x; k < nrep * kDispSlots; k += blockDim.x)
-        dmax[(size_t)k * kDispStride] = 0u;
+    for (int k = threadIdx.x; k < nslots; k += blockDim.x) dmax[(size_t)k * kDispStride] = 0u;
 }
 
-hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, hipStream_t s) {
+hipError_t launch_ctl_reset(Ctl *ctl, int nrep, uint32_t *dmax, int nslots, hipStream_t s) {
     if (nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s, ctl, nrep, dmax);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(256), 0, s, ctl, nrep, dmax, nslots);
     return hipGetLastError();
 }
 
@@ -2819,38 +3011,54 @@ size_t forces_lds_bytes(int lds_cap, bool types, bool debug) {
            (debug ? lds_cap * sizeof(int) : 0);
 }
 
-hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int block,
-                         const ForceArgs &a, hipStream_t s) {
+hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool energy, int kick,
+                         int block, const ForceArgs &a, hipStream_t s) {
+    // kick: 0 forces only, 1 with the closing half-kick, 2 with the step's
+    // kick and drift (the closing and step kicks always form the kinetic
+    // energy when energy is set; the dump pass has no kick)
+    const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
+    if (nblk <= 0) return hipSuccess;
+    dim3 g(nblk), b(block);
+    if (debug) kick = 0;
+    if (kick == 1) energy = true;
     if (list) {
         // neighbour lists: single type (the caller checks); debug: the
         // interacting-set dump over the lists (with energies)
-        const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
-        if (nblk <= 0) return hipSuccess;
-        dim3 g(nblk), b(block);
         const size_t lds = (size_t)(a.lds_cap + 1) *
                            (sizeof(float2) + (sigma ? sizeof(float) : 0) + (debug ? sizeof(int) : 0));
-#define FM_LIST(S, D, E) FM_GO((k_forces<S, false, D, true, E>), g, b, lds, s, a)
-        if (debug) { if (sigma) FM_LIST(true, true, true); else FM_LIST(false, true, true); }
-        else if (sigma) { if (energy) FM_LIST(true, false, true); else FM_LIST(true, false, false); }
-        else { if (energy) FM_LIST(false, false, true); else FM_LIST(false, false, false); }
+#define FM_LIST(S, D, E, K) FM_GO((k_forces<S, false, D, true, E, K>), g, b, lds, s, a)
+#define FM_LIST_K(S, E)                    \
+    do {                                   \
+        if (kick == 2) FM_LIST(S, false, E, 2); \
+        else if (kick == 1) FM_LIST(S, false, true, 1); \
+        else FM_LIST(S, false, E, 0);      \
+    } while (0)
+        if (debug) { if (sigma) FM_LIST(true, true, true, 0); else FM_LIST(false, true, true, 0); }
+        else if (sigma) { if (energy) FM_LIST_K(true, true); else FM_LIST_K(true, false); }
+        else { if (energy) FM_LIST_K(false, true); else FM_LIST_K(false, false); }
+#undef FM_LIST_K
 #undef FM_LIST
         return hipGetLastError();
     }
-    const int nblk = a.ntiles + a.nespans + a.nbspans > 0 ? forces_grid(a) : 0;
-    if (nblk <= 0) return hipSuccess;
     const size_t lds = forces_lds_bytes(a.lds_cap, types, debug);
-    dim3 g(nblk), b(block);
-#define FM_LAUNCH(S, T, D, E) FM_GO((k_forces<S, T, D, false, E>), g, b, lds, s, a)
+#define FM_LAUNCH(S, T, D, E, K) FM_GO((k_forces<S, T, D, false, E, K>), g, b, lds, s, a)
+#define FM_LAUNCH_K(S, T, E)                    \
+    do {                                        \
+        if (kick == 2) FM_LAUNCH(S, T, false, E, 2); \
+        else if (kick == 1) FM_LAUNCH(S, T, false, true, 1); \
+        else FM_LAUNCH(S, T, false, E, 0);      \
+    } while (0)
     if (debug) {  // the dump pass always accumulates energies
-        if (sigma) { if (types) FM_LAUNCH(true, true, true, true); else FM_LAUNCH(true, false, true, true); }
-        else { if (types) FM_LAUNCH(false, true, true, true); else FM_LAUNCH(false, false, true, true); }
+        if (sigma) { if (types) FM_LAUNCH(true, true, true, true, 0); else FM_LAUNCH(true, false, true, true, 0); }
+        else { if (types) FM_LAUNCH(false, true, true, true, 0); else FM_LAUNCH(false, false, true, true, 0); }
     } else if (energy) {
-        if (sigma) { if (types) FM_LAUNCH(true, true, false, true); else FM_LAUNCH(true, false, false, true); }
-        else { if (types) FM_LAUNCH(false, true, false, true); else FM_LAUNCH(false, false, false, true); }
+        if (sigma) { if (types) FM_LAUNCH_K(true, true, true); else FM_LAUNCH_K(true, false, true); }
+        else { if (types) FM_LAUNCH_K(false, true, true); else FM_LAUNCH_K(false, false, true); }
     } else {
-        if (sigma) { if (types) FM_LAUNCH(true, true, false, false); else FM_LAUNCH(true, false, false, false); }
-        else { if (types) FM_LAUNCH(false, true, false, false); else FM_LAUNCH(false, false, false, false); }
+        if (sigma) { if (types) FM_LAUNCH_K(true, true, false); else FM_LAUNCH_K(true, false, false); }
+        else { if (types) FM_LAUNCH_K(false, true, false); else FM_LAUNCH_K(false, false, false); }
     }
+#undef FM_LAUNCH_K
 #undef FM_LAUNCH
     return hipGetLastError();
 }
