@@ -93,20 +93,37 @@ def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None, uniform_mass=False)
 REBUILD_KERNELS = ("bin", "sort", "list_build")
 
 
-def compulsory_bytes(n, ncells, ntri, nbond, n_fea_nodes, n_bonded_nodes, excl=True):
+def compulsory_bytes(n, ncells, ntri, nbond, n_fea_nodes, n_bonded_nodes, excl=True,
+                     n_corner=0, uniform_mass=True):
     """The survey's compulsory bytes (each input array read once, each output
     written once, in its own layout-independent units: FP32 values, int32
     indices, no neighbour list), per force pass and per step.
     force pass: LJ r sorted x,y, perm (+ exclusion id), w fx,fy: 20 (+4)/particle,
                 r start 4/cell; CST r 3 ids, Bm, A0, E, 3 node x,y, w 3 corners:
                 84/triangle; bond r ids, k, r0, 2 node x,y, w 2 ends: 40/bond
+    fused force pass (the kernel launched): the same, except that for each
+    of the n - n_corner particles without element or bond corners it also
+                kicks and drifts: instead of w fx,fy it r vx,vy (+ inv_m when
+                masses differ), w x,y,vx,vy: +16 (+4)/particle
     step:       128/particle (124 without exclusions) + 12/cell + 120/triangle
                 + 20/element node + 64/bond + 20/bonded node"""
     per_p = 24 if excl else 20
     forces = per_p * n + 4 * ncells + 84 * ntri + 40 * nbond
+    fused = forces + (n - n_corner) * (16 if uniform_mass else 20)
     step = ((128 if excl else 124) * n + 12 * ncells + 120 * ntri + 20 * n_fea_nodes
             + 64 * nbond + 20 * n_bonded_nodes)
-    return {"forces": forces, "step": step}
+    return {"forces": forces, "forces_fused": fused, "step": step}
+
+
+def fused_corner_particles(w) -> int:
+    """Particles with element or bond corners (integrated by the node kick,
+    not by the force pass)."""
+    import numpy as np
+    ga = w.global_arrays()
+    u = [np.asarray(ga["tri"], np.int64).ravel()]
+    if w.n_bonds:
+        u += [np.asarray(w.bonds[0], np.int64), np.asarray(w.bonds[1], np.int64)]
+    return int(np.unique(np.concatenate(u)).size) if sum(a.size for a in u) else 0
 
 
 def measured_traffic(kernel: str, mode: str):
@@ -177,46 +194,121 @@ def batched_leg(steps: int, loop_steps: int, tune_cache, looped_leg: bool = True
             "speedup": batched / looped if looped else None, "device_errors": errs}
 
 
-def adjoint_leg(steps: int):
-    """C5: elastic sheet (128 x 256 nodes, E in [20, 120], bottom row pinned)
-    in 200,000 WCA solvent.  Discrete adjoint of `steps` steps (forward,
-    backward sweep) of the loss over the free sheet nodes, against the same
-    steps forward: with the whole trajectory stored (it fits in HBM), and
-    with checkpoints every sqrt(steps) steps and recomputation."""
+def c3_leg(steps: int, warmup: int, skin: float):
+    """C3: 4,096 disks (r log-uniform in [4, 8], about 400k FEA nodes and
+    700k CST elements, per-element E and nu) in 600k LJ solvent at density
+    0.75, each disk's rim bonded to 8 anchor solvent particles (k = 100,
+    r0 = 1), dt = 0.002: the element- and bond-gather-heavy workload.
+    Timed like the headline (warm graphs, observables every step), with the
+    per-kernel device times, the step's compulsory bytes and a run-twice
+    bitwise check of the gathers (two fresh systems, same steps)."""
     import numpy as np
     from femmd_old_amd import configs
-    w = configs.c5()
-    s = configs.build_system(w)
-    ga = w.global_arrays()
-    nodes = np.arange(w.n_solvent, w.n_total)
-    ids = nodes[np.isfinite(ga["mass"][nodes])].astype(np.int32)
-    x, y, _, _ = s.get_state()
-    tgt = np.stack([x[ids] + 0.3, y[ids]], 1)
-    ck_sqrt = max(1, int(round(steps ** 0.5)))
-    s.run(steps)  # warm-up: graph capture
-    s.adjoint(steps, ids, tgt, ckpt=ck_sqrt)  # warm-up: adjoint buffers
+    t_gen = time.time()
+    w = configs.c3()
+    s = configs.build_system(w, skin=skin)
+    print(f"[bench] workload c3: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} bonds={w.n_bonds} "
+          f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
+    ncx, ncy, _, _, _ = s.cells()
+    s.run(max(warmup, 1))
+    ls0 = s.list_stats()
     t0 = time.perf_counter()
     s.run(steps)
     t1 = time.perf_counter()
-    g = s.adjoint(steps, ids, tgt)  # whole trajectory stored (fits in HBM)
+    ls1 = s.list_stats()
+    secs = t1 - t0
+    prof = s.profile(20)
+    ga = w.global_arrays()
+    bonded = len(set(np.asarray(w.bonds[0], int).tolist()) |
+                 set(np.asarray(w.bonds[1], int).tolist())) if w.n_bonds else 0
+    cb = compulsory_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds, int(w.n_nodes), bonded,
+                          excl=True, n_corner=fused_corner_particles(w))
+    ms_step = 1e3 * secs / steps
+    # run twice: two fresh systems, the same steps, bitwise the same state
+    a = configs.build_system(w, skin=skin)
+    b = configs.build_system(w, skin=skin)
+    a.run(50)
+    b.run(50)
+    same = all(np.array_equal(u, v) for u, v in zip(a.get_state(), b.get_state()))
+    ea, eb = a.energies(), b.energies()
+    same = same and all(ea[k] == eb[k] for k in ("u_lj", "u_fea", "u_bond", "w_fea", "w_bond"))
+    errs = s.device_errors()
+    list_mode = ls1["rebuilds"] > 0
+    print(f"[bench] c3: {steps} steps in {secs:.3f}s ({ms_step:.4f} ms/step); run twice bitwise "
+          f"{same}", file=sys.stderr, flush=True)
+    return {"workload": "c3: 600,000 LJ (rho 0.75) + 4,096 CST disks r in [4, 8], rim bonded to "
+                        "8 anchors each (k 100, r0 1), dt 0.002",
+            "particles": w.n_total, "triangles": w.n_tri, "bonds": w.n_bonds,
+            "steps": steps, "ms_per_step": ms_step,
+            "value": w.n_total * steps / secs, "unit": "particle-steps/s",
+            "pair_path": "list" if list_mode else "cells",
+            "steps_per_rebuild": round(steps / max(ls1["rebuilds"] - ls0["rebuilds"], 1), 2)
+            if list_mode else None,
+            "step_kernels_ms": {k: round(v, 5) for k, v in prof.items()},
+            "forces_compulsory_bytes": cb["forces_fused"],
+            "forces_frac": round(cb["forces_fused"] / (prof["forces"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 4),
+            "step_compulsory_bytes": cb["step"],
+            "step_frac": round(cb["step"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "run_twice_bitwise": same, "device_errors": errs}
+
+
+def adjoint_leg(steps: int, ckpt: int):
+    """C5: elastic sheet (128 x 256 nodes, E a smooth random field in
+    [20, 120], bottom row pinned) in 200,000 WCA solvent.  The loss is the
+    MSE between the final free-node positions and a target rollout from the
+    same initial state with E perturbed by 10 % (a second smooth field).
+    Discrete adjoint of `steps` steps (forward, backward sweep) with a
+    checkpoint every `ckpt` steps and recomputation, and with the whole
+    trajectory stored (it fits in HBM), against the same steps forward."""
+    import numpy as np
+    from femmd_old_amd import configs
+    w = configs.c5()
+    st = configs.build_system(configs.with_params(w, E=configs.c5_target_E(w)))
+    st.run(steps)
+    ga = w.global_arrays()
+    nodes = np.arange(w.n_solvent, w.n_total)
+    ids = nodes[np.isfinite(ga["mass"][nodes])].astype(np.int32)
+    xt, yt, _, _ = st.get_state()
+    tgt = np.stack([xt[ids], yt[ids]], 1)
+    st.close()
+    s = configs.build_system(w)
+    s0 = s.get_state()
+    s.run(steps)  # warm-up: graph capture
+    s.set_state(*s0)
+    s.adjoint(steps, ids, tgt, ckpt=ckpt)  # warm-up: adjoint buffers and graphs
+    s.set_state(*s0)
+    s.adjoint(steps, ids, tgt)
+    s.set_state(*s0)
+    t0 = time.perf_counter()
+    s.run(steps)
+    t1 = time.perf_counter()
+    s.set_state(*s0)
     t2 = time.perf_counter()
-    s.adjoint(steps, ids, tgt, ckpt=ck_sqrt)  # sqrt(steps) checkpoints, recompute
+    g = s.adjoint(steps, ids, tgt, ckpt=ckpt)
     t3 = time.perf_counter()
-    fwd, adj, adj_ck = t1 - t0, t2 - t1, t3 - t2
+    s.set_state(*s0)
+    t4 = time.perf_counter()
+    gs = s.adjoint(steps, ids, tgt)  # whole trajectory stored
+    t5 = time.perf_counter()
+    fwd, adj_ck, adj = t1 - t0, t3 - t2, t5 - t4
     errs = s.device_errors()
     print(f"[bench] c5 adjoint: N={w.n_total} tri={w.n_tri} {steps} steps: forward {fwd:.3f}s, "
-          f"adjoint {adj:.3f}s, with checkpoints every {ck_sqrt} {adj_ck:.3f}s",
-          file=sys.stderr, flush=True)
-    return {"workload": "c5: 128x256-node CST sheet (64,770 tri) in 200,000 WCA solvent",
-            "particles": w.n_total, "steps": steps,
-            "adjoint_particle_steps_per_s": w.n_total * steps / adj,
-            "adjoint_ms_per_step": 1e3 * adj / steps,
+          f"checkpoint every {ckpt} {adj_ck:.3f}s ({adj_ck / fwd:.2f}x), whole trajectory "
+          f"stored {adj:.3f}s ({adj / fwd:.2f}x), loss {g['loss']:.6e}", file=sys.stderr, flush=True)
+    return {"workload": "c5: 128x256-node CST sheet (64,770 tri) in 200,000 WCA solvent; "
+                        "MSE to a target rollout with E x (1 + 0.1 field2)",
+            "particles": w.n_total, "steps": steps, "checkpoint_every": ckpt,
+            "adjoint_particle_steps_per_s": w.n_total * steps / adj_ck,
+            "adjoint_ms_per_step": 1e3 * adj_ck / steps,
             "forward_ms_per_step": 1e3 * fwd / steps,
-            "adjoint_over_forward": adj / fwd,
-            "checkpointed": {"checkpoint_every": ck_sqrt,
-                             "adjoint_ms_per_step": 1e3 * adj_ck / steps,
-                             "adjoint_over_forward": adj_ck / fwd},
+            "adjoint_over_forward": adj_ck / fwd,
+            "trajectory_stored": {"adjoint_ms_per_step": 1e3 * adj / steps,
+                                  "adjoint_over_forward": adj / fwd},
             "loss": g["loss"], "grad_norm_E": float(np.linalg.norm(g["dE"])),
+            "deps": g["deps"], "dsigma": g["dsigma"],
+            "stored_vs_checkpointed_max_rel_dE": float(np.abs(gs["dE"] - g["dE"]).max() /
+                                                       max(np.abs(g["dE"]).max(), 1e-300)),
             "device_errors": errs}
 
 
@@ -319,12 +411,15 @@ def main():
     ap.add_argument("--skin", type=float, default=0.6,
                     help="neighbour-list skin (0: walk the cells every step)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (5 launches), the "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--copy-bw", action="store_true",
                     help="only measure the device copy bandwidth (1 GiB, JSON line)")
-    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-ckpt", type=int, default=20)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (tests)")
     args = ap.parse_args()
 
@@ -394,9 +489,11 @@ def main():
     n_nodes = int(w.n_nodes)
     bonded = len(set(np.asarray(w.bonds[0], int).tolist()) |
                  set(np.asarray(w.bonds[1], int).tolist())) if w.n_bonds else 0
+    uniform = bool(np.isfinite(m[0]) and m[0] > 0 and np.all(m == m[0]))
     cb = compulsory_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds, n_nodes, bonded,
-                          excl=bool(w.elastomers))
-    achieved = cb[kern] / (prof[kern] * 1e-3) / 1e9
+                          excl=bool(w.elastomers), n_corner=fused_corner_particles(w),
+                          uniform_mass=uniform)
+    achieved = cb["forces_fused"] / (prof[kern] * 1e-3) / 1e9
     model_achieved = ab[kern] / (prof[kern] * 1e-3) / 1e9
     step_bytes = ab["kick"] + ab["forces"] + rebuild_frac * sum(ab[k] for k in REBUILD_KERNELS)
     mode = "list" if list_mode else "cells"
@@ -405,10 +502,11 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(f"k_{kern}", mode) if args.scale == 1.0 else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)",
-            "bytes_per_launch": cb[kern],
-            "bytes_rule": "compulsory bytes of the pass (each input read once, each output "
-                          "written once, no neighbour list): 24/particle + 4/cell + "
-                          "84/triangle + 40/bond",
+            "bytes_per_launch": cb["forces_fused"],
+            "bytes_rule": "compulsory bytes of the fused force pass (each input read once, "
+                          "each output written once, no neighbour list): 24/particle + "
+                          "4/cell + 84/triangle + 40/bond, + 16 per particle it kicks and "
+                          "drifts (r vx,vy, w x,y,vx,vy in place of w fx,fy)",
             "launch_ms": round(prof[kern], 5),
             "layout_bytes_per_launch": ab[kern],
             "layout_achieved": round(model_achieved, 1),
@@ -472,8 +570,10 @@ def main():
         del s
         out["batched"] = batched_leg(args.batched_steps, max(4, args.batched_steps // 10),
                                      args.tune_cache, not args.no_looped)
+    if not args.no_c3:
+        out["c3"] = c3_leg(args.c3_steps, max(args.warmup, 1), args.skin)
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg(args.adjoint_steps)
+        out["adjoint"] = adjoint_leg(args.adjoint_steps, args.adjoint_ckpt)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, args.cpu_threads)
         out["cpu_baseline"]["host"] = host_info()

@@ -493,6 +493,25 @@ class System:
                                            _dp(dE), C.byref(deps), C.byref(dsig)))
         return jx, jy, dE, deps.value, dsig.value
 
+    def adjoint_forward(self, nsteps: int, ckpt: int = 0):
+        """fm_adjoint_forward: nsteps steps (the state advances), recorded
+        for adjoint_backward (checkpoints every ckpt steps; 0: the engine's
+        choice, the whole trajectory when it fits)."""
+        self._check(self._lib.fm_adjoint_forward(self._h, int(nsteps), int(ckpt)))
+
+    def adjoint_backward(self, dx=None, dy=None) -> dict:
+        """fm_adjoint_backward: the gradient {"dE", "deps", "dsigma"} of a
+        loss whose derivative with respect to the final positions is
+        (dx, dy) (caller order), through the recorded forward pass."""
+        n = self.N
+        dx = None if dx is None else _f32(dx, n)
+        dy = None if dy is None else _f32(dy, n)
+        deps, dsig = C.c_double(), C.c_double()
+        dE = np.empty(self._nelem, np.float64)
+        self._check(self._lib.fm_adjoint_backward(self._h, _fp(dx), _fp(dy), _dp(dE),
+                                                  C.byref(deps), C.byref(dsig)))
+        return {"dE": dE, "deps": deps.value, "dsigma": dsig.value}
+
     def adjoint(self, nsteps: int, ids, targets, ckpt: int = 0) -> dict:
         """Runs nsteps forward and returns the loss
         (1/K) sum_k |x_{ids[k]}(T) - targets[k]|^2 with its gradient

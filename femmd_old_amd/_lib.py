@@ -137,6 +137,9 @@ PROTOTYPES = {
     "fm_force_jvp": (C.c_int, [_h, _f, _f, _f, _f, _d, _d, _d]),
     "fm_adjoint": (C.c_int, [_h, C.c_int32, C.c_int32, C.c_int32, _i, _f, _f, _d, _d, _d,
                              _d]),
+    "fm_adjoint_forward": (C.c_int, [_h, C.c_int32, C.c_int32]),
+    "fm_adjoint_backward": (C.c_int, [_h, _f, _f, _d, _d, _d]),
+    "fm_adjoint_run": (C.c_int, [_h, C.c_int32, C.c_int32, _f, _f, _d, _d, _d]),
     "fm_batch_create": (C.c_int, [C.c_int32, C.POINTER(_h), C.POINTER(_h)]),
     "fm_batch_destroy": (None, [_h]),
     "fm_batch_last_error": (C.c_char_p, [_h]),

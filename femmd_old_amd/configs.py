@@ -488,6 +488,36 @@ def c5(seed: int = 5, scale: float = 1.0, nx: int = 128, ny: int = 256) -> Workl
     return w
 
 
+def c5_target_E(w: Workload) -> np.ndarray:
+    """The per-element Young's moduli of C5's target rollout: the workload's
+    field E times (1 + 0.1 field2), field2 a second smooth random field of
+    16 Fourier modes drawn from the same seed (after the first), at the
+    element centroids."""
+    el = w.elastomers[0]
+    rng = np.random.default_rng()
+    rng.bit_generator.state = w._field_rng_state
+    cent = (el.verts[el.tris] + np.array([el.cx, el.cy])).mean(1)
+    field2 = fourier_field(rng, cent[:, 0], cent[:, 1], w.Lx, w.Ly)
+    return np.asarray(el.E, np.float64) * (1.0 + 0.1 * field2)
+
+
+def with_params(w: Workload, E=None, eps=None, sigma=None) -> Workload:
+    """A copy of workload w with other per-element Young's moduli E (of its
+    single elastomer), pair eps and a common sigma for every particle."""
+    import copy
+    v = copy.copy(w)
+    v.elastomers = [copy.copy(el) for el in w.elastomers]
+    if E is not None:
+        v.elastomers[0].E = np.asarray(E, np.float64).copy()
+    if eps is not None:
+        v.eps = float(eps)
+    if sigma is not None:
+        v.sigma = np.full(w.n_solvent, float(sigma))
+        for el in v.elastomers:
+            el.sigma = float(sigma)
+    return v
+
+
 # ---------------------------------------------------------------------------
 # engine construction
 # ---------------------------------------------------------------------------

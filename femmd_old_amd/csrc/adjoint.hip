@@ -209,13 +209,21 @@ int Engine::adjoint_op(const float *wx, const float *wy, float *jx, float *jy, d
     return FM_OK;
 }
 
-int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
-                        const float *ty, double *loss, double *dE, double *deps, double *dsig,
-                        std::string &err) {
-    if (nsteps < 1 || nloss < 1 || !ids || !tx || !ty) {
-        err = "nsteps and the loss set must be non-empty";
-        return FM_ERR_ARG;
-    }
+// Graphs of recorded segments (adj_record_segment) carry this bit in their
+// key (step-body graphs use bits 0..14); segments longer than
+// kMaxGraphSegment run eagerly.
+constexpr int kSegmentGraphKey = 1 << 20;
+constexpr int kMaxGraphSegment = 256;
+
+// ---------------------------------------------------------------------------
+// Forward part of an adjoint: nsteps steps from the current state, with a
+// checkpoint (x, v) at every segment start and the positions of the last
+// segment recorded as they are produced.  The state advances as with step().
+// The backward sweep (adjoint_sweep) then needs xb = dL/dx_N, vb = 0 seeded.
+// ---------------------------------------------------------------------------
+int Engine::adjoint_forward(int nsteps, int ckpt, std::string &err) {
+    if (nsteps < 1) { err = "nsteps must be >= 1"; return FM_ERR_ARG; }
+    adj_.valid = false;
     int rc = check(hipSetDevice(device_), "hipSetDevice", err);
     if (rc) return rc;
     rc = ensure_order(err);
@@ -235,43 +243,28 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         ckpt = (int)std::max<size_t>(1, std::min<size_t>(fit, (size_t)nsteps));
     }
     ckpt = std::min(ckpt, nsteps);
-    const std::vector<int32_t> &iid = iid_h_[0];
-    std::vector<int32_t> lid(nloss);
-    std::vector<float2> tgt(nloss);
-    for (int k = 0; k < nloss; ++k) {
-        if (ids[k] < 0 || ids[k] >= N) { err = "loss particle out of range"; return FM_ERR_ARG; }
-        lid[k] = iid[ids[k]];
-        tgt[k] = make_float2(tx[k], ty[k]);
-    }
     const int S = (nsteps + ckpt - 1) / ckpt;
     hipError_t he;
     if (ckpt_.n < (size_t)(S + 1) * 4 * N && (he = ckpt_.alloc((size_t)(S + 1) * 4 * N)) != hipSuccess)
         return check(he, "alloc checkpoints", err);
-    if (hist_.n < (size_t)ckpt * 2 * N && (he = hist_.alloc((size_t)ckpt * 2 * N)) != hipSuccess)
-        return check(he, "alloc history", err);
-    if ((he = loss_ids_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
-    if ((he = loss_t_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
-    const int nlb = (nloss + 255) / 256;
-    if ((he = loss_part_.alloc(nlb)) != hipSuccess) return check(he, "alloc", err);
-    (void)hipMemcpyAsync(loss_ids_.p, lid.data(), nloss * sizeof(int32_t), hipMemcpyHostToDevice,
-                         stream_);
-    (void)hipMemcpyAsync(loss_t_.p, tgt.data(), nloss * sizeof(float2), hipMemcpyHostToDevice,
-                         stream_);
+    if (hist_.n < (size_t)ckpt * 2 * N) {
+        // the recording graphs hold the history's address: drop them with it
+        for (auto it = graphs_.begin(); it != graphs_.end();) {
+            if (it->first & kSegmentGraphKey) {
+                (void)hipGraphExecDestroy(it->second);
+                it = graphs_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if ((he = hist_.alloc((size_t)ckpt * 2 * N)) != hipSuccess)
+            return check(he, "alloc history", err);
+    }
     rc = ensure_primed(err);
     if (rc) return rc;
-    const size_t fb = (size_t)N * sizeof(float);
-    // positions x_n of step k of a segment -> hist slot k
-    auto record = [&](int k) -> int {
-        float *hx = hist_.p + (size_t)k * 2 * N;
-        if (hipMemcpyAsync(hx, x_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess ||
-            hipMemcpyAsync(hx + N, y_.p, fb, hipMemcpyDeviceToDevice, stream_) != hipSuccess)
-            return -1;
-        return 0;
-    };
     // ---- forward, checkpoint at every segment start and at the end; the
     // last segment records its positions as it goes (no energies needed) ----
     obs_valid_ = false;
-    energy_ = false;
     for (int s = 0; s < S; ++s) {
         if (list_ && S > 1) {
             // a fresh neighbour list at every checkpoint, as the recomputation
@@ -282,46 +275,90 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         }
         if (save_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "ckpt", err);
         const int len = std::min(ckpt, nsteps - s * ckpt);
-        const bool last = s == S - 1;
-        if (last && record(0)) return check(hipGetLastError(), "history", err);
-        // the drift of body k records x_{k+1} into history slot k + 1
-        rec_ = (last && len > 1) ? hist_.p + (size_t)2 * N : nullptr;
+        if (s == S - 1) {
+            if (adj_record_segment(len)) return check(hipGetLastError(), "step", err);
+        } else {
+            // graphs of the step without observables: open, bodies, close
+            int bad = launch_bodies(BodyKind{1, false, true, false, false, false, false});
+            for (int k = 1; k < len && !bad; ++k)
+                bad = launch_bodies(BodyKind{1, true, true, false, false, false, false});
+            if (!bad) bad = launch_bodies(BodyKind{1, true, false, false, false, false, false});
+            if (bad) return check(hipGetLastError(), "step", err);
+        }
+    }
+    steps_ += nsteps;
+    if (save_state(ckpt_.p + (size_t)S * 4 * N)) return check(hipGetLastError(), "ckpt", err);
+    rc = sync(err);
+    if (rc) return rc;
+    adj_.valid = true;
+    adj_.nsteps = nsteps;
+    adj_.ckpt = ckpt;
+    adj_.S = S;
+    adj_.steps_at_end = steps_;
+    return FM_OK;
+}
+
+// The positions x_0 .. x_{len-1} of a segment into history slots 0 .. len-1
+// (slot 0 before the first drift; the drift of body k writes x_{k+1} into
+// slot k + 1) while the segment is stepped from the current state, and the
+// closing half-kick.  Every launch of the segment is one graph, captured the
+// first time a segment of this length is recorded (the slots are the same
+// buffers for every segment).
+int Engine::adj_record_segment(int len, bool close) {
+    auto body = [&]() -> int {
+        if (launch_copy2(hist_.p, x_.p, y_.p, N_, stream_) != hipSuccess) return -1;  // slot 0
+        rec_ = len > 1 ? hist_.p + (size_t)2 * N_ : nullptr;
         int bad = launch_step_body(false, true);
         for (int k = 1; k < len && !bad; ++k) {
-            rec_ = (last && k + 1 < len) ? hist_.p + (size_t)(k + 1) * 2 * N : nullptr;
+            // the drift of body k records x_{k+1} into slot k + 1
+            rec_ = (k + 1 < len) ? hist_.p + (size_t)(k + 1) * 2 * N_ : nullptr;
             bad = launch_step_body(true, true);
         }
         rec_ = nullptr;
-        if (bad) return check(hipGetLastError(), "step", err);
-        if (launch_step_body(true, false)) return check(hipGetLastError(), "kick", err);
+        if (!bad && close) bad = launch_step_body(true, false);
+        return bad;
+    };
+    // a long segment (the whole trajectory stored) runs once per forward
+    // pass: captured, it would only cost the capture
+    if (!use_graph_ || len > kMaxGraphSegment) return body();
+    const int key = kSegmentGraphKey | (len << 1) | (close ? 1 : 0) | (par_ << 19);
+    hipGraphExec_t ex = nullptr;
+    for (auto &g : graphs_)
+        if (g.first == key) ex = g.second;
+    const int par0 = par_;
+    const bool pend0 = bins_pending_;
+    if (!ex) {
+        if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
+        const int bad = body();
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(stream_, &g);
+        par_ = par0;  // nothing ran: the launch below sets the host state
+        bins_pending_ = pend0;
+        if (bad || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return -1;
+        }
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return -1;
+        graphs_.emplace_back(key, ex);
     }
-    energy_ = true;
-    steps_ += nsteps;
-    float *final_state = ckpt_.p + (size_t)S * 4 * N;
-    if (save_state(final_state)) return check(hipGetLastError(), "ckpt", err);
-    // ---- loss and its gradient ----
-    (void)hipMemsetAsync(xbx_.p, 0, fb, stream_);
-    (void)hipMemsetAsync(xby_.p, 0, fb, stream_);
-    (void)hipMemsetAsync(vbx_.p, 0, fb, stream_);
-    (void)hipMemsetAsync(vby_.p, 0, fb, stream_);
-    (void)hipMemsetAsync(dE_.p, 0, dE_.n * sizeof(double), stream_);
-    (void)hipMemsetAsync(pacc_.p, 0, pacc_.n * sizeof(double), stream_);
-    LossArgs la{};
-    la.reps = reps_.p;
-    la.n = nloss;
-    la.ids = loss_ids_.p;
-    la.target = loss_t_.p;
-    la.x = x_.p;
-    la.y = y_.p;
-    la.xbx = xbx_.p;
-    la.xby = xby_.p;
-    la.scale = 2.0f / (float)nloss;
-    la.part = loss_part_.p;
-    if (launch_adj_loss(la, stream_) != hipSuccess) return check(hipGetLastError(), "loss", err);
-    std::vector<double> lp(nlb);
-    (void)hipMemcpyAsync(lp.data(), loss_part_.p, nlb * sizeof(double), hipMemcpyDeviceToHost,
-                         stream_);
-    // ---- backward sweep, segment by segment ----
+    if (hipGraphLaunch(ex, stream_) != hipSuccess) return -1;
+    // the host state the segment leaves: one drift per step of it (each
+    // flips the list-mode parity), bins pending after a drift (cell walk)
+    par_ = par0 ^ ((len & 1) && list_ ? 1 : 0);
+    bins_pending_ = !list_ && !close;
+    ene_valid_ = false;
+    return 0;
+}
+
+// Backward sweep from the seeded xb = dL/dx_N (vb = 0): segment by segment,
+// recomputing the positions of every segment but the last from its
+// checkpoint.  Accumulates dE (per element) and the eps / sigma terms (per
+// tile); restores the state at the end of the forward pass.
+int Engine::adjoint_sweep(std::string &err) {
+    const int N = N_, nsteps = adj_.nsteps, ckpt = adj_.ckpt, S = adj_.S;
+    int rc;
     const float dt = reps_h_[0].dt, h = reps_h_[0].hdt;
     bool first = true;
     for (int s = S - 1; s >= 0; --s) {
@@ -331,14 +368,7 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
             if (load_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "restore", err);
             rc = ensure_primed(err);
             if (rc) return rc;
-            if (record(0)) return check(hipGetLastError(), "history", err);
-            int bad = 0;
-            for (int k = 1; k < len && !bad; ++k) {
-                rec_ = hist_.p + (size_t)k * 2 * N;  // the drift records x_k
-                bad = launch_step_body(k > 1, true);
-            }
-            rec_ = nullptr;
-            if (bad) return check(hipGetLastError(), "step", err);
+            if (adj_record_segment(len, false)) return check(hipGetLastError(), "step", err);
         }
         for (int k = len - 1; k >= 0; --k) {
             const int n = s * ckpt + k;
@@ -359,22 +389,125 @@ int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, con
         }
     }
     // ---- restore the state at the end of the forward pass ----
-    if (load_state(final_state)) return check(hipGetLastError(), "restore", err);
-    rc = ensure_primed(err);
-    if (rc) return rc;
+    if (load_state(ckpt_.p + (size_t)S * 4 * N)) return check(hipGetLastError(), "restore", err);
+    return ensure_primed(err);
+}
+
+int Engine::adj_grads(double *dE, double *deps, double *dsig, std::string &err) {
     std::vector<double> de(dE_.n), pa(pacc_.n);
     (void)hipMemcpyAsync(de.data(), dE_.p, de.size() * sizeof(double), hipMemcpyDeviceToHost, stream_);
     (void)hipMemcpyAsync(pa.data(), pacc_.p, pa.size() * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    rc = sync(err);
+    const int rc = sync(err);
     if (rc) return rc;
-    double l = 0.0;
-    for (double v : lp) l += v;
-    if (loss) *loss = l / nloss;
     if (dE)
         for (int e = 0; e < ntri_; ++e) dE[ecaller_h_[e]] = de[e];
     if (deps) *deps = sum_pairs(pa, 0);
     if (dsig) *dsig = sum_pairs(pa, 1);
     return FM_OK;
+}
+
+int Engine::adj_zero() {
+    const size_t fb = (size_t)N_ * sizeof(float);
+    if (hipMemsetAsync(xbx_.p, 0, fb, stream_) != hipSuccess ||
+        hipMemsetAsync(xby_.p, 0, fb, stream_) != hipSuccess ||
+        hipMemsetAsync(vbx_.p, 0, fb, stream_) != hipSuccess ||
+        hipMemsetAsync(vby_.p, 0, fb, stream_) != hipSuccess ||
+        hipMemsetAsync(dE_.p, 0, dE_.n * sizeof(double), stream_) != hipSuccess ||
+        hipMemsetAsync(pacc_.p, 0, pacc_.n * sizeof(double), stream_) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+// Backward sweep of the last adjoint_forward with the terminal cotangent
+// (dL/dx_N, dL/dy_N) in caller order (either may be null: zero).
+int Engine::adjoint_backward(const float *dx, const float *dy, double *dE, double *deps,
+                             double *dsig, std::string &err) {
+    if (!adj_.valid || adj_.steps_at_end != steps_) {
+        err = "no recorded forward pass: call fm_adjoint_forward first (the state has moved since)";
+        return FM_ERR_STATE;
+    }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_order(err);
+    if (rc) return rc;
+    const int N = N_;
+    if (adj_zero()) return check(hipGetLastError(), "memset", err);
+    const std::vector<int32_t> &iid = iid_h_[0];
+    std::vector<float> bx(N, 0.0f), by(N, 0.0f);
+    for (int u = 0; u < N; ++u) {
+        if (dx) bx[iid[u]] = dx[u];
+        if (dy) by[iid[u]] = dy[u];
+    }
+    const size_t fb = (size_t)N * sizeof(float);
+    if (hipMemcpyAsync(xbx_.p, bx.data(), fb, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+        hipMemcpyAsync(xby_.p, by.data(), fb, hipMemcpyHostToDevice, stream_) != hipSuccess)
+        return check(hipGetLastError(), "cotangent", err);
+    rc = sync(err);  // the host buffers are reused
+    if (rc) return rc;
+    rc = adjoint_sweep(err);
+    if (rc) return rc;
+    adj_.valid = false;
+    return adj_grads(dE, deps, dsig, err);
+}
+
+int Engine::adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
+                        const float *ty, double *loss, double *dE, double *deps, double *dsig,
+                        std::string &err) {
+    if (nsteps < 1 || nloss < 1 || !ids || !tx || !ty) {
+        err = "nsteps and the loss set must be non-empty";
+        return FM_ERR_ARG;
+    }
+    int rc = check(hipSetDevice(device_), "hipSetDevice", err);
+    if (rc) return rc;
+    rc = ensure_order(err);
+    if (rc) return rc;
+    const int N = N_;
+    for (int k = 0; k < nloss; ++k)
+        if (ids[k] < 0 || ids[k] >= N) { err = "loss particle out of range"; return FM_ERR_ARG; }
+    rc = adjoint_forward(nsteps, ckpt, err);
+    if (rc) return rc;
+    // ---- the loss (1/K) sum |x_k - t_k|^2 and its gradient as the seed ----
+    const std::vector<int32_t> &iid = iid_h_[0];
+    std::vector<int32_t> lid(nloss);
+    std::vector<float2> tgt(nloss);
+    for (int k = 0; k < nloss; ++k) {
+        lid[k] = iid[ids[k]];
+        tgt[k] = make_float2(tx[k], ty[k]);
+    }
+    hipError_t he;
+    if ((he = loss_ids_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
+    if ((he = loss_t_.alloc(nloss)) != hipSuccess) return check(he, "alloc", err);
+    const int nlb = (nloss + 255) / 256;
+    if ((he = loss_part_.alloc(nlb)) != hipSuccess) return check(he, "alloc", err);
+    (void)hipMemcpyAsync(loss_ids_.p, lid.data(), nloss * sizeof(int32_t), hipMemcpyHostToDevice,
+                         stream_);
+    (void)hipMemcpyAsync(loss_t_.p, tgt.data(), nloss * sizeof(float2), hipMemcpyHostToDevice,
+                         stream_);
+    if (adj_zero()) return check(hipGetLastError(), "memset", err);
+    LossArgs la{};
+    la.reps = reps_.p;
+    la.n = nloss;
+    la.ids = loss_ids_.p;
+    la.target = loss_t_.p;
+    la.x = x_.p;
+    la.y = y_.p;
+    la.xbx = xbx_.p;
+    la.xby = xby_.p;
+    la.scale = 2.0f / (float)nloss;
+    la.part = loss_part_.p;
+    if (launch_adj_loss(la, stream_) != hipSuccess) return check(hipGetLastError(), "loss", err);
+    std::vector<double> lp(nlb);
+    (void)hipMemcpyAsync(lp.data(), loss_part_.p, nlb * sizeof(double), hipMemcpyDeviceToHost,
+                         stream_);
+    rc = sync(err);  // (lid, tgt: host buffers of the uploads above)
+    if (rc) return rc;
+    rc = adjoint_sweep(err);
+    if (rc) return rc;
+    adj_.valid = false;
+    double l = 0.0;
+    for (double v : lp) l += v;
+    if (loss) *loss = l / nloss;
+    return adj_grads(dE, deps, dsig, err);
 }
 
 }  // namespace fm

@@ -584,6 +584,35 @@ int fm_adjoint(fm_system *h, int32_t nsteps, int32_t ckpt, int32_t nloss, const 
     FM_END(h)
 }
 
+int fm_adjoint_forward(fm_system *h, int32_t nsteps, int32_t ckpt) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    return h->eng->adjoint_forward(nsteps, ckpt, h->err);
+    FM_END(h)
+}
+
+int fm_adjoint_backward(fm_system *h, const float *dL_dx, const float *dL_dy, double *dE,
+                        double *deps, double *dsig) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    const int32_t n = (int32_t)h->hs.x.size();
+    for (int32_t u = 0; u < n; ++u)
+        if ((dL_dx && !std::isfinite(dL_dx[u])) || (dL_dy && !std::isfinite(dL_dy[u])))
+            return fail(h, FM_ERR_ARG, "non-finite cotangent");
+    return h->eng->adjoint_backward(dL_dx, dL_dy, dE, deps, dsig, h->err);
+    FM_END(h)
+}
+
+int fm_adjoint_run(fm_system *h, int32_t nsteps, int32_t ckpt, const float *dL_dx,
+                   const float *dL_dy, double *dE, double *deps, double *dsig) {
+    FM_GUARD(h)
+    FM_NEED_ENGINE(h);
+    const int rc = h->eng->adjoint_forward(nsteps, ckpt, h->err);
+    if (rc) return rc;
+    return fm_adjoint_backward(h, dL_dx, dL_dy, dE, deps, dsig);
+    FM_END(h)
+}
+
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------

@@ -628,6 +628,7 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
 
 int Engine::set_dt(int r, double dt, std::string &err) {
     if (r < 0 || r >= R_ || !(dt > 0)) { err = "bad replica or dt"; return FM_ERR_ARG; }
+    adj_.valid = false;
     reps_h_[r].dt = (float)dt;
     reps_h_[r].hdt = 0.5f * reps_h_[r].dt;
     int rc = check(hipMemcpyAsync(reps_.p + r, &reps_h_[r], sizeof(RepDesc), hipMemcpyHostToDevice,
@@ -1454,8 +1455,12 @@ int Engine::set_state(int r, const float *x, const float *y, const float *vx, co
     }
     if (x || y) primed_ = false;
     // the kinetic-energy partials (velocities) and the energies (positions)
-    // of the last formed observables no longer describe the state
-    if (x || y || vx || vy) obs_valid_ = false;
+    // of the last formed observables no longer describe the state, and a
+    // recorded adjoint forward pass no longer ends at it
+    if (x || y || vx || vy) {
+        obs_valid_ = false;
+        adj_.valid = false;
+    }
     return FM_OK;
 }
 

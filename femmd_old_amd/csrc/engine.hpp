@@ -109,6 +109,12 @@ public:
     int adjoint_run(int nsteps, int ckpt, int nloss, const int32_t *ids, const float *tx,
                     const float *ty, double *loss, double *dE, double *deps, double *dsig,
                     std::string &err);
+    // the forward part (nsteps steps recorded for the backward sweep; the
+    // state advances) and the backward sweep of a terminal cotangent
+    // dL/dx_N, dL/dy_N (caller order)
+    int adjoint_forward(int nsteps, int ckpt, std::string &err);
+    int adjoint_backward(const float *dx, const float *dy, double *dE, double *deps,
+                         double *dsig, std::string &err);
     int ntri() const { return ntri_; }
     int set_tuning(int tile_w, int block, int use_graph, std::string &err);
     // internal order: re-sort the movable particles to the last counting
@@ -220,6 +226,17 @@ private:
     // displacement from the last build) + adjoint kernel over the lists
     int adj_pass_list(bool fresh, std::string &err);
     int adj_kick(bool accumulate, bool drift, bool make_w, float c);
+    // a segment of len steps from the current state with its positions
+    // recorded into the history slots (and the closing half-kick)
+    int adj_record_segment(int len, bool close = true);
+    int adjoint_sweep(std::string &err);
+    int adj_grads(double *dE, double *deps, double *dsig, std::string &err);
+    int adj_zero();
+    struct AdjRun {
+        bool valid = false;
+        int nsteps = 0, ckpt = 0, S = 0;
+        int64_t steps_at_end = 0;
+    } adj_;
     int save_state(float *dst);
     int load_state(const float *src);
     int build_tiles(const std::vector<int> &tile_w, std::string &err);
@@ -361,6 +378,8 @@ hipError_t launch_total_force(int n, const float *fx, const float *fy, const flo
                               hipStream_t s);
 hipError_t launch_observe(const ObsArgs &o, bool accumulate, hipStream_t s);
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
+// dst[0, n) = x, dst[n, 2n) = y (graph-safe copy kernel)
+hipError_t launch_copy2(float *dst, const float *x, const float *y, int n, hipStream_t s);
 hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s);
 hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s);

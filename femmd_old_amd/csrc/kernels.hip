@@ -3106,6 +3106,20 @@ hipError_t launch_total_force(int n, const float *fx, const float *fy,
     return hipGetLastError();
 }
 
+__global__ void k_copy2(float *dst, const float *x, const float *y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        dst[i] = x[i];
+        dst[n + i] = y[i];
+    }
+}
+
+hipError_t launch_copy2(float *dst, const float *x, const float *y, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy2, dim3((n + 255) / 256), dim3(256), 0, s, dst, x, y, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,

@@ -211,6 +211,7 @@ int Engine::set_temperature(int r, double temp, int n, const int32_t *ids, std::
     // the kinetic energy of the last formed observables is stale: the next
     // query forms them again from the new velocities
     obs_valid_ = false;
+    adj_.valid = false;
     return sync(err);
 }
 
@@ -326,6 +327,7 @@ int Engine::set_temperature_noalloc(int r, double temp, int n, const int32_t *id
     if (launch_vel_scale(dsel, m, vx_.p, vy_.p, f, stream_) != hipSuccess)
         return check(hipGetLastError(), "rescale", err);
     obs_valid_ = false;
+    adj_.valid = false;
     return sync(err);
 }
 

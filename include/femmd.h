@@ -294,6 +294,22 @@ int fm_adjoint(fm_system *h, int32_t nsteps, int32_t ckpt, int32_t nloss,
                const int32_t *ids, const float *tx, const float *ty,
                double *loss, double *dE, double *deps, double *dsig);
 
+/* The adjoint for an arbitrary loss of the final positions, in two calls:
+ * fm_adjoint_forward runs nsteps steps from the current state (the state
+ * advances as with fm_step), keeping what the backward sweep needs (the
+ * stored trajectory, or checkpoints every ckpt steps as fm_adjoint chooses
+ * them); fm_adjoint_backward then takes the terminal cotangent dL/dx_N,
+ * dL/dy_N (caller order, n floats each; NULL = zero) and returns dL/dE per
+ * element (caller order), dL/deps and dL/dsigma, FP64.  The state must not
+ * change in between (FM_ERR_STATE otherwise); the backward call consumes
+ * the recording.  fm_adjoint_run is both in one call, for a cotangent known
+ * beforehand. */
+int fm_adjoint_forward(fm_system *h, int32_t nsteps, int32_t ckpt);
+int fm_adjoint_backward(fm_system *h, const float *dL_dx, const float *dL_dy, double *dE,
+                        double *deps, double *dsig);
+int fm_adjoint_run(fm_system *h, int32_t nsteps, int32_t ckpt, const float *dL_dx,
+                   const float *dL_dy, double *dE, double *deps, double *dsig);
+
 /* ---- batched replicas (no reference counterpart) ---- */
 int fm_batch_create(int32_t n, fm_system *const *systems, fm_batch **out);
 void fm_batch_destroy(fm_batch *b);

@@ -73,6 +73,9 @@ def lib() -> C.CDLL:
         L.or_run.argtypes = [C.POINTER(OrSystem), _d, _d, _d, _d, _d, _d, C.c_int32, C.c_double,
                              C.c_int32, _d]
         L.or_run.restype = C.c_int32
+        L.or_tangent.argtypes = [C.POINTER(OrSystem), _d, _d, C.c_int32, C.c_double, _d, _d,
+                                 C.c_double, C.c_double, C.c_double, _d, _d]
+        L.or_tangent.restype = C.c_int32
         L.or_bl_create.argtypes = [C.POINTER(OrSystem), _f, _f, _f, C.c_float, C.c_float,
                                    C.c_int32]
         L.or_bl_create.restype = C.c_void_p
@@ -245,6 +248,42 @@ def run(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt, pred_f32=False, hist
                       int(nsteps), float(dt), int(pred_f32), _dp(hist))
     assert rc == 0, rc
     return x, y, vx, vy, hist
+
+
+def _inv_mass(mass):
+    mass = np.asarray(mass, np.float64)
+    return np.where(np.isfinite(mass) & (mass > 0), 1.0 / np.where(mass > 0, mass, 1.0), 0.0)
+
+
+def trajectory(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt):
+    """The FP64 or_run rollout's positions at steps 0..nsteps, each an
+    (nsteps + 1, n) array."""
+    x, y, vx, vy = (np.array(a, np.float64) for a in (x, y, vx, vy))
+    tx = np.empty((nsteps + 1, x.shape[0]))
+    ty = np.empty_like(tx)
+    tx[0], ty[0] = x, y
+    for k in range(nsteps):
+        x, y, vx, vy, _ = run(osys, x, y, vx, vy, mass, 1, dt)
+        tx[k + 1], ty[k + 1] = x, y
+    return tx, ty
+
+
+def tangent(osys: OracleSystem, tx, ty, dt, mass, dE=None, deps=0.0, dsig=0.0, h=1e-6):
+    """Forward-mode derivative (dx, dy) of the final positions of a rollout
+    along the recorded trajectory (tx, ty), (nsteps + 1, n) each, for the
+    parameter direction (dE per element, deps on eps, dsig on every sigma);
+    force derivatives by FP64 central differences of step h (or_tangent)."""
+    tx = np.ascontiguousarray(tx, np.float64)
+    ty = np.ascontiguousarray(ty, np.float64)
+    n = tx.shape[1]
+    inv_m = _inv_mass(mass)
+    dEa = None if dE is None else np.ascontiguousarray(dE, np.float64)
+    dx, dy = np.empty(n), np.empty(n)
+    rc = lib().or_tangent(C.byref(osys.s), _dp(tx), _dp(ty), tx.shape[0] - 1, float(dt),
+                          _dp(inv_m), _dp(dEa), float(deps), float(dsig), float(h), _dp(dx),
+                          _dp(dy))
+    assert rc == 0, rc
+    return dx, dy
 
 
 class Baseline:

@@ -477,6 +477,105 @@ int32_t or_run(const or_system *s0, double *x, double *y, double *vx,
     return 0;
 }
 
+// Forward-mode derivative of an nsteps KDK rollout (the map of or_run)
+// along a recorded trajectory: positions (tx, ty)[k * n + i] at steps
+// k = 0..nsteps.  The parameters move along (dE per element, deps added to
+// every eps entry, dsig added to every sigma); the initial state does not
+// depend on them.  Every force derivative is a central difference in FP64,
+//   G_k = [F(x_k + h dx_k; p + h d) - F(x_k - h dx_k; p - h d)] / 2h,
+// which is J dx_k + (dF/dp) d to O(h^2); the tangent follows the step:
+//   dv += dt/2 G_k inv_m; dx += dt dv; dv += dt/2 G_{k+1} inv_m.
+// Both evaluations of G_k use the interacting pairs (and their periodic
+// images) of the unperturbed x_k: the derivative of the smooth branch the
+// step is on.  (A pair within h |dr| of the cutoff would otherwise enter one
+// evaluation and not the other; where the perturbed positions move far, as
+// a sigma direction drives them over hundreds of steps, such pairs are
+// frequent enough to bias a plain difference by parts in 1e3.)
+// Returns (dx, dy) at step nsteps.  Reentrant (callers run directions in
+// threads).
+int32_t or_tangent(const or_system *s0, const double *tx, const double *ty, int32_t nsteps,
+                   double dt, const double *inv_m, const double *dE, double deps, double dsig,
+                   double hfd, double *dx, double *dy) {
+    if (!s0 || nsteps < 0 || !(hfd > 0)) return -1;
+    const int n = s0->n;
+    const int nt2 = s0->ntypes * s0->ntypes;
+    std::vector<double> sig[2], eps[2], tE[2];
+    for (int q = 0; q < 2; ++q) {
+        const double sg = q ? -hfd : hfd;
+        sig[q].resize(n);
+        for (int i = 0; i < n; ++i) sig[q][i] = s0->sigma[i] + sg * dsig;
+        eps[q].resize(nt2);
+        for (int k = 0; k < nt2; ++k) eps[q][k] = s0->eps[k] + sg * deps;
+        tE[q].resize(s0->ntri);
+        for (int e = 0; e < s0->ntri; ++e) tE[q][e] = s0->tE[e] + (dE ? sg * dE[e] : 0.0);
+    }
+    struct PairRec { int32_t i, j, sx, sy; };
+    std::vector<PairRec> pairs;
+    std::vector<double> px(n), py(n), fpx(n), fpy(n), fmx(n), fmy(n), gx(n), gy(n);
+    std::vector<double> vx(n, 0.0), vy(n, 0.0);
+    double en[OR_NEN];
+    auto G = [&](int k) -> int {
+        const double *xk = tx + (size_t)k * n, *yk = ty + (size_t)k * n;
+        pairs.clear();
+        if (s0->lj_on && n > 0) {
+            or_system sb = *s0;
+            sb.x = xk;
+            sb.y = yk;
+            PairCtx c;
+            make_ctx(&sb, c);
+            for_candidates(c, 0, [&](int i, int j) {
+                int sx, sy;
+                if (pred64(&sb, i, j, sx, sy)) pairs.push_back({i, j, sx, sy});
+            });
+        }
+        for (int q = 0; q < 2; ++q) {
+            const double sg = q ? -hfd : hfd;
+            // not wrapped: each pair keeps the image shift of x_k
+            for (int i = 0; i < n; ++i) {
+                px[i] = xk[i] + sg * dx[i];
+                py[i] = yk[i] + sg * dy[i];
+            }
+            or_system s = *s0;
+            s.x = px.data();
+            s.y = py.data();
+            s.sigma = sig[q].data();
+            s.eps = eps[q].data();
+            if (s.ntri > 0) s.tE = tE[q].data();
+            double *fx = q ? fmx.data() : fpx.data(), *fy = q ? fmy.data() : fpy.data();
+            std::fill(fx, fx + n, 0.0);
+            std::fill(fy, fy + n, 0.0);
+            if (s.nbond > 0) bonds64(&s, s.x, s.y, fx, fy, en, nullptr);
+            if (s.ntri > 0) cst64(&s, s.x, s.y, fx, fy, en, nullptr);
+            for (const PairRec &p : pairs) lj_pair64(&s, p.i, p.j, p.sx, p.sy, fx, fy, en, nullptr);
+        }
+        for (int i = 0; i < n; ++i) {
+            gx[i] = (fpx[i] - fmx[i]) / (2 * hfd);
+            gy[i] = (fpy[i] - fmy[i]) / (2 * hfd);
+        }
+        return 0;
+    };
+    std::fill(dx, dx + n, 0.0);
+    std::fill(dy, dy + n, 0.0);
+    const double h = 0.5 * dt;
+    int rc = G(0);
+    if (rc) return rc;
+    for (int k = 0; k < nsteps; ++k) {
+        for (int i = 0; i < n; ++i) {
+            vx[i] += h * gx[i] * inv_m[i];
+            vy[i] += h * gy[i] * inv_m[i];
+            dx[i] += dt * vx[i];
+            dy[i] += dt * vy[i];
+        }
+        rc = G(k + 1);
+        if (rc) return rc;
+        for (int i = 0; i < n; ++i) {
+            vx[i] += h * gx[i] * inv_m[i];
+            vy[i] += h * gy[i] * inv_m[i];
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"
 
 // ===========================================================================

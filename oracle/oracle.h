@@ -115,6 +115,15 @@ int32_t or_run(const or_system *s, double *x, double *y, double *vx,
                double *vy, const double *mass, const double *inv_m,
                int32_t nsteps, double dt, int32_t pred_f32, double *en_hist);
 
+/* Forward-mode derivative of an nsteps or_run rollout along the recorded
+ * trajectory tx, ty ((nsteps + 1) * n, step-major) for the parameter
+ * direction (dE per element or NULL, deps on every eps entry, dsig on every
+ * sigma); force derivatives by FP64 central differences of step hfd.
+ * Writes (dx, dy) at step nsteps. */
+int32_t or_tangent(const or_system *s, const double *tx, const double *ty, int32_t nsteps,
+                   double dt, const double *inv_m, const double *dE, double deps, double dsig,
+                   double hfd, double *dx, double *dy);
+
 /* ---- CPU baseline: the reference DEV=CPU path in FP32, Verlet list ---- */
 typedef struct or_baseline or_baseline;
 or_baseline *or_bl_create(const or_system *s, const float *vx,
