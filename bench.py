@@ -412,6 +412,8 @@ def main():
                     help="neighbour-list skin (0: walk the cells every step)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel of the step one by one (no graph replay)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (5 launches), the "
@@ -441,6 +443,8 @@ def main():
     t_gen = time.time()
     w = configs.c2(scale=args.scale)
     s = configs.build_system(w, skin=args.skin)
+    if args.no_graph:
+        s.set_tuning(use_graph=0)
     print(f"[bench] workload c2: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} L={w.Lx:.2f} "
           f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
     ncx, ncy, _, _, _ = s.cells()

@@ -376,6 +376,16 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     }
 
     // ---- device allocations and uploads ----
+    // The copies are asynchronous: their host sources (the vectors above and
+    // these) stay alive until the stream has drained, on every return path
+    // (the guard, declared after them, is destroyed first).
+    std::vector<int2> hbrep;
+    std::vector<Span> hns;
+    std::vector<int32_t> hnr;
+    struct DrainGuard {
+        hipStream_t s;
+        ~DrainGuard() { (void)hipStreamSynchronize(s); }
+    } drain_guard{st};
     const int N = E->N_;
     hipError_t he = hipSuccess;
 #define FM_UP(buf, vec) \
@@ -437,10 +447,10 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     FM_AL(tmp_, (size_t)E->ncells_ * E->cap_);
     E->nsort_ = (E->ncells_ + kSortCells - 1) / kSortCells;
     FM_AL(sort_status_, std::max(E->nsort_, 1));
+    hbrep.assign(std::max(E->nsort_, 1), make_int2(0, 0));
     {
         // per sort block: the replicas its cells belong to (replica r owns
         // cells [cell0, cell0 + ncx * ncy + 1))
-        std::vector<int2> hbrep(std::max(E->nsort_, 1));
         int r = 0;
         for (int b = 0; b < E->nsort_; ++b) {
             const int c0 = b * kSortCells, c1 = std::min(c0 + kSortCells, E->ncells_) - 1;
@@ -477,21 +487,20 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         // the node kick: per replica its particles [pg, p1) (those with
         // element or bond corners, and every particle of a replica without
         // pair forces: no tile kicks them), spans of kKickThreads
-        std::vector<Span> hs;
-        std::vector<int32_t> hr(2 * R, 0);
+        hnr.assign(2 * R, 0);
         E->nnode_ = 0;
         for (int r = 0; r < R; ++r) {
-            hr[2 * r] = (int)hs.size();
+            hnr[2 * r] = (int)hns.size();
             for (int b = rd[r].pg; b < rd[r].p1; b += kKickThreads)
-                hs.push_back(Span{r, b, std::min(b + kKickThreads, rd[r].p1), 0});
-            hr[2 * r + 1] = (int)hs.size();
+                hns.push_back(Span{r, b, std::min(b + kKickThreads, rd[r].p1), 0});
+            hnr[2 * r + 1] = (int)hns.size();
             E->nnode_ += rd[r].p1 - rd[r].pg;
         }
-        E->nspans_h_ = hs;
-        if ((he = upload(E->nspans_, hs, st)) != hipSuccess ||
-            (he = upload(E->nrng_, hr, st)) != hipSuccess)
+        E->nspans_h_ = hns;
+        if ((he = upload(E->nspans_, hns, st)) != hipSuccess ||
+            (he = upload(E->nrng_, hnr, st)) != hipSuccess)
             return E->check(he, "upload node kick", err);
-        FM_AL(ke_node_, 2 * std::max<size_t>(hs.size(), 1));
+        FM_AL(ke_node_, 2 * std::max<size_t>(hns.size(), 1));
     }
     FM_AL(fea_part_, 2 * std::max<size_t>(es.size(), 1));
     FM_AL(bond_part_, 2 * std::max<size_t>(bs.size(), 1));

@@ -46,9 +46,12 @@ int Engine::build_clusters(const std::vector<const HostSystem *> &reps, std::str
         (he = put(cl_nodes_, nodes, stream_)) != hipSuccess ||
         (he = put(cid_, cid, stream_)) != hipSuccess ||
         (he = cl_out_.alloc((size_t)std::max(ncl, 1) * kClusterOut)) != hipSuccess ||
-        (he = cl_cm_.alloc(std::max(ncl, 1))) != hipSuccess)
+        (he = cl_cm_.alloc(std::max(ncl, 1))) != hipSuccess) {
+        (void)hipStreamSynchronize(stream_);  // nodes, cid: sources of the copies
         return check(he, "clusters", err);
-    return FM_OK;
+    }
+    // nodes and cid are the sources of asynchronous copies
+    return check(hipStreamSynchronize(stream_), "clusters", err);
 }
 
 int Engine::ncluster(int r) const {
