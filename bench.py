@@ -38,55 +38,68 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# --no-graph: every leg launches the step kernels one by one (profiling runs)
+NO_GRAPH = False
 
 
-def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None, uniform_mass=False):
-    """Bytes each kernel must move per executed launch (each input array
-    read once, each output written once; FP32 values, int32 indices).
-    uniform_mass: every particle has the same (finite) mass, which the kick
-    takes from the replica's descriptor: 4/particle less (no inv_m).
+def _eager(obj):
+    if NO_GRAPH:
+        obj.set_tuning(use_graph=0)
+    return obj
+
+
+def algorithmic_bytes(n, ncells, ntri, nbond, list_len=None, uniform_mass=False, n_corner=0):
+    """Bytes each kernel moves per executed launch in this layout (each
+    input array read once, each output written once; FP32 values, int32
+    indices).  The force pass also integrates the n - n_corner particles
+    without element or bond corners; the node kick integrates the n_corner
+    others.  uniform_mass: the kick takes the mass from the replica's
+    descriptor (no inv_m read).
 
     Per-step cell walk (list_len None):
-    kick     r x,y,vx,vy,fx,fy,inv_m,goff; w x,y,vx,vy,slot: 52/particle;
-             counter read-modify-write 8/cell; gather 12/corner
-    sort     r count, w count (zeroed), w start: 12/cell; r own slot entry,
-             x,y,group; w perm, sorted (16): 36/particle
-    forces   r sorted (16), perm; w fx,fy: 28/particle; r start: 4/cell;
+    forces   r sorted (16), perm: 20/particle; w fx,fy: 8/corner particle;
+             kicked particles r vx,vy, w x,y,vx,vy, slot: +28 (+4 inv_m);
+             r start 4/cell, counter read-modify-write 8/cell;
              CST: r 3 ids, Dm, Bm, A0*lambda, A0*mu, 3 node x,y; w 3 corners:
              100/element; bond: r ids, k, r0, 2 node x,y; w 2 ends: 48/bond
+    kick     (node kick) r fx,fy,x,y,vx,vy,goff; w x,y,vx,vy,slot: 48 (+4
+             inv_m)/corner particle; gather 12/corner
+    sort     r count, w count (zeroed), w start: 12/cell; r own slot entry,
+             x,y,group; w perm, sorted (16): 36/particle
     Neighbour lists (list_len = mean list length L, padded to groups of 4:
     about L + 1.5 slots of 2 bytes):
-    kick     r x,y,vx,vy,inv_m,spos,goff, pair force (8); w x,y,vx,vy and the
-             sorted-order copy of x,y (8): 60/particle; gather 12/corner
+    forces   r sorted x,y (8), length, slots, internal id (4): 16 + 2(L+1.5)
+             /particle; w fx,fy: 8/corner particle; kicked particles r vx,vy,
+             w x,y,vx,vy and the sorted-order copy of x,y: +32 (+4 inv_m);
+             r start: 4/cell; elements and bonds as above
+    kick     (node kick) r fx,fy,x,y,vx,vy,spos,goff; w x,y,vx,vy and the
+             sorted-order x,y: 56 (+4 inv_m)/corner particle; gather 12/corner
     bin      r x,y; w slot: 12/particle; counters 8/cell     (rebuild steps)
     sort     as above plus spos and the sorted x,y: 48/particle (rebuild steps)
     list     r sorted (16); w slots, length: 20 + 2(L+1.5)/particle;
              r start: 4/cell                                   (rebuild steps)
-    forces   r sorted x,y (8), length, slots, internal id (4); w pair force
-             (8, internal order): 24 + 2(L+1.5)/particle; r start: 4/cell;
-             elements and bonds as above
     """
     corners = 36 * ntri + 24 * nbond
     elems = 100 * ntri + 48 * nbond
+    im = 0 if uniform_mass else 4
+    nk = n - n_corner  # integrated by the force pass
     if list_len is None:
         b = {
-            "kick": 52 * n + 8 * ncells + corners,
+            "kick": (48 + im) * n_corner + corners,
             "bin": 0,
             "sort": 36 * n + 12 * ncells,
             "list_build": 0,
-            "forces": 28 * n + 4 * ncells + elems,
+            "forces": 20 * n + 8 * n_corner + (28 + im) * nk + 12 * ncells + elems,
         }
     else:
         slots = 2.0 * (float(list_len) + 1.5)
         b = {
-            "kick": 60 * n + corners,
+            "kick": (56 + im) * n_corner + corners,
             "bin": 12 * n + 8 * ncells,
             "sort": 48 * n + 12 * ncells,
             "list_build": int((20 + slots) * n) + 4 * ncells,
-            "forces": int((24 + slots) * n) + 4 * ncells + elems,
+            "forces": int((16 + slots) * n) + 8 * n_corner + (32 + im) * nk + 4 * ncells + elems,
         }
-    if uniform_mass:
-        b["kick"] -= 4 * n
     return b
 
 
@@ -168,6 +181,7 @@ def batched_leg(steps: int, loop_steps: int, tune_cache, looped_leg: bool = True
     t_tune = time.time()
     b.tune(tune_cache)
     t_tune = time.time() - t_tune
+    _eager(b)
     b.run(20)
     tb0 = time.perf_counter()
     b.run(steps)
@@ -177,6 +191,7 @@ def batched_leg(steps: int, loop_steps: int, tune_cache, looped_leg: bool = True
     if looped_leg:
         for s in systems:  # warm-up also captures each replica's step graph
             s.allocate_cuda()
+            _eager(s)
             s.run(loop_steps)
         tl0 = time.perf_counter()
         for s in systems:
@@ -206,7 +221,7 @@ def c3_leg(steps: int, warmup: int, skin: float):
     from femmd_old_amd import configs
     t_gen = time.time()
     w = configs.c3()
-    s = configs.build_system(w, skin=skin)
+    s = _eager(configs.build_system(w, skin=skin))
     print(f"[bench] workload c3: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} bonds={w.n_bonds} "
           f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
     ncx, ncy, _, _, _ = s.cells()
@@ -225,8 +240,8 @@ def c3_leg(steps: int, warmup: int, skin: float):
                           excl=True, n_corner=fused_corner_particles(w))
     ms_step = 1e3 * secs / steps
     # run twice: two fresh systems, the same steps, bitwise the same state
-    a = configs.build_system(w, skin=skin)
-    b = configs.build_system(w, skin=skin)
+    a = _eager(configs.build_system(w, skin=skin))
+    b = _eager(configs.build_system(w, skin=skin))
     a.run(50)
     b.run(50)
     same = all(np.array_equal(u, v) for u, v in zip(a.get_state(), b.get_state()))
@@ -264,7 +279,7 @@ def adjoint_leg(steps: int, ckpt: int):
     import numpy as np
     from femmd_old_amd import configs
     w = configs.c5()
-    st = configs.build_system(configs.with_params(w, E=configs.c5_target_E(w)))
+    st = _eager(configs.build_system(configs.with_params(w, E=configs.c5_target_E(w))))
     st.run(steps)
     ga = w.global_arrays()
     nodes = np.arange(w.n_solvent, w.n_total)
@@ -272,7 +287,7 @@ def adjoint_leg(steps: int, ckpt: int):
     xt, yt, _, _ = st.get_state()
     tgt = np.stack([xt[ids], yt[ids]], 1)
     st.close()
-    s = configs.build_system(w)
+    s = _eager(configs.build_system(w))
     s0 = s.get_state()
     s.run(steps)  # warm-up: graph capture
     s.set_state(*s0)
@@ -412,8 +427,11 @@ def main():
                     help="neighbour-list skin (0: walk the cells every step)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--only", choices=["c3", "adjoint", "batched"], default=None,
+                    help="run one secondary leg alone and print its JSON (profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="launch every kernel of the step one by one (no graph replay)")
+                    help="launch every kernel of every leg one by one (no graph replay; "
+                         "profiling runs)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (5 launches), the "
@@ -438,13 +456,22 @@ def main():
         print(json.dumps({"gbps": gbs, "device": dev}), flush=True)
         return
 
+    global NO_GRAPH
+    NO_GRAPH = args.no_graph
+    if args.only:
+        # one secondary leg alone (its kernels profiled apart from the headline's)
+        leg = {"c3": lambda: c3_leg(args.c3_steps, max(args.warmup, 1), args.skin),
+               "adjoint": lambda: adjoint_leg(args.adjoint_steps, args.adjoint_ckpt),
+               "batched": lambda: batched_leg(args.batched_steps, max(4, args.batched_steps // 10),
+                                              args.tune_cache, not args.no_looped)}[args.only]()
+        print(json.dumps({args.only: leg}), flush=True)
+        return
+
     from femmd_old_amd import configs
 
     t_gen = time.time()
     w = configs.c2(scale=args.scale)
-    s = configs.build_system(w, skin=args.skin)
-    if args.no_graph:
-        s.set_tuning(use_graph=0)
+    s = _eager(configs.build_system(w, skin=args.skin))
     print(f"[bench] workload c2: N={w.n_total} nodes={w.n_nodes} tri={w.n_tri} L={w.Lx:.2f} "
           f"(setup {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
     ncx, ncy, _, _, _ = s.cells()
@@ -487,7 +514,8 @@ def main():
     m = w.global_arrays()["mass"]
     ab = algorithmic_bytes(w.n_total, ncx * ncy, w.n_tri, w.n_bonds,
                            ls1["mean"] if list_mode else None,
-                           uniform_mass=bool(np.isfinite(m[0]) and m[0] > 0 and np.all(m == m[0])))
+                           uniform_mass=bool(np.isfinite(m[0]) and m[0] > 0 and np.all(m == m[0])),
+                           n_corner=fused_corner_particles(w))
     kern = "forces"
     ga = w.global_arrays()
     n_nodes = int(w.n_nodes)

@@ -21,26 +21,29 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_algorithmic_bytes_model():
+    """Structural properties of the layout byte model: moving a particle's
+    integration from the node kick into the force pass saves bytes (its
+    forces never leave the pass), the inverse masses cost 4 bytes per
+    integrated particle, and only list mode has a list build."""
     n, ncells, ntri, nb = 1000, 100, 50, 10
-    cells = bench.algorithmic_bytes(n, ncells, ntri, nb)
-    lists = bench.algorithmic_bytes(n, ncells, ntri, nb, 20.0)
-    corners = 36 * ntri + 24 * nb
-    elems = 100 * ntri + 48 * nb
-    # cell walk: the binning is part of the kick, no list build
-    assert cells["kick"] == 52 * n + 8 * ncells + corners
-    assert cells["bin"] == 0 and cells["list_build"] == 0
-    assert cells["sort"] == 36 * n + 12 * ncells
-    assert cells["forces"] == 28 * n + 4 * ncells + elems
-    # lists: 2 bytes per slot, L + 1.5 slots per particle
-    assert lists["kick"] == 60 * n + corners
-    assert lists["list_build"] == int((20 + 2 * 21.5) * n) + 4 * ncells
-    assert lists["forces"] == int((24 + 2 * 21.5) * n) + 4 * ncells + elems
+    for L in (None, 20.0):
+        allnode = bench.algorithmic_bytes(n, ncells, ntri, nb, L, n_corner=n)
+        fused = bench.algorithmic_bytes(n, ncells, ntri, nb, L, n_corner=100)
+        tot = lambda d: d["kick"] + d["forces"]
+        assert tot(fused) < tot(allnode)
+        # per particle moved into the pass: the node kick's read of fx,fy
+        # and the pass's write of them are gone
+        assert (tot(allnode) - tot(fused)) % 900 == 0 and tot(allnode) - tot(fused) >= 16 * 900
+        uni = bench.algorithmic_bytes(n, ncells, ntri, nb, L, uniform_mass=True, n_corner=100)
+        assert tot(fused) - tot(uni) == 4 * n
+        assert {k: uni[k] for k in ("bin", "sort", "list_build")} == \
+            {k: fused[k] for k in ("bin", "sort", "list_build")}
+        assert (fused["list_build"] > 0) == (L is not None)
     assert set(bench.REBUILD_KERNELS) == {"bin", "sort", "list_build"}
-    # one mass for every particle: the kick reads no inverse masses
-    uni = bench.algorithmic_bytes(n, ncells, ntri, nb, 20.0, uniform_mass=True)
-    assert uni["kick"] == lists["kick"] - 4 * n
-    assert {k: v for k, v in uni.items() if k != "kick"} == \
-        {k: v for k, v in lists.items() if k != "kick"}
+    # the survey's compulsory figures: the fused pass adds 16 per kicked particle
+    cb = bench.compulsory_bytes(n, ncells, ntri, nb, 300, 20, n_corner=300)
+    assert cb["forces_fused"] - cb["forces"] == 16 * (n - 300)
+    assert cb["forces"] == 24 * n + 4 * ncells + 84 * ntri + 40 * nb
 
 
 def test_committed_counter_summaries_are_read():
@@ -50,7 +53,8 @@ def test_committed_counter_summaries_are_read():
     t = bench.measured_traffic("k_forces", "list")
     v = bench.measured_valu_us("k_forces", "list")
     assert t is not None and v is not None
-    ab = bench.algorithmic_bytes(1141760, 380 * 380, 153600, 0, 21.66)
+    ab = bench.algorithmic_bytes(1141760, 380 * 380, 153600, 0, 21.66, uniform_mass=True,
+                                 n_corner=93184)
     assert 0.9 <= t / ab["forces"] <= 1.5
     assert 5.0 < v < 100.0
     assert bench.measured_valu_us("k_none", "list") is None
