@@ -229,7 +229,7 @@ int fm_set_pair_table(fm_system *h, int32_t ntypes, const float *eps, const floa
     FM_GUARD(h)
     if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
     if (ntypes < 1 || ntypes > fm::kMaxPairTypes || !eps || !rcut || !ushift)
-        return fail(h, FM_ERR_ARG, "ntypes must be in [1, 4] with three tables");
+        return fail(h, FM_ERR_ARG, "ntypes must be in [1, 16] with three tables");
     const int n2 = ntypes * ntypes;
     for (int q = 0; q < n2; ++q)
         if (!(rcut[q] > 0)) return fail(h, FM_ERR_ARG, "cutoffs must be positive");

@@ -20,7 +20,7 @@ constexpr int kMaxSlots = 3 * (kMaxTileW + 2);
 constexpr int kSortCells = 256;
 constexpr int kKickThreads = 256;
 constexpr int kElemThreads = 256;
-constexpr int kMaxPairTypes = 4;  // ntypes <= 4 on the typed path
+constexpr int kMaxPairTypes = 16;  // particle types of a pair table
 constexpr int kMaxListCap = 512;  // neighbour-list capacity per particle
 constexpr int kMaxListGroups = kMaxListCap / 4;
 
@@ -50,7 +50,10 @@ struct alignas(16) RepDesc {
     int32_t pg;        // particles [pg, p1) have element or bond corners (or
                        // the replica has no pair forces): the node kick
                        // integrates them, the force pass the others
-    int32_t pad_[3];
+    float rc_list;     // list mode: the largest cutoff of the pair table (in
+                       // sigma units); lists hold every pair within
+                       // rc_list * sigma_ij + skin
+    int32_t pad_[2];
 };
 
 // Per-step observables (femmd2D/src/properties.cpp:15-86, evaluated and

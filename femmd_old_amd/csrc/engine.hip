@@ -108,16 +108,15 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     E->reorder_every_ = reps[0]->reorder_every;
     E->obs_every_ = reps[0]->observe_every;
 
-    // neighbour-list mode: one skin for the batch, single-type pair tables
+    // neighbour-list mode: one skin for the batch
     const float skin = reps[0]->skin;
-    bool any_types = false, any_lj = false;
+    bool any_lj = false;
     for (int r = 0; r < R; ++r) {
         if (reps[r]->skin != skin) { err = "all replicas of a batch must use one skin"; return FM_ERR_ARG; }
-        any_types |= reps[r]->ntypes > 1;
         any_lj |= reps[r]->lj_on;
     }
     if (!(skin >= 0.0f) || !std::isfinite(skin)) { err = "skin must be >= 0"; return FM_ERR_ARG; }
-    E->list_ = skin > 0.0f && !any_types && any_lj;
+    E->list_ = skin > 0.0f && any_lj;
     E->skin_ = E->list_ ? skin : 0.0f;
 
     // ---- pass 1: validation and cell geometry per replica ----
@@ -132,7 +131,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         if (!(S.Lx > 0) || !(S.Ly > 0)) { err = "box must be positive"; return FM_ERR_ARG; }
         if (S.model != E->model_) { err = "all replicas of a batch must use one material model"; return FM_ERR_ARG; }
         if (S.ntypes < 1 || S.ntypes > kMaxPairTypes) {
-            err = "ntypes must be in [1, 4]";
+            err = "ntypes must be in [1, 16]";
             return FM_ERR_ARG;
         }
         for (int32_t t : S.type)
@@ -180,7 +179,12 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
             const float Lm = std::max(D.Lx, D.Ly);
             D.dslack = 4.0f * (std::nextafter(Lm, INFINITY) - Lm);
         }
-        const double rl = (double)rcf * s0 + E->skin_;
+        // lists cover the largest cutoff of the table (a typed pair with a
+        // smaller one is filtered by the pass)
+        float rcl = 0.0f;
+        for (int k = 0; k < S.ntypes * S.ntypes; ++k) rcl = std::max(rcl, S.rc[k]);
+        D.rc_list = rcl;
+        const double rl = (double)rcl * s0 + E->skin_;
         D.rl2_uniform = (float)(rl * rl * 1.00002);
         // a listed pair was within rc*sigma + skin at the build and each end
         // moved at most skin / 2 since: rc*sigma_max + 2 skin bounds its

@@ -1095,7 +1095,7 @@ __device__ void pair_tile_body(const ForceArgs &a, const RepDesc &R, int rep, in
             if (kTypes) {
                 const PairEntry pe = L.pt[ti * nt + (kTypes ? L.typ[j] : 0)];
                 e48 = pe.e48; e4 = pe.e4; shift = pe.shift;
-                const float sij = kSigma ? (si + pj.w) * 0.5f : si;
+                const float sij = kSigma ? (si + pj.w) * 0.5f : R.s_uniform;  // (no sigma staged)
                 rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                 s2ij = sij * sij;
             } else if (kSigma) {
@@ -1216,7 +1216,8 @@ __device__ int stage_layout(const ForceArgs &a, const Tile &T, const RepDesc &R,
         L.cstart[s] = st;
         L.off[s + 1] = a.start[c + 1] - st;  // count, scanned below
     }
-    for (int k = tid; k < R.ntypes * R.ntypes; k += blockDim.x) L.pt[k] = a.ptab[R.pt + k];
+    if (L.pt)  // (the list build needs no pair table)
+        for (int k = tid; k < R.ntypes * R.ntypes; k += blockDim.x) L.pt[k] = a.ptab[R.pt + k];
     __syncthreads();
     // exclusive scan of S <= 198 counts by the first wavefront (4 per lane)
     if (tid < 64) {
@@ -1374,7 +1375,6 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
     const int nc = c_hi - c_lo;
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
-    const PairEntry pe = L.pt[0];
     const int bd = blockDim.x;
     const int cap = a.nbr_cap;
     for (int k = threadIdx.x; k < nc; k += bd) {
@@ -1410,7 +1410,7 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
                 const float dr2 = dx * dx + dy * dy;
                 float rl2;
                 if (kSigma) {
-                    const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
+                    const float rl = R.rc_list * ((si + pj.w) * 0.5f) + R.skin;
                     rl2 = (rl * rl) * 1.00002f;
                 } else {
                     rl2 = R.rl2_uniform;
@@ -1490,7 +1490,6 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
     const int nc = c_hi - c_lo;
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
-    const PairEntry pe = L.pt[0];
     const int bd = blockDim.x;  // held in a register across the loops
     const int tid = threadIdx.x;
     const int cap = a.nbr_cap, capm1 = a.nbr_cap - 1;
@@ -1538,7 +1537,7 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
                 const float dr2 = __builtin_fmaf(dx, dx, dy * dy);
                 float rl2;
                 if (kSigma) {
-                    const float rl = pe.rc * ((si + pj.w) * 0.5f) + R.skin;
+                    const float rl = R.rc_list * ((si + pj.w) * 0.5f) + R.skin;
                     rl2 = (rl * rl) * 1.00002f;
                 } else {
                     rl2 = R.rl2_uniform;
@@ -1580,7 +1579,6 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
     __shared__ int s_off[kMaxSlots + 1];
-    __shared__ PairEntry s_pt[kMaxPairTypes * kMaxPairTypes];
     __shared__ int s_hist[(kMaxListGroups + 1) * 4];
     PairLds L;
     L.ent = dyn;
@@ -1588,7 +1586,7 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     L.sp = nullptr;
     L.cstart = s_cstart;
     L.off = s_off;
-    L.pt = s_pt;
+    L.pt = nullptr;  // the list radius comes from the replica (rc_list)
     L.red = nullptr;
     // list buffer after the staging and its read headroom of 3 cells
     uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
@@ -1635,10 +1633,11 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // entries, so U_i = 4 eps (A - B) + n shift = 4 eps T - 2 eps (Q - kq L)
 // and W_i = 48 eps (A - B / 2) = 48 eps T (femmd2D/src/interactionsCuda.cu:
 // 91-96).  kCount (eps = 0, or the dump): n is counted instead.
-template <bool kWrap, bool kSigma, bool kEnergy, bool kUnit, bool kDebug, bool kCount, int kKick>
+template <bool kWrap, bool kSigma, bool kTypes, bool kEnergy, bool kUnit, bool kDebug, bool kCount,
+          int kKick>
 __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int rep, int W, int tile,
                                     const PairLds &L, const float2 *lp, const float *ls,
-                                    const int *lid) {
+                                    const int *lid, const int *lt) {
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -1646,8 +1645,10 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const PairEntry pe = L.pt[0];
     const float rc2_u = R.rc2_uniform;
-    // (not used when kCount: eps may be 0 there)
-    const float kq = kCount ? 0.0f : 2.0f * pe.shift / pe.e4;
+    // (not used when kCount: eps may be 0 there; nor with a type table,
+    // whose pairs accumulate their own eps-weighted terms)
+    const float kq = (kCount || kTypes) ? 0.0f : 2.0f * pe.shift / pe.e4;
+    const int nt = R.ntypes;
     const uint32_t Nu = (uint32_t)a.n;
     const int capg = a.nbr_cap >> 2;
     const uint64_t *nb = reinterpret_cast<const uint64_t *>(a.nbr);
@@ -1664,14 +1665,18 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
         const float xi = pi.x, yi = pi.y;
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
+        const int ti = kTypes ? lt[me] : 0;
         const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the sentinel slot
-        float fx = 0.f, fy = 0.f, accT = 0.f, accQ = 0.f;
+        // single type: T = sum_hit tv, Q as described above; with a type
+        // table: T = sum_hit e4 (s12 - s6), Q = sum_hit e48 tv, ush = sum_hit
+        // shift (femmd2D/src/interactions.cpp:347-389 per type pair)
+        float fx = 0.f, fy = 0.f, accT = 0.f, accQ = 0.f, ush = 0.f;
         int nint = 0;
         long long d1 = 0, d2 = 0;  // diagnostic dump: sums of (id + 1), (id + 1)^2
         // one list entry, without branches: the force of a non-interacting
         // entry (or of the padding entry) is computed and discarded by a
         // select
-        auto term = [&](float2 pj, float sj, int jo) {
+        auto term = [&](float2 pj, float sj, int tj, int jo) {
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
@@ -1685,8 +1690,16 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
             const float dx = ex - ox;
             const float dy = ey - oy;
             const float dr2 = dx * dx + dy * dy;
-            float s2ij, rc2;
-            if (kSigma) {
+            float s2ij, rc2, e48 = 0.0f, e4 = 0.0f, shift = 0.0f;
+            if (kTypes) {
+                const PairEntry q = L.pt[ti * nt + tj];
+                e48 = q.e48;
+                e4 = q.e4;
+                shift = q.shift;
+                const float sij = kSigma ? (si + sj) * 0.5f : R.s_uniform;
+                rc2 = q.rc * q.rc * sij * sij;
+                s2ij = sij * sij;
+            } else if (kSigma) {
                 const float sij = (si + sj) * 0.5f;
                 rc2 = pe.rc * pe.rc * sij * sij;
                 s2ij = sij * sij;
@@ -1713,11 +1726,15 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
             const float tv = __builtin_fmaf(-0.5f, sr6, s12);
             // (a non-interacting entry has r2a > 0, so its rcp is finite)
             const float tvh = hit ? tv : 0.0f;
-            const float fc = tvh * inv;
+            const float fc = kTypes ? e48 * tvh * inv : tvh * inv;
             fx = __builtin_fmaf(fc, dxa, fx);
             fy = __builtin_fmaf(fc, dya, fy);
             if (kCount) nint += hit ? 1 : 0;
-            if (kEnergy) {
+            if (kEnergy && kTypes) {
+                accT = __builtin_fmaf(e4, hit ? s12 - sr6 : 0.0f, accT);
+                accQ = __builtin_fmaf(e48, tvh, accQ);
+                ush += hit ? shift : 0.0f;
+            } else if (kEnergy) {
                 accT += tvh;
                 if (!kCount) accQ += hit ? sr6 : kq;
                 else accQ += hit ? sr6 : 0.0f;
@@ -1744,10 +1761,17 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
                 s2 = ls[j2 >> 3];
                 s3 = ls[j3 >> 3];
             }
-            term(q0, s0, j0);
-            term(q1, s1, j1);
-            term(q2, s2, j2);
-            term(q3, s3, j3);
+            int t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+            if (kTypes) {
+                t0 = lt[j0 >> 3];
+                t1 = lt[j1 >> 3];
+                t2 = lt[j2 >> 3];
+                t3 = lt[j3 >> 3];
+            }
+            term(q0, s0, t0, j0);
+            term(q1, s1, t1, j1);
+            term(q2, s2, t2, j2);
+            term(q3, s3, t3, j3);
         };
         // the next group is loaded while this one is evaluated (the group
         // index is clamped inside the array; a group past the list is loaded
@@ -1761,8 +1785,14 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
             group(cur);
         }
         const int id = a.perm[p];
-        tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx * pe.e48, fy * pe.e48, tk);
-        if (kEnergy) {
+        if (kTypes)
+            tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx, fy, tk);
+        else
+            tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx * pe.e48, fy * pe.e48, tk);
+        if (kEnergy && kTypes) {
+            Ublk += 0.5 * ((double)accT + (double)ush);
+            Wblk += 0.5 * (double)accQ;
+        } else if (kEnergy) {
             const double T = accT, Q = accQ, e4 = pe.e4;
             // kCount: Q = B exactly; otherwise Q - kq L = B - kq n
             const double Ui = kCount ? e4 * (T - 0.5 * Q) + (double)nint * pe.shift
@@ -1783,9 +1813,9 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
     }
 }
 
-template <bool kSigma, bool kEnergy, bool kDebug, int kKick>
+template <bool kSigma, bool kTypes, bool kEnergy, bool kDebug, int kKick>
 __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 *lp,
-                               float *ls, int *lid) {
+                               float *ls, int *lid, int *lt) {
     const Tile T = a.tiles[tile];
     const RepDesc R = a.reps[T.rep];
     const int W = T.w_flags & 0xffff;
@@ -1806,20 +1836,25 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
             if (kSigma) ls[e] = u.z;
             if (kDebug) lid[e] = __float_as_int(u.w);  // caller id (diagnostic dump)
         });
+    if (kTypes)  // the types in the same staging order (sorted positions)
+        stage_copy(
+            T, R, W, L, [&](int p) { return make_float4(__int_as_float(a.sorted_typ[p]), 0.f, 0.f, 0.f); },
+            [&](int e, const float4 &u) { lt[e] = __float_as_int(u.x); });
     if (threadIdx.x == 0) {  // the padding entries' slot, far from any particle
         lp[a.lds_cap] = make_float2(kSentinel, kSentinel);
         if (kSigma) ls[a.lds_cap] = 1.0f;
         if (kDebug) lid[a.lds_cap] = -1;
+        if (kTypes) lt[a.lds_cap] = 0;
     }
     __syncthreads();
     const bool wrapT = (T.w_flags >> 16) & 1;
 #define FM_LPB(WR, U, C) \
-    list_tile_pass_body<WR, kSigma, kEnergy, U, kDebug, C, kKick>(a, R, T.rep, W, tile, L, lp, ls, lid)
+    list_tile_pass_body<WR, kSigma, kTypes, kEnergy, U, kDebug, C, kKick>(a, R, T.rep, W, tile, L, lp, ls, lid, lt)
     // hits are counted for the dump, and for the energy when eps = 0
-    if (kDebug || (kEnergy && L.pt[0].e4 == 0.0f)) {
+    if (kDebug || (!kTypes && kEnergy && L.pt[0].e4 == 0.0f)) {
         if (wrapT) FM_LPB(true, false, true);
         else FM_LPB(false, false, true);
-    } else if (!kSigma && R.s2_uniform == 1.0f) {
+    } else if (!kSigma && R.s2_uniform == 1.0f) {  // (sigma^2 = 1: the same bits)
         if (wrapT) FM_LPB(true, true, false);
         else FM_LPB(false, true, false);
     } else {
@@ -2023,7 +2058,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             float2 *lp = reinterpret_cast<float2 *>(dyn);
             float *ls = reinterpret_cast<float *>(lp + a.lds_cap + 1);
             int *lid = reinterpret_cast<int *>(ls + (kSigma ? a.lds_cap + 1 : 0));
-            list_tile_pass<kSigma, kEnergy, kDebug, kKick>(a, xcd_remap(b, a.ntiles), L, lp, ls, lid);
+            int *lt = lid + (kDebug ? a.lds_cap + 1 : 0);
+            list_tile_pass<kSigma, kTypes, kEnergy, kDebug, kKick>(a, xcd_remap(b, a.ntiles), L, lp,
+                                                                   ls, lid, lt);
         }
     } else if (b < a.ntiles) {
         PairLds L;
@@ -2841,7 +2878,7 @@ __global__ __launch_bounds__(256) void k_cluster_virial(ClusterVirialArgs A) {
                 float s2ij;
                 unsigned rc2b;
                 if (kSigma || kTypes) {
-                    const float sij = kSigma ? (si + pj.w) * 0.5f : si;
+                    const float sij = kSigma ? (si + pj.w) * 0.5f : R.s_uniform;  // (no sigma staged)
                     rc2b = __float_as_uint(pe.rc * pe.rc * sij * sij);
                     s2ij = sij * sij;
                 } else {
@@ -3022,20 +3059,29 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
     if (debug) kick = 0;
     if (kick == 1) energy = true;
     if (list) {
-        // neighbour lists: single type (the caller checks); debug: the
+        // neighbour lists (types: the pair table per pair); debug: the
         // interacting-set dump over the lists (with energies)
         const size_t lds = (size_t)(a.lds_cap + 1) *
-                           (sizeof(float2) + (sigma ? sizeof(float) : 0) + (debug ? sizeof(int) : 0));
-#define FM_LIST(S, D, E, K) FM_GO((k_forces<S, false, D, true, E, K>), g, b, lds, s, a)
-#define FM_LIST_K(S, E)                    \
-    do {                                   \
-        if (kick == 2) FM_LIST(S, false, E, 2); \
-        else if (kick == 1) FM_LIST(S, false, true, 1); \
-        else FM_LIST(S, false, E, 0);      \
+                           (sizeof(float2) + (sigma ? sizeof(float) : 0) + (debug ? sizeof(int) : 0) +
+                            (types ? sizeof(int) : 0));
+#define FM_LIST(S, T, D, E, K) FM_GO((k_forces<S, T, D, true, E, K>), g, b, lds, s, a)
+#define FM_LIST_K(S, T, E)                    \
+    do {                                      \
+        if (kick == 2) FM_LIST(S, T, false, E, 2); \
+        else if (kick == 1) FM_LIST(S, T, false, true, 1); \
+        else FM_LIST(S, T, false, E, 0);      \
     } while (0)
-        if (debug) { if (sigma) FM_LIST(true, true, true, 0); else FM_LIST(false, true, true, 0); }
-        else if (sigma) { if (energy) FM_LIST_K(true, true); else FM_LIST_K(true, false); }
-        else { if (energy) FM_LIST_K(false, true); else FM_LIST_K(false, false); }
+#define FM_LIST_T(S, E)                       \
+    do {                                      \
+        if (types) FM_LIST_K(S, true, E); else FM_LIST_K(S, false, E); \
+    } while (0)
+        if (debug) {
+            if (sigma) { if (types) FM_LIST(true, true, true, true, 0); else FM_LIST(true, false, true, true, 0); }
+            else { if (types) FM_LIST(false, true, true, true, 0); else FM_LIST(false, false, true, true, 0); }
+        }
+        else if (sigma) { if (energy) FM_LIST_T(true, true); else FM_LIST_T(true, false); }
+        else { if (energy) FM_LIST_T(false, true); else FM_LIST_T(false, false); }
+#undef FM_LIST_T
 #undef FM_LIST_K
 #undef FM_LIST
         return hipGetLastError();

@@ -3,10 +3,14 @@ oracle (femmd2D/src/interactionsCuda.cu:43-100: sigma_ij = (sigma_i +
 sigma_j) / 2, eps and rc (in sigma units) and the additive shift from the
 row-major ntypes x ntypes tables, femmd2D/src/interactions.cpp:65-79).
 
-  * two types with different eps, rc and shift, mixed sigmas: the typed
-    cell walk (typed tables always walk the cells);
+  * two types with different eps, rc and shift, mixed sigmas, and six
+    types (more than the reference's usual two) with uniform and mixed
+    sigmas: the neighbour-list path (lists cover the table's largest cutoff,
+    each pair is then filtered by its own) and the cell walk;
   * one type with mixed sigmas: the neighbour-list path with per-particle
-    sigma (the list radius from the largest sigma pair) and the cell walk.
+    sigma (the list radius from the largest sigma pair) and the cell walk;
+  * a six-type system stepped 100 steps on the lists against the FP64
+    oracle's rollout.
 Interacting sets are bit-exact with the oracle's FP32 predicate; forces
 meet the 1e-5 bar of oracle.force_error; energies and virials 1e-5.
 
@@ -77,6 +81,65 @@ def test_two_type_table_with_mixed_sigma(skin):
     Lf = O.box_f32(L)
     _check(s, lambda px, py: O.OracleSystem(Lf, Lf, px, py, sigma=sig, type=typ, eps=eps, rc=rc,
                                             ushift=us))
+    if skin > 0:
+        assert s.list_stats()["rebuilds"] >= 1
+
+
+def _six_type_tables(rng):
+    nt = 6
+    e = rng.uniform(0.5, 1.5, (nt, nt))
+    r = rng.uniform(1.6, 2.6, (nt, nt))
+    u = rng.uniform(-0.05, 0.05, (nt, nt))
+    sym = lambda a: ((a + a.T) / 2).astype(np.float32).ravel()  # symmetric like the reference's tables
+    return nt, sym(e), sym(r), sym(u)
+
+
+@pytest.mark.parametrize("sigma", ["uniform", "mixed"])
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_six_type_table(skin, sigma):
+    L, n = 70.0, 3200
+    x, y, vx, vy, rng = _liquid(n, L, 21)
+    nt, eps, rc, us = _six_type_tables(rng)
+    typ = rng.integers(0, nt, n).astype(np.int32)
+    sig = (rng.uniform(0.9, 1.1, n) if sigma == "mixed" else np.ones(n)).astype(np.float32)
+    s = System(L, L, dt=0.002)
+    s.set_skin(skin)
+    s.set_pair_potential(eps, rc, us)
+    s.add_particles(x, y, vx, vy, 1.0, sig, typ)
+    s.allocate_cuda()
+    Lf = O.box_f32(L)
+    _check(s, lambda px, py: O.OracleSystem(Lf, Lf, px, py, sigma=sig, type=typ, eps=eps, rc=rc,
+                                            ushift=us))
+    if skin > 0:
+        assert s.list_stats()["rebuilds"] >= 1
+
+
+def test_six_type_lists_trajectory():
+    L, n, dt = 70.0, 3200, 0.004
+    x, y, vx, vy, rng = _liquid(n, L, 22)
+    nt, eps, rc, us = _six_type_tables(rng)
+    typ = rng.integers(0, nt, n).astype(np.int32)
+    s = System(L, L, dt=dt)
+    s.set_skin(0.6)
+    s.set_pair_potential(eps, rc, us)
+    s.add_particles(x, y, vx, vy, 1.0, 1.0, typ)
+    s.allocate_cuda()
+    x0, y0, vx0, vy0 = s.get_state()
+    Lf = O.box_f32(L)
+    osys = O.OracleSystem(Lf, Lf, x0, y0, type=typ, eps=eps, rc=rc, ushift=us)
+    s.run(100)
+    x1, y1, vx1, vy1 = s.get_state()
+    ox, oy, ovx, ovy, _ = O.run(osys, x0, y0, vx0, vy0, np.ones(n), 100, dt)
+    dx = x1 - ox
+    dx -= Lf * np.round(dx / Lf)
+    dy = y1 - oy
+    dy -= Lf * np.round(dy / Lf)
+    dpos = np.hypot(dx, dy)
+    print(f"  six types, lists: max |dx| {dpos.max():.3e} rms {np.sqrt((dpos ** 2).mean()):.3e}")
+    assert dpos.max() < 2e-3
+    assert np.sqrt((dpos ** 2).mean()) < 1e-4
+    assert s.list_stats()["rebuilds"] >= 1
+    assert s.device_errors() == 0
 
 
 @pytest.mark.parametrize("skin", [0.6, 0.4, 0.0])
