@@ -128,7 +128,8 @@ class System:
         return first.value
 
     def add_elastomer(self, nodes, cells, x, y, typ=0, g=0, sig=1.0, mass=0.0, exclude=True,
-                      *, index_base=1, node_mass=0.0, E=None, nu=None) -> int:
+                      *, index_base=1, node_mass=0.0, E=None, nu=None, edge_bonds=None,
+                      bond_k=0.0) -> int:
         """System.add_elastomer(nodes, cells, x, y, typ, g, sig, mass, exclude)
         (femmd2D/src/interface.cpp:533-563, femmd2D/src/particles.cpp:295-385).
 
@@ -137,8 +138,14 @@ class System:
         index_base says otherwise; mass: total mass spread as rho*A/3 per
         corner, or node_mass per node when given.  g (the reference's group
         id) only tags particles for its group observables and is ignored.
-        E / nu: optional per-triangle arrays.  Returns the first node index.
+        E / nu: optional per-triangle arrays.  edge_bonds: None (no bonds),
+        "all" (the reference's edge bonds, femmd2D/src/particles.cpp:335-337:
+        every triangle's three edges, shared edges twice) or "dedup" (each
+        edge once), stiffness bond_k, r0 the current length.  Returns the
+        first node index.
         """
+        if edge_bonds not in (None, "all", "dedup"):
+            raise ValueError("edge_bonds must be None, 'all' or 'dedup'")
         v = _f32(np.asarray(nodes, np.float32).reshape(-1))
         t = _i32(np.asarray(cells, np.int32).reshape(-1))
         nv, nt = v.shape[0] // 2, t.shape[0] // 3
@@ -153,6 +160,10 @@ class System:
         self._rec_typ.append(np.full(nv, int(typ), np.int32))
         self._rec_sig.append(np.full(nv, float(sig), np.float32))
         self._rec_tri.append(t.reshape(-1, 3) - int(index_base) + first.value)
+        if edge_bonds is not None:
+            self._check(self._lib.fm_add_edge_bonds(self._h, first.value, nt, _ip(t),
+                                                    int(index_base), float(bond_k),
+                                                    int(edge_bonds == "dedup")))
         return first.value
 
     @property

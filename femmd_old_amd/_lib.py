@@ -96,6 +96,7 @@ PROTOTYPES = {
                                    C.c_float, C.c_int32, C.c_float, C.c_float, C.c_float,
                                    C.c_int32, _f, _f, _i]),
     "fm_add_bonds": (C.c_int, [_h, C.c_int32, _i, _i, _f, _f]),
+    "fm_add_edge_bonds": (C.c_int, [_h, C.c_int32, C.c_int32, _i, C.c_int32, C.c_float, C.c_int32]),
     "fm_set_pair_table": (C.c_int, [_h, C.c_int32, _f, _f, _f]),
     "fm_set_material": (C.c_int, [_h, C.c_int32, C.c_float, C.c_float]),
     "fm_set_dt": (C.c_int, [_h, C.c_double]),

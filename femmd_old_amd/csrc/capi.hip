@@ -11,6 +11,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <set>
 #include <vector>
 
 #include "handles.hpp"
@@ -221,6 +222,41 @@ int fm_add_bonds(fm_system *h, int32_t nb, const int32_t *i, const int32_t *j, c
         S.br0.push_back(len);
     }
     return FM_OK;
+    FM_END(h)
+}
+
+// Particles::addElastomer's edge bonds (femmd2D/src/particles.cpp:335-337:
+// createBond on the three edges of every triangle, shared edges twice),
+// explicit and opt-in; dedup keeps the first bond of each edge.
+int fm_add_edge_bonds(fm_system *h, int32_t first_node, int32_t nt, const int32_t *tris,
+                      int32_t index_base, float k, int32_t dedup) {
+    FM_GUARD(h)
+    if (frozen(h)) return fail(h, FM_ERR_STATE, "system already committed");
+    if (nt < 0 || (nt > 0 && !tris)) return fail(h, FM_ERR_ARG, "bad triangle array");
+    if (!std::isfinite(k)) return fail(h, FM_ERR_ARG, "bond stiffness must be finite");
+    const int64_t n = (int64_t)h->hs.x.size();
+    std::vector<int32_t> bi, bj;
+    bi.reserve(3 * (size_t)nt);
+    bj.reserve(3 * (size_t)nt);
+    std::set<std::pair<int32_t, int32_t>> seen;
+    for (int32_t t = 0; t < nt; ++t) {
+        int32_t v[3];
+        for (int q = 0; q < 3; ++q) {
+            const int64_t u = (int64_t)first_node + tris[3 * t + q] - index_base;
+            if (u < 0 || u >= n) return fail(h, FM_ERR_ARG, "triangle node out of range");
+            v[q] = (int32_t)u;
+        }
+        const int32_t e[3][2] = {{v[0], v[1]}, {v[0], v[2]}, {v[1], v[2]}};
+        for (const auto &ed : e) {
+            if (ed[0] == ed[1]) return fail(h, FM_ERR_ARG, "degenerate triangle");
+            if (dedup && !seen.insert({std::min(ed[0], ed[1]), std::max(ed[0], ed[1])}).second)
+                continue;
+            bi.push_back(ed[0]);
+            bj.push_back(ed[1]);
+        }
+    }
+    const std::vector<float> kk(bi.size(), k);
+    return fm_add_bonds(h, (int32_t)bi.size(), bi.data(), bj.data(), kk.data(), nullptr);
     FM_END(h)
 }
 

@@ -135,8 +135,19 @@ int fm_add_elastomer(fm_system *h, int32_t nv, const float *verts, int32_t nt,
 int fm_add_bonds(fm_system *h, int32_t nb, const int32_t *i, const int32_t *j,
                  const float *k, const float *r0);
 
+/* The edge bonds of Particles::addElastomer (femmd2D/src/particles.cpp:
+ * 335-337: Bonds::createBond on the three edges of every triangle, r0 =
+ * the current distance, an edge shared by two triangles bonded twice).
+ * Opt-in here (fm_add_elastomer creates none): tris / index_base as given
+ * to fm_add_elastomer, first_node its result; every bond gets stiffness k;
+ * dedup != 0 bonds each edge once.  Edges are taken in the reference's
+ * order (i0-i1, i0-i2, i1-i2 per triangle). */
+int fm_add_edge_bonds(fm_system *h, int32_t first_node, int32_t nt, const int32_t *tris,
+                      int32_t index_base, float k, int32_t dedup);
+
 /* Interactions::setPairForce (femmd2D/src/interactions.cpp:65-79): row-major
- * ntypes x ntypes tables; rcut in sigma units; ushift additive. */
+ * ntypes x ntypes tables (ntypes in [1, 16]); rcut in sigma units; ushift
+ * additive. */
 int fm_set_pair_table(fm_system *h, int32_t ntypes, const float *eps,
                       const float *rcut, const float *ushift);
 
