@@ -19,6 +19,7 @@ MESH = Path(__file__).resolve().parent / "golden" / "meshes"
 @pytest.mark.parametrize("mode", ["all", "dedup"])
 def test_edge_bonds_rest_length_and_stiffness(mode):
     v, t = load_mesh(str(MESH / "diskVertices.txt"), str(MESH / "diskTriangles.txt"))
+    v = v * np.float32(8.0)  # edges of about one sigma (the unit disk mesh is 1 across)
     L, k = 40.0, 12.5
     s = System(L, L, dt=0.002)
     s.set_lj(False)
