@@ -84,6 +84,8 @@ struct ObsArgs {
     const double *fea_part;    // 2 per element span: U, W
     const double *bond_part;   // 2 per bond span: U, W
     double *chunk;             // kObsChunks * kObsRaw per replica (level 1)
+    int32_t nchunk;            // level-1 chunks per replica (1..kObsChunks; of
+                               // the replica's partials in order)
     double *obs;               // kObsStride per replica
     int32_t nrep;
 };

@@ -999,6 +999,10 @@ ObsArgs Engine::obs_args() {
     o.chunk = obs_chunk_.p;
     o.obs = obs_.p;
     o.nrep = R_;
+    // a single system's partials in 16 chunks (one workgroup each); a batch
+    // of many replicas one chunk per replica (each replica's partials are
+    // few, and the jobs are many)
+    o.nchunk = R_ > 8 ? 1 : kObsChunks;
     return o;
 }
 

@@ -135,10 +135,10 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
 // run 256), so its bits do not depend on which kernel forms it.  Level 2:
 // one thread sums the replica's chunks in chunk order.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void obs_slice(int b, int e, int c, int &lo, int &hi) {
+__device__ __forceinline__ void obs_slice(int b, int e, int c, int nc, int &lo, int &hi) {
     const long long len = e - b;
-    lo = b + (int)(len * c / kObsChunks);
-    hi = b + (int)(len * (c + 1) / kObsChunks);
+    lo = b + (int)(len * c / nc);
+    hi = b + (int)(len * (c + 1) / nc);
 }
 
 // Level 1 for chunk c of replica r: out[0, 8) of the chunk (see kObsRaw).
@@ -149,19 +149,19 @@ __device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
     double v[kObsRaw] = {0, 0, 0, 0, 0, 0, 0, 0};
     int lo, hi;
     if (o.ke_kick) {  // a query: the partials of k_ke over the kick spans
-        obs_slice(g[0], g[1], c, lo, hi);
+        obs_slice(g[0], g[1], c, o.nchunk, lo, hi);
         for (int k = lo + tid; k < hi; k += 256) {
             v[0] += o.ke_part[2 * k];
             v[1] = fmax(v[1], o.ke_part[2 * k + 1]);
         }
     } else {  // a step: the particles the node kick integrated
-        obs_slice(o.nrng[2 * r], o.nrng[2 * r + 1], c, lo, hi);
+        obs_slice(o.nrng[2 * r], o.nrng[2 * r + 1], c, o.nchunk, lo, hi);
         for (int k = lo + tid; k < hi; k += 256) {
             v[0] += o.ke_node[2 * k];
             v[1] = fmax(v[1], o.ke_node[2 * k + 1]);
         }
     }
-    obs_slice(g[2], g[3], c, lo, hi);
+    obs_slice(g[2], g[3], c, o.nchunk, lo, hi);
     for (int k = lo + tid; k < hi; k += 256) {
         v[2] += o.lj_part[2 * k];
         v[3] += o.lj_part[2 * k + 1];
@@ -170,12 +170,12 @@ __device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
             v[1] = fmax(v[1], o.ke_tile[2 * k + 1]);
         }
     }
-    obs_slice(g[4], g[5], c, lo, hi);
+    obs_slice(g[4], g[5], c, o.nchunk, lo, hi);
     for (int k = lo + tid; k < hi; k += 256) {
         v[4] += o.fea_part[2 * k];
         v[5] += o.fea_part[2 * k + 1];
     }
-    obs_slice(g[6], g[7], c, lo, hi);
+    obs_slice(g[6], g[7], c, o.nchunk, lo, hi);
     for (int k = lo + tid; k < hi; k += 256) {
         v[6] += o.bond_part[2 * k];
         v[7] += o.bond_part[2 * k + 1];
@@ -207,8 +207,8 @@ __device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
 // The level-1 jobs (replica, chunk) of one step, spread over a kernel's
 // workgroups (256 threads each; uniform per workgroup).
 __device__ void obs_level1(const ObsArgs &o, double *red) {
-    const int njob = o.nrep * kObsChunks;
-    for (int q = blockIdx.x; q < njob; q += gridDim.x) obs_chunk(o, q / kObsChunks, q % kObsChunks, red);
+    const int njob = o.nrep * o.nchunk;
+    for (int q = blockIdx.x; q < njob; q += gridDim.x) obs_chunk(o, q / o.nchunk, q % o.nchunk, red);
 }
 
 // Level 2 for replica r (one thread): the step's totals from its chunks in
@@ -217,7 +217,7 @@ __device__ void obs_level1(const ObsArgs &o, double *red) {
 __device__ void obs_finish(const ObsArgs &o, int r, bool accumulate) {
     const double *ch = o.chunk + (size_t)r * kObsChunks * kObsRaw;
     double t[kObsRaw] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < kObsChunks; ++c) {
+    for (int c = 0; c < o.nchunk; ++c) {
         const double *q = ch + c * kObsRaw;
         for (int k = 0; k < kObsRaw; ++k) t[k] = (k == 1) ? fmax(t[1], q[1]) : t[k] + q[k];
     }
@@ -2119,7 +2119,7 @@ __global__ __launch_bounds__(64) void k_observe_l2(ObsArgs o, int accumulate) {
 
 hipError_t launch_observe(const ObsArgs &o, bool accumulate, hipStream_t s) {
     if (o.nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_observe_l1, dim3(std::min(o.nrep * kObsChunks, 4096)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_observe_l1, dim3(std::min(o.nrep * o.nchunk, 4096)), dim3(256), 0, s,
                        o);
     hipLaunchKernelGGL(k_observe_l2, dim3((o.nrep + 63) / 64), dim3(64), 0, s, o,
                        accumulate ? 1 : 0);
