@@ -496,6 +496,7 @@ def main():
     t2 = time.perf_counter()
     s.run(args.long_steps)
     t3 = time.perf_counter()
+    e2 = s.energies()
     s.set_observe_every(0)
     s.run(max(args.warmup, 1))
     t4 = time.perf_counter()
@@ -588,6 +589,9 @@ def main():
                                   "pressure, formed on the device and accumulated into running "
                                   "averages (Integrator::doStep -> Properties::compute)"},
         "energy_drift": (E1 - E0) / abs(E0),
+        "energy_drift_long": {"steps": args.steps + args.long_steps,
+                              "relative": (e2["ke"] + e2["u_lj"] + e2["u_fea"] + e2["u_bond"] - E0)
+                              / abs(E0)},
         "averages": {"steps": avg["count"], "temperature": avg["uKinetic"]["avg"],
                      "pressure": avg["pressure"]["avg"], "uPotential": avg["uPotential"]["avg"]},
         "steady_state": {"steps": args.long_steps,

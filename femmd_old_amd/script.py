@@ -29,7 +29,15 @@ Departures (each raises or is documented rather than silently different):
     ``set_kArea`` with a non-zero value raise (walls, gravity and area forces
     are not part of this engine);
   * ``set_velocities`` draws its angles from numpy's generator (seed
-    ``velocity_seed``), not the reference's RANDOM01.
+    ``velocity_seed``), not the reference's RANDOM01;
+  * ``set_kBond(k)`` with k > 0 switches bonds on as the reference does
+    (femmd2D/src/interface.cpp:455-465): every elastomer then carries the
+    bonds ``Particles::addElastomer`` creates on its triangle edges
+    (femmd2D/src/particles.cpp:335-337), shared edges twice unless
+    ``edge_bonds`` is set to "dedup", and every bond, edge or
+    ``create_bond``, has stiffness k (the global kBond of the CPU bond loop,
+    femmd2D/src/interactions.cpp:143-173; the last value set before
+    ``allocate_cuda`` counts).  With k <= 0 no bond acts, and none is built.
 """
 from __future__ import annotations
 
@@ -109,6 +117,7 @@ class ScriptSystem:
         self._factory = engine_factory
         self.velocity_seed = 1234
         self.sample_every = 1
+        self.edge_bonds = "all"  # or "dedup": each shared triangle edge bonded once
         self.reset()
 
     # -- set-up -------------------------------------------------------------
@@ -214,7 +223,7 @@ class ScriptSystem:
                              float(sig), float(mass), bool(exclude))))
 
     def create_bond(self, i: int, j: int):
-        self._bonds.append((int(i), int(j), self._kbond))
+        self._bonds.append((int(i), int(j)))
 
     def set_velocities(self, v0: float):
         """Particles::setParticlesVel (femmd2D/src/particles.cpp:233-261):
@@ -248,18 +257,23 @@ class ScriptSystem:
             s.set_skin(self._skin)
         if self._pair is not None:
             s.set_pair_potential(*self._pair)
+        if self.edge_bonds not in ("all", "dedup"):
+            raise ValueError("edge_bonds must be 'all' or 'dedup'")
+        kb = self._kbond
+        edges = self.edge_bonds if kb > 0 else None
         el = dict(self._elast)
         for k, b in enumerate(self._parts):
             if k in el:
                 a = el[k]
-                s.add_elastomer(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8])
+                s.add_elastomer(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8],
+                                edge_bonds=edges, bond_k=kb)
             else:
                 s.add_particles(b["x"], b["y"], b["vx"], b["vy"], b["m"], b["sig"], b["typ"])
         if self._E is not None:
             s.set_youngs_modulus(self._E, self._nu)
-        if self._bonds:
-            i, j, kk = zip(*self._bonds)
-            s.add_bonds(list(i), list(j), list(kk))
+        if self._bonds and kb > 0:
+            i, j = zip(*self._bonds)
+            s.add_bonds(list(i), list(j), [kb] * len(i))
         vx = np.concatenate([b["vx"] for b in self._parts])
         vy = np.concatenate([b["vy"] for b in self._parts])
         s.set_state(None, None, vx, vy)

@@ -237,6 +237,23 @@ def force_error(fx, fy, rx, ry, fs):
     return d / np.where(den > 0, den, 1.0)
 
 
+def force_error_terms(fx, fy, rx, ry, fs, tol=1e-5):
+    """Which denominator term of force_error binds, per particle: the
+    fraction of particles where each of |f_ref|, sum_j |f_ij| and
+    0.1 * sum_j r_ij |df_ij/dr_ij| is the largest, and the largest error
+    and the number of particles above tol under the two-term rule
+    max(|f_ref|, sum_j |f_ij|) alone."""
+    d = np.hypot(np.asarray(fx, float) - rx, np.asarray(fy, float) - ry)
+    terms = np.stack([np.hypot(rx, ry), np.asarray(fs[0], float), 0.1 * np.asarray(fs[1], float)])
+    which = np.argmax(terms, axis=0)
+    den2 = np.maximum(terms[0], terms[1])
+    e2 = d / np.where(den2 > 0, den2, 1.0)
+    n = max(1, d.shape[0])
+    return {"net": float(np.mean(which == 0)), "cancel": float(np.mean(which == 1)),
+            "resolution": float(np.mean(which == 2)), "max_two_term": float(e2.max(initial=0.0)),
+            "above_tol_two_term": int(np.count_nonzero(e2 > tol)), "n": n}
+
+
 def run(osys: OracleSystem, x, y, vx, vy, mass, nsteps, dt, pred_f32=False, history=False):
     """FP64 KDK rollout; returns (x, y, vx, vy, hist or None)."""
     x, y, vx, vy = (np.array(a, np.float64) for a in (x, y, vx, vy))

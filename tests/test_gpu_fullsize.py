@@ -53,6 +53,10 @@ def test_full_size_forces_energies_momentum(cfg):
         rx, ry, en, fs = O.forces(osys, pred_f32=True, scale=True)
         err = O.force_error(fx, fy, rx, ry, fs)
         worst = max(worst, float(err.max()))
+        t = O.force_error_terms(fx, fy, rx, ry, fs)
+        print(f"[{cfg} step {40 * k}] force bar binds on |f| {t['net']:.4f}, cancellation "
+              f"{t['cancel']:.4f}, FP32 resolution {t['resolution']:.4f} of particles; two-term "
+              f"rule: max {t['max_two_term']:.3e}, {t['above_tol_two_term']} of {t['n']} above 1e-5")
         e = check_energies(s, en, w)
         print(f"[{cfg} step {40 * k}] w_fea {e['w_fea']:.6e} (oracle {en[O.OR_W_FEA]:.6e}), "
               f"w_bond {e['w_bond']:.6e} (oracle {en[O.OR_W_BOND]:.6e}), P {e['pressure']:.6e}")

@@ -9,7 +9,12 @@ of the last averaging block, femmd2D/src/properties.cpp:15-98) are checked
 against the same quantities formed from an FP64 oracle rollout of the same
 study (same initial state, masses, elements and exclusions; 400 steps),
 relative 1e-4 (the virial and the pressure relative to the sum of the
-magnitudes of the element virial's terms, which cancel)."""
+magnitudes of the element virial's terms, which cancel).  Run once without
+bonds and once with System.set_kBond(k > 0), which (as in the reference,
+femmd2D/src/interface.cpp:455-465 with femmd2D/src/particles.cpp:335-337)
+bonds every triangle edge of every elastomer, shared edges twice, rest
+length the edge's length at creation; the oracle then carries those bonds
+independently built from the mesh."""
 import math
 import sys
 from pathlib import Path
@@ -25,7 +30,8 @@ pytestmark = pytest.mark.gpu
 MESH = Path(__file__).resolve().parent / "golden" / "meshes"
 
 
-def test_test_js_study(tmp_path, monkeypatch, capsys):
+@pytest.mark.parametrize("kbond", [0.0, 20.0])
+def test_test_js_study(tmp_path, monkeypatch, capsys, kbond):
     monkeypatch.chdir(tmp_path)
     S = script.ScriptSystem()
     S.sample_every = 50
@@ -54,10 +60,14 @@ def test_test_js_study(tmp_path, monkeypatch, capsys):
             for a in (0.0, 0.5):
                 S.add_elastomer(nodes, cells, (nx + a) * b, (ny + a) * b, 0, 0, 1.0, N, True)
     S.set_velocities(v0)
+    if kbond:
+        S.set_kBond(kbond)
     assert S.info()["nelastomers"] == 2 * ncells * ncells
     S.allocate_cuda()
     S.init_neighbor_list()
-    oracle_props = _oracle_study(S, nodes, cells, ncells, b, N, scale * L)
+    ntri = cells.size // 3 * 2 * ncells * ncells
+    assert S.engine.n_bonds() == (3 * ntri if kbond else 0)
+    oracle_props = _oracle_study(S, nodes, cells, ncells, b, N, scale * L, kbond)
     script.make_dir("data")
     script.change_dir("data")
     S.save_trajectory(True)
@@ -94,11 +104,13 @@ def test_test_js_study(tmp_path, monkeypatch, capsys):
     assert Path("traj.xyz").exists() and len(list(Path(".").glob("movie*.vtk"))) == 2
 
 
-def _oracle_study(S, nodes, cells, ncells, b, N, L):
+def _oracle_study(S, nodes, cells, ncells, b, N, L, kbond):
     """An FP64 oracle of the script's engine system (the same elastomers in
     the same order: masses spread as rho A / 3 per corner, one exclusion
-    group each, reference shapes at their placement) and a function giving
-    the averaged properties over the samples at the given steps."""
+    group each, reference shapes at their placement; with kbond > 0 a bond
+    of stiffness kbond on each of every triangle's three edges, rest length
+    the edge's length in the reference shape) and a function giving the
+    averaged properties over the samples at the given steps."""
     e = S.engine
     x0, y0, vx0, vy0 = e.get_state()
     n = x0.shape[0]
@@ -119,9 +131,17 @@ def _oracle_study(S, nodes, cells, ncells, b, N, L):
                for a in (0.0, 0.5)]
     refx = np.concatenate([verts[:, 0] + np.float32(cx) for cx, _ in centres])
     refy = np.concatenate([verts[:, 1] + np.float32(cy) for _, cy in centres])
+    bonds = None
+    if kbond:
+        edges = np.concatenate([tri1[:, [0, 1]], tri1[:, [0, 2]], tri1[:, [1, 2]]])
+        bi = np.concatenate([edges[:, 0] + c * nv for c in range(len(centres))])
+        bj = np.concatenate([edges[:, 1] + c * nv for c in range(len(centres))])
+        r0 = np.hypot(refx[bi].astype(np.float64) - refx[bj], refy[bi].astype(np.float64) - refy[bj])
+        bonds = (bi, bj, np.full(bi.shape[0], kbond), r0)
     Lf = O.box_f32(L)
     osys = O.OracleSystem(Lf, Lf, x0, y0, excl=excl, eps=[1.0], rc=[2 ** (1 / 6)], ushift=[1.0],
-                          tri=e.triangles(), refx=refx, refy=refy, E=50.0, nu=0.3, model=0)
+                          bonds=bonds, tri=e.triangles(), refx=refx, refy=refy, E=50.0, nu=0.3,
+                          model=0)
     _, _, _, _, hist = O.run(osys, x0, y0, vx0, vy0, mass, 400, float(np.float32(0.001)),
                              history=True)
     V = Lf * Lf

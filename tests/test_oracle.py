@@ -530,3 +530,16 @@ def test_oracle_under_host_sanitizers():
         assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
     finally:
         (odir / "sanitize_check").unlink(missing_ok=True)
+
+
+def test_force_error_terms_report():
+    # particle 0: |f| binds; 1: the cancellation term; 2: the resolution term
+    rx, ry = np.array([3.0, 0.1, 0.0]), np.zeros(3)
+    fs = (np.array([1.0, 2.0, 0.1]), np.array([1.0, 1.0, 50.0]))
+    fx = rx + np.array([3e-6, 1e-5, 1e-5])
+    t = O.force_error_terms(fx, ry, rx, ry, fs)
+    assert (t["net"], t["cancel"], t["resolution"]) == pytest.approx((1 / 3, 1 / 3, 1 / 3))
+    # two-term rule: 1e-6, 5e-6, 1e-4 -> one above 1e-5
+    assert t["max_two_term"] == pytest.approx(1e-4)
+    assert t["above_tol_two_term"] == 1 and t["n"] == 3
+    assert O.force_error(fx, ry, rx, ry, fs).max() == pytest.approx(5e-6)

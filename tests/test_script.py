@@ -22,7 +22,7 @@ class _Rec:
 
     def __getattr__(self, name):
         def f(*a, **k):
-            self.calls.append((name,) + a)
+            self.calls.append((name,) + a + ((k,) if k else ()))
         return f
 
 
@@ -56,6 +56,37 @@ def test_script_replays_setup(tmp_path):
         S.add_particle(0, 0, 0, 0, 1, 1, 0)
     with pytest.raises(NotImplementedError):
         S.set_gravity(1.0)
+
+
+@pytest.mark.parametrize("k, mode", [(0.0, "all"), (5.0, "all"), (5.0, "dedup")])
+def test_script_set_kbond_switches_edge_bonds_on(k, mode):
+    """System.set_kBond (femmd2D/src/interface.cpp:455-465): with k > 0 the
+    bonds Particles::addElastomer puts on every triangle edge
+    (femmd2D/src/particles.cpp:335-337) act, and every bond, edge or
+    create_bond, has stiffness k (the last value set); with k <= 0 none."""
+    S = script.ScriptSystem(engine_factory=_Rec)
+    S.edge_bonds = mode
+    S.init(40.0, 40.0)
+    nodes = script.read_file(str(MESH / "diskVertices.txt"), script.FILECONTENT.FLOAT)
+    cells = script.read_file(str(MESH / "diskTriangles.txt"), script.FILECONTENT.INT)
+    S.add_elastomer(nodes * 8, cells, 20.0, 20.0, 0, 0, 1.0, 10.0, True)
+    S.add_particle(1.0, 2.0, 0.0, 0.0, 1.0, 1.0, 0)
+    S.set_kBond(1.0)
+    S.create_bond(0, 71)
+    S.set_kBond(k)
+    S.allocate_cuda()
+    calls = {c[0]: c for c in _Rec.last.calls}
+    kw = calls["add_elastomer"][-1]
+    assert kw == {"edge_bonds": mode if k > 0 else None, "bond_k": k}
+    if k > 0:
+        assert calls["add_bonds"][1:] == ([0], [71], [k])
+    else:
+        assert "add_bonds" not in calls
+    S = script.ScriptSystem(engine_factory=_Rec)
+    S.init(40.0, 40.0)
+    S.edge_bonds = "some"
+    with pytest.raises(ValueError):
+        S.allocate_cuda()
 
 
 def test_script_helpers(tmp_path, monkeypatch):
