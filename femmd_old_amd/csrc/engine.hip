@@ -1113,6 +1113,13 @@ int Engine::launch_reorder_now() {
     if (launch_reorder(a, stream_) != hipSuccess) return -1;
     order_stale_ = true;
     ++nreorder_;
+    if (!list_ && bins_pending_) {
+        // the cell walk's pending binning (made by the last drift) holds the
+        // old ids: bin the current positions again under the new ones (the
+        // cells are cleared first) for the next step's sort
+        if (launch_bin()) return -1;
+        bins_pending_ = true;
+    }
     return 0;
 }
 

@@ -5,9 +5,10 @@ new internal ids in the current cell order.  It is a pure relabelling: the
 state, the forces, the cluster properties, the interacting sets and the
 cell keys read in caller order are bitwise the same just after it; J w
 (recomputed from lists built after it) agrees to FP32 summation order.
-After further steps the run stays within the parity bars, and a batch and
-the same systems stepped alone still agree bitwise when both re-sort on
-the same schedule.
+After further steps the run stays within the parity bars, re-sorts inside
+a run (neighbour lists and the per-step cell walk) keep it on the
+trajectory of the run without them, and a batch and the same systems
+stepped alone still agree bitwise when both re-sort on the same schedule.
 """
 import numpy as np
 import pytest
@@ -90,4 +91,38 @@ def test_reorder_schedule_batched_equals_looped():
     for r, st in enumerate(singles):
         for a, c in zip(st, b.get_state(r)):
             np.testing.assert_array_equal(a, c)
+    assert b.device_errors() == 0
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.6])
+def test_reorder_inside_a_run(skin):
+    """Re-sorts between the pieces of a run (every 3 steps here), after a
+    drift: with the per-step cell walk the drift has already binned the new
+    positions under the old ids, which the re-sort must not leave behind.
+    The run stays on the trajectory of the same run without re-sorts (FP32
+    summation order aside) and its final forces meet the parity bar."""
+    w = configs.c2(scale=0.02)
+    runs = []
+    for every in (0, 3):
+        s = _shuffled(w, skin, every)
+        s.run(40)
+        runs.append(s)
+    a, b = runs
+    assert a.reorder_info()["count"] == 0 and b.reorder_info()["count"] >= 4
+    xa, ya, _, _ = a.get_state()
+    xb, yb, _, _ = b.get_state()
+    L = np.float32(w.Lx)
+    dx = xb - xa
+    dx -= L * np.round(dx / L)
+    dy = yb - ya
+    dy -= L * np.round(dy / L)
+    d = np.hypot(dx, dy)
+    print(f"[reorder in run, skin {skin}] max |dx| vs no re-sort {d.max():.3e}")
+    assert d.max() < 1e-3
+    fx, fy = b.get_forces()
+    rx, ry, en, fs = O.forces(O.from_workload(w, xb, yb), pred_f32=True, scale=True)
+    assert O.force_error(fx, fy, rx, ry, fs).max() <= 1e-5
+    ea, eb = a.energies(), b.energies()
+    for k in ("u_lj", "ke"):
+        assert abs(eb[k] - ea[k]) <= 1e-4 * abs(ea[k])
     assert b.device_errors() == 0
