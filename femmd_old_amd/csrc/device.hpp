@@ -23,6 +23,10 @@ constexpr int kElemThreads = 256;
 constexpr int kMaxPairTypes = 16;  // particle types of a pair table
 constexpr int kMaxListCap = 512;  // neighbour-list capacity per particle
 constexpr int kMaxListGroups = kMaxListCap / 4;
+// A list slot's word: length in the low bits, kListImg when the particle's
+// neighbours include a halo cell across the periodic boundary (its entries
+// need the minimum image), the particle's index in its tile above bit 16.
+constexpr int kListLen = 0x7fff, kListImg = 0x8000;
 
 // Per-replica geometry and integration constants.
 struct alignas(16) RepDesc {
@@ -254,8 +258,10 @@ struct ForceArgs {
                                // p as uint16 byte offsets of float2 slots of
                                // its tile's LDS staging, four per uint64
                                // group, group g at element g * n + p
-    int32_t *nnum;             // per list slot q: length | (k << 16), k the
-                               // tile index of the particle whose list it is
+    int32_t *nnum;             // per list slot q: length | kListImg when its
+                               // entries need the minimum image | (k << 16),
+                               // k the tile index of the particle whose list
+                               // it is
     int32_t lbuf;              // list build: stage entries in LDS (coalesced
                                // column writes) when set
     int32_t gated_grid;        // list build grid when gated (resident workgroups)

@@ -1674,7 +1674,7 @@ int Engine::list_stats(int r, double *mean, int32_t *maxlen, int64_t *nbuild, st
     double s = 0.0;
     int32_t m = 0;
     for (int32_t v : nn) {
-        v &= 0xffff;  // length | (tile index << 16)
+        v &= kListLen;  // length | image flag | (tile index << 16)
         s += v;
         m = std::max(m, v);
     }

@@ -1368,8 +1368,29 @@ __device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p
 
 // List build of one staged tile, entries written straight to the list (used
 // when the LDS holds no list buffer: one 2-byte store per entry).
+// Which centre columns (1 .. W in the staging) of a tile need the minimum
+// image: those next to a halo column across the periodic boundary in x, and
+// all of them when a halo row crosses it in y or the box has fewer than 5
+// cells along an axis (elsewhere a neighbour is less than 2 cells <= 0.4 L
+// away in the same periodic image, so the image shift is zero).
+struct ImgCols {
+    bool all, left, right;
+    __device__ bool needs(int col, int W) const {
+        return all || (left && col == 1) || (right && col == W);
+    }
+};
+__device__ __forceinline__ ImgCols img_cols(const Tile &T, const RepDesc &R, int W) {
+    ImgCols c;
+    c.all = R.ncx < 5 || R.ncy < 5 || T.cy == 0 || T.cy >= R.ncy - 1;
+    c.left = T.cx0 == 0;
+    c.right = T.cx0 + W >= R.ncx;
+    return c;
+}
+
 template <bool kWrap, bool kSigma>
-__device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int W, PairLds &L) {
+__device__ void list_tile_body_direct(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
+                                      PairLds &L) {
+    const ImgCols ic = img_cols(T, R, W);
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -1427,7 +1448,8 @@ __device__ void list_tile_body_direct(const ForceArgs &a, const RepDesc &R, int 
         // groups of four, padded with the sentinel slot (staged far away:
         // the pair pass's cutoff test rejects it); cap is a multiple of 8
         for (int kk = m; kk < ((m + 3) & ~3); ++kk) list_store_slot(a, kk, p, 8 * a.lds_cap);
-        a.nnum[p] = m | (k << 16);  // list slot p holds particle k of the tile
+        // list slot p holds particle k of the tile
+        a.nnum[p] = m | (kWrap && ic.needs(col, W) ? kListImg : 0) | (k << 16);
         if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
     }
 }
@@ -1438,14 +1460,16 @@ __device__ __forceinline__ int wave_max_int(int v) {
     return v;
 }
 
-// Rank of this lane's list among the active lanes of the workgroup, longest
-// first (by groups of four entries), ties by lane: a counting sort over
-// (length, wavefront) with the ties inside a wavefront ordered by ballots.
-// hist holds (capg + 1) * (blockDim / 64) ints.  Deterministic.
-__device__ int list_rank(int n4, bool act, int capg, int *hist) {
+// Rank of this lane's list among the active lanes of the workgroup: the
+// lists of particles that need the minimum image after all others, each
+// part longest first (by groups of four entries), ties by lane: a counting
+// sort over (image, length, wavefront) with the ties inside a wavefront
+// ordered by ballots.  hist holds 2 (capg + 1) (blockDim / 64) ints.
+// Deterministic.
+__device__ int list_rank(int n4, bool img, bool act, int capg, int *hist) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nw = blockDim.x >> 6, nk = capg + 1, tot = nw * nk;
-    const int key = capg - min(n4, capg);  // 0 = longest
+    const int nw = blockDim.x >> 6, nk = 2 * (capg + 1), tot = nw * nk;
+    const int key = (img ? capg + 1 : 0) + capg - min(n4, capg);  // 0 = longest
     __syncthreads();  // earlier readers of hist are done
     for (int i = tid; i < tot; i += blockDim.x) hist[i] = 0;
     __syncthreads();
@@ -1467,7 +1491,7 @@ __device__ int list_rank(int n4, bool act, int capg, int *hist) {
     __syncthreads();
     unsigned long long eq = __ballot(act);
 #pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {  // key < 256
+    for (int bit = 0; bit < 9; ++bit) {  // key < 2 (kMaxListGroups + 1) <= 512
         const bool kb = (key >> bit) & 1;
         const unsigned long long m = __ballot(kb);
         eq &= kb ? m : ~m;
@@ -1483,8 +1507,9 @@ __device__ int list_rank(int n4, bool act, int capg, int *hist) {
 // next buffer slot while only hits advance it, so the loop has no branches.
 // Reads past a lane's run stay inside the staging's headroom of 3 cells.
 template <bool kWrap, bool kSigma>
-__device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, PairLds &L,
-                                   uint16_t *buf, int *hist) {
+__device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
+                                   PairLds &L, uint16_t *buf, int *hist) {
+    const ImgCols ic = img_cols(T, R, W);
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
     const int nc = c_hi - c_lo;
@@ -1553,7 +1578,8 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
         // the chunk's lists are stored longest first (list slot q = sbase +
         // k0 + rank), so that the pass's wavefronts take lists of similar
         // length; nnum[q] = length | (tile index k << 16)
-        const int rank = list_rank((m + 3) >> 2, act, cap >> 2, hist);
+        const bool img = kWrap && ic.needs(col, W);
+        const int rank = list_rank((m + 3) >> 2, img, act, cap >> 2, hist);
         if (act) {
             const int p = sbase + k0 + rank;  // list slot
             // groups of four, padded with the sentinel slot (staged far
@@ -1568,7 +1594,7 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const RepDesc &R, int W, 
                 }
                 list_store_group(a, g, p, jj);
             }
-            a.nnum[p] = m | (k << 16);
+            a.nnum[p] = m | (img ? kListImg : 0) | (k << 16);
             if (cnt > cap) atomicOr(a.err, 16u);  // FM_DEV_LIST_OVERFLOW
         }
     }
@@ -1579,7 +1605,6 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     extern __shared__ float4 dyn[];
     __shared__ int s_cstart[kMaxSlots];
     __shared__ int s_off[kMaxSlots + 1];
-    __shared__ int s_hist[(kMaxListGroups + 1) * 4];
     PairLds L;
     L.ent = dyn;
     L.typ = nullptr;
@@ -1590,6 +1615,8 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     L.red = nullptr;
     // list buffer after the staging and its read headroom of 3 cells
     uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
+    // the ranking histogram after the list buffer (list_build_lds_bytes)
+    int *hist = reinterpret_cast<int *>(buf + (size_t)a.nbr_cap * blockDim.x);
     if (a.ctl && a.any && !*a.any) return;  // gated, and no replica rebuilds
     // gated batches walk the listed tiles of the rebuilding replicas
     const int nt = a.ftile ? *a.nft : a.ntiles;
@@ -1605,11 +1632,11 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
         }
         const bool wrapT = (T.w_flags >> 16) & 1;
         if (a.lbuf) {
-            if (wrapT) list_tile_body_buf<true, kSigma>(a, R, W, L, buf, s_hist);
-            else list_tile_body_buf<false, kSigma>(a, R, W, L, buf, s_hist);
+            if (wrapT) list_tile_body_buf<true, kSigma>(a, T, R, W, L, buf, hist);
+            else list_tile_body_buf<false, kSigma>(a, T, R, W, L, buf, hist);
         } else {
-            if (wrapT) list_tile_body_direct<true, kSigma>(a, R, W, L);
-            else list_tile_body_direct<false, kSigma>(a, R, W, L);
+            if (wrapT) list_tile_body_direct<true, kSigma>(a, T, R, W, L);
+            else list_tile_body_direct<false, kSigma>(a, T, R, W, L);
         }
         __syncthreads();  // the next tile restages the LDS
     }
@@ -1666,7 +1693,11 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
         const float xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
         const int ti = kTypes ? lt[me] : 0;
-        const int n4 = ((nq & 0xffff) + 3) >> 2;  // padded with the sentinel slot
+        const int n4 = ((nq & kListLen) + 3) >> 2;  // padded with the sentinel slot
+        // in a tile that needs the minimum image, only the wavefronts
+        // holding a list flagged for it take the image path (the build
+        // ranks those lists after the others); elsewhere the shift is 0
+        const bool wimg = kWrap && __ballot((nq & kListImg) != 0) != 0ull;
         // single type: T = sum_hit tv, Q as described above; with a type
         // table: T = sum_hit e4 (s12 - s6), Q = sum_hit e48 tv, ush = sum_hit
         // shift (femmd2D/src/interactions.cpp:347-389 per type pair)
@@ -1676,11 +1707,12 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
         // one list entry, without branches: the force of a non-interacting
         // entry (or of the padding entry) is computed and discarded by a
         // select
-        auto term = [&](float2 pj, float sj, int tj, int jo) {
+        auto term = [&](auto wrapc, float2 pj, float sj, int tj, int jo) {
+            constexpr bool kW = decltype(wrapc)::value;
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
-            if (kWrap) {
+            if (kW) {
                 // minimum image as two selects per axis
                 ox = (ex >= hLx) ? Lx : 0.0f;
                 ox = (ex < -hLx) ? -Lx : ox;
@@ -1711,7 +1743,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
             // the particle itself: the padding is the far sentinel slot)
             const bool hit = dr2 < rc2;
             float dxa = dx, dya = dy, r2a = dr2;
-            if (kWrap) {
+            if (kW) {
                 // image-exact difference: the far coordinate is shifted by L
                 // before subtracting (same bits as dx where no shift applies)
                 dxa = (ox > 0.0f) ? (xim - xj) : ((ox < 0.0f) ? (xi - (xj - Lx)) : dx);
@@ -1745,7 +1777,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
                 d2 += v * v;
             }
         };
-        auto group = [&](uint64_t cur) {
+        auto group = [&](auto wrapc, uint64_t cur) {
             const int j0 = (int)(cur & 0xffffu), j1 = (int)((cur >> 16) & 0xffffu);
             const int j2 = (int)((cur >> 32) & 0xffffu), j3 = (int)(cur >> 48);
             // entries are byte offsets of float2 staging slots
@@ -1768,22 +1800,26 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
                 t2 = lt[j2 >> 3];
                 t3 = lt[j3 >> 3];
             }
-            term(q0, s0, t0, j0);
-            term(q1, s1, t1, j1);
-            term(q2, s2, t2, j2);
-            term(q3, s3, t3, j3);
+            term(wrapc, q0, s0, t0, j0);
+            term(wrapc, q1, s1, t1, j1);
+            term(wrapc, q2, s2, t2, j2);
+            term(wrapc, q3, s3, t3, j3);
         };
         // the next group is loaded while this one is evaluated (the group
         // index is clamped inside the array; a group past the list is loaded
         // and not used); the empty asm keeps the compiler from moving the
         // load down to its use in the next iteration
-        uint64_t gv = nb[(uint32_t)q];
-        for (int g = 0; g < n4; ++g) {
-            const uint64_t cur = gv;
-            gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)q];
-            __asm__ volatile("" ::: "memory");
-            group(cur);
-        }
+        auto groups = [&](auto wrapc) {
+            uint64_t gv = nb[(uint32_t)q];
+            for (int g = 0; g < n4; ++g) {
+                const uint64_t cur = gv;
+                gv = nb[(uint32_t)min(g + 1, capg - 1) * Nu + (uint32_t)q];
+                __asm__ volatile("" ::: "memory");
+                group(wrapc, cur);
+            }
+        };
+        if (wimg) groups(std::integral_constant<bool, kWrap>{});
+        else groups(std::integral_constant<bool, false>{});
         const int id = a.perm[p];
         if (kTypes)
             tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx, fy, tk);
@@ -2310,7 +2346,7 @@ __device__ void adj_list_body(const AdjArgs &A, const RepDesc &R, int W, int til
         const float2 wi = lw[me];
         const float xi = pi.x, yi = pi.y, xim = xi - Lx, yim = yi - Ly;
         const float si = kSigma ? ls[me] : 0.0f;
-        const int n4 = ((nq & 0xffff) + 3) >> 2;
+        const int n4 = ((nq & kListLen) + 3) >> 2;
         float jx = 0.f, jy = 0.f, fx = 0.f, fy = 0.f, gx = 0.f, gy = 0.f;
         auto term = [&](int off) {
             const float2 pj = *reinterpret_cast<const float2 *>(lpb + off);
@@ -3110,8 +3146,11 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
 }
 
 size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered) {
+    // staging; list buffer and the ranking histogram (list_rank)
     return (size_t)(lds_cap + 3 * cell_cap) * sizeof(float4) +
-           (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) : 0);
+           (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) +
+                           (size_t)2 * (nbr_cap / 4 + 1) * (block / 64) * sizeof(int)
+                     : 0);
 }
 
 // Workgroups of the list build that fit on the device at once (the gated
