@@ -1642,6 +1642,40 @@ int Engine::time_forces(int reps, double *ms, std::string &err) {
     return sync(err);
 }
 
+int Engine::time_parts(int reps, double *ms_update, double *ms_pass, std::string &err) {
+    int rc = ensure_primed(err);
+    if (rc) return rc;
+    hipEvent_t ev[3];
+    for (auto &e : ev) (void)hipEventCreate(&e);
+    auto cleanup = [&]() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    };
+    int bad = 0;
+    (void)hipEventRecord(ev[0], stream_);
+    // (the cell walk's update runs every step: timed with its pass)
+    for (int k = 0; k < reps && !bad && list_; ++k)
+        bad = launch_bin() || launch_sort() || launch_list_build(false);
+    (void)hipEventRecord(ev[1], stream_);
+    for (int k = 0; k < reps && !bad; ++k) {
+        energy_ = true;
+        bad = list_ ? launch_pair_pass() : (launch_bin() || launch_forces_only(false));
+    }
+    (void)hipEventRecord(ev[2], stream_);
+    if (bad) {
+        cleanup();
+        return check(hipGetLastError(), "time parts", err);
+    }
+    rc = check(hipEventSynchronize(ev[2]), "time", err);
+    float t1 = 0.f, t2 = 0.f;
+    (void)hipEventElapsedTime(&t1, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&t2, ev[1], ev[2]);
+    cleanup();
+    *ms_update = t1 / std::max(reps, 1);
+    *ms_pass = t2 / std::max(reps, 1);
+    if (rc) return rc;
+    return sync(err);
+}
+
 // Pair energies and virials at the current positions (the forces are
 // recomputed with the same bits: same lists, or the same deterministic sort).
 int Engine::energy_pass(std::string &err) {

@@ -97,6 +97,11 @@ public:
                   int32_t *perm, std::string &err);
     int pair_sets(int r, int32_t *cnt, int64_t *s1, int64_t *s2, std::string &err);
     int time_forces(int reps, double *ms, std::string &err);
+    // The two parts of time_forces timed apart, per repetition: the
+    // neighbour-list update of every replica (binning, sort, list build;
+    // 0 for the cell walk) and the pair pass with its energies (the cell
+    // walk: binning, sort and pair pass).
+    int time_parts(int reps, double *ms_update, double *ms_pass, std::string &err);
     int profile(int nsteps, double *out, std::string &err);
     // neighbour-list statistics of replica r: mean and max list length,
     // rebuilds so far (0 / 0 / 0 when the engine walks cells every step)
