@@ -56,29 +56,21 @@ int Engine::adj_alloc(std::string &err) {
     return FM_OK;
 }
 
-int Engine::adj_pass_list(bool fresh, std::string &err) {
-    if (fresh) {
-        // lists for this configuration; later configurations of the sweep
-        // rebuild them when a particle has moved more than skin / 2 from
-        // the positions of this build
-        if (launch_bin() || launch_sort() || launch_list_build(false))
-            return check(hipGetLastError(), "adjoint lists", err);
-        if (launch_ctl_reset(ctl_.p, R_, dmax_.p, 2 * R_ * kDispSlots, stream_) != hipSuccess)
-            return check(hipGetLastError(), "ctl", err);
-    } else {
-        KickArgs k{};
-        k.reps = reps_.p;
-        k.spans = kspans_.p;
-        k.x = xcur();
-        k.y = ycur();
-        k.spos = spos_.p;
-        k.xys = xys_cur();  // the positions of the last build
-        k.dmax = dmax_cur();
-        if (launch_adj_disp((int)kspans_h_.size(), k, stream_) != hipSuccess)
-            return check(hipGetLastError(), "adjoint displacement", err);
-        if (launch_bin(true) || launch_sort(true) || launch_list_build(true))
-            return check(hipGetLastError(), "adjoint lists", err);
-    }
+// Neighbour lists built at the configuration held at xy (x, then y: a
+// history slot), the way the forward pass built them there: the same sort
+// of the same positions, so the same lists.
+int Engine::adj_lists_at(float *xy, std::string &err) {
+    xalt_ = xy;
+    yalt_ = xy + N_;
+    const int bad = launch_bin() || launch_sort() || launch_list_build(false);
+    xalt_ = yalt_ = nullptr;
+    if (bad) return check(hipGetLastError(), "adjoint lists", err);
+    return FM_OK;
+}
+
+// J w over the lists of the configuration in xalt_ / yalt_ (lists built at
+// the forward pass's last rebuild at or before it).
+int Engine::adj_pass_list(std::string &err) {
     AdjArgs A{};
     A.f = force_args();
     A.w = w_.p;
@@ -259,20 +251,19 @@ int Engine::adjoint_forward(int nsteps, int ckpt, std::string &err) {
         }
         if ((he = hist_.alloc((size_t)ckpt * 2 * N)) != hipSuccess)
             return check(he, "alloc history", err);
+        if ((he = rblog_buf_.alloc((size_t)ckpt)) != hipSuccess)
+            return check(he, "alloc rebuild log", err);
     }
-    rc = ensure_primed(err);
-    if (rc) return rc;
     // ---- forward, checkpoint at every segment start and at the end; the
     // last segment records its positions as it goes (no energies needed) ----
     obs_valid_ = false;
     for (int s = 0; s < S; ++s) {
-        if (list_ && S > 1) {
-            // a fresh neighbour list at every checkpoint, as the recomputation
-            // from that checkpoint will have (same summation order)
-            primed_ = false;
-            rc = ensure_primed(err);
-            if (rc) return rc;
-        }
+        // list mode: fresh neighbour lists at every segment start, as the
+        // recomputation from its checkpoint will have (same summation order)
+        // and as the backward sweep builds them there
+        if (list_) primed_ = false;
+        rc = ensure_primed(err);
+        if (rc) return rc;
         if (save_state(ckpt_.p + (size_t)s * 4 * N)) return check(hipGetLastError(), "ckpt", err);
         const int len = std::min(ckpt, nsteps - s * ckpt);
         if (s == S - 1) {
@@ -301,9 +292,11 @@ int Engine::adjoint_forward(int nsteps, int ckpt, std::string &err) {
 // The positions x_0 .. x_{len-1} of a segment into history slots 0 .. len-1
 // (slot 0 before the first drift; the drift of body k writes x_{k+1} into
 // slot k + 1) while the segment is stepped from the current state, and the
-// closing half-kick.  Every launch of the segment is one graph, captured the
-// first time a segment of this length is recorded (the slots are the same
-// buffers for every segment).
+// closing half-kick; in list mode the rebuild decision taken for x_k (body
+// k's gated binning) into rblog_buf_[k] (x_0's lists are fresh).  Every
+// launch of the segment is one graph, captured the first time a segment of
+// this length is recorded (the slots are the same buffers for every
+// segment).
 int Engine::adj_record_segment(int len, bool close) {
     auto body = [&]() -> int {
         if (launch_copy2(hist_.p, x_.p, y_.p, N_, stream_) != hipSuccess) return -1;  // slot 0
@@ -312,9 +305,11 @@ int Engine::adj_record_segment(int len, bool close) {
         for (int k = 1; k < len && !bad; ++k) {
             // the drift of body k records x_{k+1} into slot k + 1
             rec_ = (k + 1 < len) ? hist_.p + (size_t)(k + 1) * 2 * N_ : nullptr;
+            rblog_ = list_ ? rblog_buf_.p + k : nullptr;
             bad = launch_step_body(true, true);
         }
         rec_ = nullptr;
+        rblog_ = nullptr;
         if (!bad && close) bad = launch_step_body(true, false);
         return bad;
     };
@@ -361,6 +356,7 @@ int Engine::adjoint_sweep(std::string &err) {
     int rc;
     const float dt = reps_h_[0].dt, h = reps_h_[0].hdt;
     bool first = true;
+    std::vector<int32_t> rb;
     for (int s = S - 1; s >= 0; --s) {
         const int len = std::min(ckpt, nsteps - s * ckpt);
         if (s < S - 1) {
@@ -370,17 +366,41 @@ int Engine::adjoint_sweep(std::string &err) {
             if (rc) return rc;
             if (adj_record_segment(len, false)) return check(hipGetLastError(), "step", err);
         }
+        if (list_) {
+            // where the forward pass built its lists in this segment: at x_0
+            // and wherever the rebuild rule fired
+            rb.assign(len, 0);
+            if (len > 1) {
+                rc = check(hipMemcpyAsync(rb.data() + 1, rblog_buf_.p + 1, (len - 1) * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, stream_), "rebuild log", err);
+                if (rc) return rc;
+                rc = sync(err);
+                if (rc) return rc;
+            }
+            rb[0] = 1;
+        }
+        int built = -1;  // the configuration whose lists are in place
         for (int k = len - 1; k >= 0; --k) {
             const int n = s * ckpt + k;
             if (first) {
                 if (adj_kick(false, true, true, n > 0 ? dt : h)) return check(hipGetLastError(), "adj kick", err);
                 first = false;
             }
+            if (list_) {
+                // the lists the forward pass used for x_k: those it built at
+                // its last rebuild at or before k (every list is valid for the
+                // configurations up to the next rebuild, by the rebuild rule)
+                int b = k;
+                while (!rb[b]) --b;
+                if (b != built) {
+                    rc = adj_lists_at(hist_.p + (size_t)b * 2 * N, err);
+                    if (rc) return rc;
+                    built = b;
+                }
+            }
             xalt_ = hist_.p + (size_t)k * 2 * N;
             yalt_ = xalt_ + N;
-            // list mode: the segment's first configuration (the layout was
-            // last laid out by the forward pass) builds fresh lists
-            rc = list_ ? adj_pass_list(k == len - 1, err) : adj_pass(err);
+            rc = list_ ? adj_pass_list(err) : adj_pass(err);
             xalt_ = yalt_ = nullptr;
             if (rc) return rc;
             const bool more = n > 0;

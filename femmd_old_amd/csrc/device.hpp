@@ -151,6 +151,8 @@ struct KickArgs {
     const int32_t *fspan;      // gated batches: the rebuilding replicas' spans
     const int32_t *nfs;        // (decided by k_decide), and their count
     int32_t one_rep;           // gated binning of a single replica (decided first)
+    int32_t *rblog;            // if set (single replica), the gated binning
+                               // writes its rebuild decision there
     ObsArgs obs;               // per-step observables
     int32_t obs_l1;            // the binning forms the previous step's level-1 chunks
 };

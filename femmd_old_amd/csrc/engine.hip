@@ -706,6 +706,7 @@ int Engine::launch_bin(bool gated) {
         a.any = any_.p;
         a.dmax = dmax_cur();  // the drift that produced the current positions
         a.one_rep = R_ == 1;
+        a.rblog = R_ == 1 ? rblog_ : nullptr;
         if (R_ > kDecideMinReps) {
             DecideArgs d{};
             d.reps = reps_.p;

@@ -229,7 +229,8 @@ private:
     int adj_pass(std::string &err);  // bin + sort + adjoint force kernel at xcur
     // list mode backward sweep at xcur: lists rebuilt (fresh, or gated on the
     // displacement from the last build) + adjoint kernel over the lists
-    int adj_pass_list(bool fresh, std::string &err);
+    int adj_lists_at(float *xy, std::string &err);
+    int adj_pass_list(std::string &err);
     int adj_kick(bool accumulate, bool drift, bool make_w, float c);
     // a segment of len steps from the current state with its positions
     // recorded into the history slots (and the closing half-kick)
@@ -351,12 +352,14 @@ private:
     // adjoint state
     float *xalt_ = nullptr, *yalt_ = nullptr;
     float *rec_ = nullptr;  // drift of the next step body records x, y here
+    int32_t *rblog_ = nullptr;  // its gated binning records the rebuild decision here
     std::vector<int32_t> ecaller_h_;  // internal element -> caller element index
     DBuf<float> eE_;                  // Young's modulus per element
     DBuf<float2> w_, cfa_;
     DBuf<float> jx_, jy_, xbx_, xby_, vbx_, vby_;
     DBuf<double> dE_, pacc_, loss_part_;
     DBuf<float> ckpt_, hist_;
+    DBuf<int32_t> rblog_buf_;  // per recorded configuration: lists rebuilt there
 
     // elastomer clusters
     std::vector<int32_t> cl_rep_h_, cl_off_h_;  // replica, node range per cluster
@@ -386,7 +389,6 @@ hipError_t launch_fill_int(int32_t *p, int32_t v, size_t n, hipStream_t s);
 // dst[0, n) = x, dst[n, 2n) = y (graph-safe copy kernel)
 hipError_t launch_copy2(float *dst, const float *x, const float *y, int n, hipStream_t s);
 hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A, hipStream_t s);
-hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s);
 hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s);
 hipError_t launch_adj_loss(const LossArgs &a, hipStream_t s);
 hipError_t launch_cluster_props(const ClusterArgs &a, hipStream_t s);

@@ -593,6 +593,7 @@ __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
               a.ctl->nbuild += 1u;
               *a.any = 1;
           }
+          if (a.rblog) *a.rblog = flag;  // (adjoint recording)
       }
       if (!flag) return;
       last_rep = 0;
@@ -2602,14 +2603,6 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
     __shared__ double s_red[2 * 4];
     const ForceArgs &a = A.f;
     int b = blockIdx.x;
-    if (kList) {
-        // the rebuild decision of this configuration is taken: clear its
-        // summary and the displacement slots (workgroup r: replica r)
-        if (b == 0 && threadIdx.x == 0) *a.any = 0;
-        if (threadIdx.x < kDispSlots)
-            for (int r = blockIdx.x; r < a.nrep; r += gridDim.x)
-                a.dmax[((size_t)r * kDispSlots + threadIdx.x) * kDispStride] = 0u;
-    }
     if (b < a.ntiles) {
         PairLds L;
         L.ent = dyn;
@@ -2637,34 +2630,6 @@ __global__ __launch_bounds__(256) void k_adj_forces(AdjArgs A) {
     }
     b -= a.nespans;
     if (b < a.nbspans) adj_bond_block(A, b);
-}
-
-// Backward sweep in list mode: the displacement of each particle from the
-// positions of the last list build (held in sorted order), as the largest
-// squared displacement per workgroup into the replica's slots (minimum image:
-// a particle may have crossed the box since).
-__global__ __launch_bounds__(kKickThreads) void k_adj_disp(KickArgs a) {
-    __shared__ float redd[kKickThreads / 64];
-    const Span sp = a.spans[blockIdx.x];
-    const RepDesc R = a.reps[sp.rep];
-    const int p = sp.begin + (int)threadIdx.x;
-    float d2 = 0.0f;
-    if (p < sp.end) {
-        const float2 r = a.xys[a.spos[p]];
-        float dx = a.x[p] - r.x, dy = a.y[p] - r.y;
-        dx = (dx >= R.hLx) ? dx - R.Lx : ((dx < -R.hLx) ? dx + R.Lx : dx);
-        dy = (dy >= R.hLy) ? dy - R.Ly : ((dy < -R.hLy) ? dy + R.Ly : dy);
-        d2 = dx * dx + dy * dy;
-    }
-    const float m = wave_max(d2);
-    if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float mm = redd[0];
-        for (int k = 1; k < kKickThreads / 64; ++k) mm = fmaxf(mm, redd[k]);
-        atomicMax(a.dmax + ((size_t)sp.rep * kDispSlots + blockIdx.x % kDispSlots) * kDispStride,
-                  __float_as_uint(mm));
-    }
 }
 
 __global__ __launch_bounds__(kKickThreads) void k_adj_kick(AdjKickArgs a) {
@@ -2992,11 +2957,6 @@ hipError_t launch_adj_forces(bool sigma, bool list, int block, const AdjArgs &A,
     return hipGetLastError();
 }
 
-hipError_t launch_adj_disp(int nblk, const KickArgs &a, hipStream_t s) {
-    if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_disp, dim3(nblk), dim3(kKickThreads), 0, s, a);
-    return hipGetLastError();
-}
 
 hipError_t launch_adj_kick(int nblk, const AdjKickArgs &a, hipStream_t s) {
     if (nblk <= 0) return hipSuccess;
