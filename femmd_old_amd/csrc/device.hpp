@@ -153,6 +153,8 @@ struct KickArgs {
     int32_t one_rep;           // gated binning of a single replica (decided first)
     int32_t *rblog;            // if set (single replica), the gated binning
                                // writes its rebuild decision there
+    int32_t gather_lds;        // gather corner forces through LDS in
+                               // coalesced chunks (uniform)
     ObsArgs obs;               // per-step observables
     int32_t obs_l1;            // the binning forms the previous step's level-1 chunks
 };

@@ -18,6 +18,9 @@ namespace fm {
 namespace {
 
 constexpr int kGraphSteps = 16;  // step bodies per captured graph of a run
+// node kicks of at least this many workgroups gather through LDS (measured:
+// 1950 workgroups, 500k nodes: faster; 364 workgroups: slower)
+constexpr size_t kGatherLdsSpans = 1024;
 
 float wrap_host(float r, float L) {  // femmd2D/src/def.h:112, FP32
     float q = r / L;
@@ -819,6 +822,8 @@ int Engine::launch_step_body(bool kick2, bool drift, bool energy, bool l1, bool 
         KickArgs k = kick_args(drift);
         k.spans = nspans_.p;
         k.ke_part = ke_node_.p;
+        // (a throughput-bound node kick reads its corner forces coalesced)
+        k.gather_lds = nspans_h_.size() >= kGatherLdsSpans ? 1 : 0;
         time_next(0);
         const hipError_t e =
             launch_kick(drift ? 3 : 2, gather_, energy_ || !drift, (int)nspans_h_.size(), k, stream_);

@@ -322,6 +322,34 @@ __device__ __forceinline__ void kick_particle(const RepDesc &R, float ax, float 
 // a = (f_pair + sum of the particle's corner forces) * inv_m; the corner
 // forces of particle p are the consecutive entries [goff[p], goff[p + 1]).
 // ---------------------------------------------------------------------------
+// The corner forces of a span's particles are one contiguous run of the
+// gather order, [goff[begin], goff[end]): the workgroup reads it in
+// coalesced chunks through LDS and each lane adds its own particle's
+// entries in order (the same sums in the same order as reading its run
+// directly, which leaves each lane a strided walk of its own).  It pays
+// where the node kick is throughput-bound (many workgroups); with few, the
+// two barriers per chunk lengthen each workgroup's path and the direct
+// walk is faster.
+constexpr int kGatherChunk = 2048;
+__device__ __forceinline__ void gather_corners(const KickArgs &a, const Span &sp, int p,
+                                               bool active, float &fx, float &fy) {
+    __shared__ float2 s_cf[kGatherChunk];
+    const int E0 = a.goff[sp.begin], E1 = a.goff[sp.end];
+    const int e0 = active ? a.goff[p] : 0, e1 = active ? a.goff[p + 1] : 0;
+    for (int c = E0; c < E1; c += kGatherChunk) {  // uniform per workgroup
+        const int n = min(kGatherChunk, E1 - c);
+        __syncthreads();  // the previous chunk's readers are done
+        for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) s_cf[i] = a.cf[c + i];
+        __syncthreads();
+        const int lo = max(e0, c), hi = min(e1, c + n);
+        for (int e = lo; e < hi; ++e) {
+            const float2 v = s_cf[e - c];
+            fx += v.x;
+            fy += v.y;
+        }
+    }
+}
+
 template <bool kKick2, bool kDrift, bool kGather, bool kList, bool kEnergy>
 __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     __shared__ double red[2 * (kKickThreads / 64)];
@@ -333,16 +361,23 @@ __global__ __launch_bounds__(kKickThreads) void k_kick(KickArgs a) {
     double ke = 0.0;
     float vvmax = 0.0f, d2 = 0.0f;
     int cell = 0;
+    float fx = 0.0f, fy = 0.0f;
     if (active) {
-        float fx = a.fx[p], fy = a.fy[p];
-        if (kGather) {
-            const int e0 = a.goff[p], e1 = a.goff[p + 1];
-            for (int e = e0; e < e1; ++e) {
+        fx = a.fx[p];
+        fy = a.fy[p];
+    }
+    if (kGather) {
+        if (a.gather_lds) {  // uniform: set for node kicks of many workgroups
+            gather_corners(a, sp, p, active, fx, fy);
+        } else if (active) {
+            for (int e = a.goff[p]; e < a.goff[p + 1]; ++e) {
                 const float2 c = a.cf[e];
                 fx += c.x;
                 fy += c.y;
             }
         }
+    }
+    if (active) {
         float im;  // (the same value either way: no load when uniform)
         if (R.im_uniform > 0.0f) im = R.im_uniform;
         else im = a.inv_m[p];
