@@ -74,6 +74,7 @@ struct alignas(16) RepDesc {
 // slice of the per-workgroup partials (ke, lj, fea, bond), written by one
 // workgroup of 256 threads; then the chunks in chunk order.
 constexpr int kObsRaw = 8, kObsAvg = 8, kObsStride = 32, kObsChunks = 16;
+constexpr int kObsRed = 4 * kObsRaw;  // LDS doubles of a level-1 reduction
 struct ObsArgs {
     const RepDesc *reps;
     const int32_t *rng;        // 8 per replica: kick, tile, element, bond ranges

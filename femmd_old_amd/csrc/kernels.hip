@@ -94,6 +94,11 @@ __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
 __device__ __forceinline__ int wave_sum_int(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -180,26 +185,24 @@ __device__ void obs_chunk(const ObsArgs &o, int r, int c, double *red) {
         v[6] += o.bond_part[2 * k];
         v[7] += o.bond_part[2 * k + 1];
     }
-    // max v^2: order-independent maximum over the workgroup
-    double m = v[1];
+    // the chunk's seven sums and its max v^2: each wavefront's in a fixed
+    // butterfly, then the four wavefronts in order (one exchange through
+    // LDS; red holds kObsRed doubles)
+    const double m = v[1];
 #pragma unroll
-    for (int s = 32; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s));
-    double *out = o.chunk + ((size_t)r * kObsChunks + c) * kObsRaw;
-    double z = 0.0;
-    block_sum2(v[0], z, red);
-    if (tid == 0) out[0] = v[0];
-    block_sum2(v[2], v[3], red);
-    if (tid == 0) { out[2] = v[2]; out[3] = v[3]; }
-    block_sum2(v[4], v[5], red);
-    if (tid == 0) { out[4] = v[4]; out[5] = v[5]; }
-    block_sum2(v[6], v[7], red);
-    if (tid == 0) { out[6] = v[6]; out[7] = v[7]; }
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = m;
+    for (int k = 0; k < kObsRaw; ++k) v[k] = (k == 1) ? wave_max_d(m) : wave_sum(v[k]);
+    __syncthreads();  // earlier readers of red are done
+    if ((tid & 63) == 0)
+        for (int k = 0; k < kObsRaw; ++k) red[(tid >> 6) * kObsRaw + k] = v[k];
     __syncthreads();
     if (tid == 0) {
-        for (int k = 1; k < 4; ++k) m = fmax(m, red[k]);
-        out[1] = m;
+        double *out = o.chunk + ((size_t)r * kObsChunks + c) * kObsRaw;
+        for (int k = 0; k < kObsRaw; ++k) {
+            double t = red[k];
+            for (int w = 1; w < 4; ++w)
+                t = (k == 1) ? fmax(t, red[w * kObsRaw + k]) : t + red[w * kObsRaw + k];
+            out[k] = t;
+        }
     }
     __syncthreads();  // red is free again
 }
@@ -612,7 +615,7 @@ __global__ __launch_bounds__(1024) void k_decide(DecideArgs d) {
 }
 
 __global__ __launch_bounds__(kKickThreads) void k_bin(KickArgs a) {
-  __shared__ double ored[8];
+  __shared__ double ored[kObsRed];
   // the previous step's observables, level 1 (before any early exit)
   if (a.obs_l1) obs_level1(a.obs, ored);
   int last_rep = -1, flag = 1;
@@ -718,7 +721,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ int s_carry, s_head;
     __shared__ int s_ids[kSortIds];       // the block's slot entries, cell order
     __shared__ uint8_t s_cl[kSortIds];    // their cell (local index)
-    __shared__ double s_ored[8];
+    __shared__ double s_ored[kObsRed];
     // (cell walk) the previous step's observables, level 1, before the
     // ticket: no workgroup waits on one that has not taken its ticket
     if (a.obs_l1) obs_level1(a.obs, s_ored);
@@ -2181,7 +2184,7 @@ __global__ __launch_bounds__(256) void k_total_force(int n, const float *fx,
 // and order as the per-step path: level 1 (one workgroup per chunk), then
 // level 2 (one thread per replica); accumulate: add them to the running sums.
 __global__ __launch_bounds__(256) void k_observe_l1(ObsArgs o) {
-    __shared__ double red[8];
+    __shared__ double red[kObsRed];
     obs_level1(o, red);
 }
 __global__ __launch_bounds__(64) void k_observe_l2(ObsArgs o, int accumulate) {
