@@ -1,7 +1,8 @@
 // oracle.cpp -- CPU restatement of femmd2D's per-step force-and-integrate
-// path.  TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the
-// product library never loads it.  Every function cites the reference lines
-// it restates (paths relative to the reference root).
+// path.  TEST INFRASTRUCTURE ONLY (what the engine is tested against, and
+// the CPU baseline); the product library never loads it.  Every function
+// cites the reference lines it restates (paths relative to the reference
+// root).
 //
 // Build: see oracle/Makefile (g++ -O2 -ffp-contract=off, as the reference's
 // own g++ -O2 build of femmd2D/src/Makefile:9; the -fopenmp variant enables

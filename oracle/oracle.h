@@ -1,8 +1,8 @@
 /*
  * oracle.h -- CPU restatement of femmd2D's per-step force-and-integrate path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine
- * and the CPU baseline timed beside it.  Only tests/, __graft_entry__.smoke()
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference the HIP engine's
+ * results are tested against, and the CPU baseline timed beside it.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load it.  Nothing in femmd_old_amd/
  * links, loads or falls back to it.
  *
