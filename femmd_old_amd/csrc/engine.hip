@@ -593,7 +593,12 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
         W = std::max(1, std::min(W, std::min(kMaxTileW, (int)D.ncx)));
         tile_w_[r] = W;
         const int t_begin = (int)tiles_h_.size();
-        for (int cy = 0; cy < (lj_on_[r] ? D.ncy : 0); ++cy)
+        // rows 0 and ncy - 1 first (their halo rows cross the periodic
+        // boundary, so all their lists take the minimum image): the force
+        // pass hands each XCD a contiguous band of tiles in order, and the
+        // costlier tiles then start early instead of closing the last band
+        for (int iy = 0; iy < (lj_on_[r] ? D.ncy : 0); ++iy) {
+            const int cy = iy == 0 ? 0 : (iy == 1 ? D.ncy - 1 : iy - 1);
             for (int cx0 = 0; cx0 < D.ncx; cx0 += W) {
                 const int w = std::min(W, D.ncx - cx0);
                 // interior tiles (no staged cell wraps, >= 5 cells per axis)
@@ -602,6 +607,7 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
                                       cx0 >= 1 && cx0 + w <= D.ncx - 1;
                 tiles_h_.push_back(Tile{r, cy, cx0, w | (interior ? 0 : (1 << 16))});
             }
+        }
         // staged particles of a tile at close packing (1.16 / sigma_min^2)
         const double acell = (double)(D.Lx / D.ncx) * (D.Ly / D.ncy);
         worst = std::max(worst, 3.0 * (W + 2) * acell * rho_cp_);
