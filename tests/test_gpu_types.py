@@ -10,7 +10,9 @@ row-major ntypes x ntypes tables, femmd2D/src/interactions.cpp:65-79).
   * one type with mixed sigmas: the neighbour-list path with per-particle
     sigma (the list radius from the largest sigma pair) and the cell walk;
   * a six-type system stepped 100 steps on the lists against the FP64
-    oracle's rollout.
+    oracle's rollout;
+  * replicas with different two- and six-type tables stepped as one batch,
+    bitwise equal to each stepped alone.
 Interacting sets are bit-exact with the oracle's FP32 predicate; forces
 meet the 1e-5 bar of oracle.force_error; energies and virials 1e-5.
 
@@ -187,3 +189,49 @@ def test_masses_trajectory(masses, skin):
     assert dpos.max() < 2e-3
     assert np.sqrt((dpos ** 2).mean()) < 1e-4
     assert s.device_errors() == 0
+
+
+def _typed_system(seed, nt, n, L, skin, commit):
+    """A liquid with an nt-type table (1, 2 or 6 types) and mixed sigma."""
+    from femmd_old_amd import System as Sys
+    x, y, vx, vy, rng = _liquid(n, L, seed)
+    if nt == 1:
+        eps, rc, us = [1.0], [2.5], [0.0]
+    elif nt == 2:
+        eps, rc, us = [1.0, 0.6, 0.6, 1.4], [2.5, 1.8, 1.8, 2.2], [0.0, 0.05, 0.05, -0.02]
+    else:
+        nt, eps, rc, us = _six_type_tables(rng)
+    typ = rng.integers(0, nt, n).astype(np.int32)
+    sig = rng.uniform(0.9, 1.1, n).astype(np.float32)
+    s = Sys(L, L, dt=0.004)
+    s.set_skin(skin)
+    s.set_pair_potential(eps, rc, us)
+    s.add_particles(x, y, vx, vy, 1.0, sig, typ)
+    if commit:
+        s.allocate_cuda()
+    return s
+
+
+@pytest.mark.parametrize("skin", [0.6, 0.0])
+def test_batch_of_typed_tables_equals_looped(skin):
+    """Replicas with different pair tables (two and six types, mixed sigma)
+    stepped as one batch give the bits of each stepped alone: every replica
+    carries its own table and list radius (the largest cutoff of its own
+    table)."""
+    from femmd_old_amd import Batch
+    specs = [(31, 6, 2000, 50.0), (32, 2, 2400, 60.0), (33, 6, 3200, 70.0), (34, 2, 900, 36.0)]
+    singles = []
+    for seed, nt, n, L in specs:
+        s = _typed_system(seed, nt, n, L, skin, True)
+        s.run(60)
+        assert s.device_errors() == 0
+        singles.append((s.get_state(), s.energies()))
+    b = Batch([_typed_system(seed, nt, n, L, skin, False) for seed, nt, n, L in specs])
+    b.run(60)
+    assert b.device_errors() == 0
+    for r, (st, en) in enumerate(singles):
+        for u, v in zip(st, b.get_state(r)):
+            np.testing.assert_array_equal(u, v)
+        be = b.energies(r)
+        for k in ("ke", "u_lj", "w_lj"):
+            assert be[k] == pytest.approx(en[k], rel=1e-12, abs=1e-12)
