@@ -50,7 +50,12 @@ constexpr int kGatedGrid = 2048;
 // ---------------------------------------------------------------------------
 
 // femmd2D/src/def.h:112 pbcCalc: r - floor(r / L) * L, FP32, IEEE division.
+// For 0 < r < L the expression is r itself: r / L rounds to at most
+// 1 - 2^-24 (the largest float below L over L is at most that), so its
+// floor is 0; the division is then skipped (0 and -0 take the full path,
+// which maps -0 to +0).
 __device__ __forceinline__ float wrap1(float r, float L) {
+    if (r > 0.0f && r < L) return r;
     float q = r / L;
     float f = floorf(q);
     float p = f * L;

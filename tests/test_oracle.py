@@ -543,3 +543,24 @@ def test_force_error_terms_report():
     assert t["max_two_term"] == pytest.approx(1e-4)
     assert t["above_tol_two_term"] == 1 and t["n"] == 3
     assert O.force_error(fx, ry, rx, ry, fs).max() == pytest.approx(5e-6)
+
+
+@pytest.mark.parametrize("L", [1.0, 16.0, 74.0, 117.86813, 1179.468, 1024.0, 3.0e5])
+def test_wrap_is_identity_inside_the_box(L):
+    # pbcCalc (femmd2D/src/def.h:112) in FP32: for 0 < r < L, r / L rounds
+    # to at most 1 - 2^-24, so floor is 0 and the result is r itself; the
+    # engine's wrap skips the division there.  Checked on the floats just
+    # below L, just above 0 and random ones in between.
+    Lf = np.float32(L)
+    below = [Lf]
+    for _ in range(64):
+        below.append(np.nextafter(below[-1], np.float32(0)))
+    rng = np.random.default_rng(7)
+    r = np.concatenate([np.array(below[1:], np.float32),
+                        np.float32(1e-30) * np.arange(1, 65, dtype=np.float32),
+                        rng.uniform(0, L, 100000).astype(np.float32)])
+    r = r[(r > 0) & (r < Lf)]
+    q = (r / Lf).astype(np.float32)
+    assert q.dtype == np.float32 and np.all(np.floor(q) == 0)
+    out = r - np.floor(q) * Lf
+    np.testing.assert_array_equal(out.view(np.uint32), r.view(np.uint32))
