@@ -211,6 +211,8 @@ struct ScanArgs {
     const int32_t *any;
     int32_t *start;            // per cell: first sorted position (replica based)
     int32_t *perm;             // sorted position -> particle
+    const int32_t *uid;        // particle -> its index in the caller's order
+                               // (the in-cell rank key)
     const float *x, *y;
     const int32_t *grp;        // exclusion group per particle
     const float *sig;          // per-particle sigma (may be null)

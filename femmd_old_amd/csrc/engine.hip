@@ -856,6 +856,7 @@ int Engine::launch_sort(bool gated) {
     s.ticket = sort_ticket_.p;
     s.start = start_.p;
     s.perm = perm_.p;
+    s.uid = uid_.p;
     s.x = xcur();
     s.y = ycur();
     s.grp = grp_.p;

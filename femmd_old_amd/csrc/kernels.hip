@@ -724,7 +724,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __shared__ int s_w[kSortCells / 64];
     __shared__ unsigned long long s_tk;
     __shared__ int s_carry, s_head;
-    __shared__ int s_ids[kSortIds];       // the block's slot entries, cell order
+    __shared__ int s_ids[kSortIds];       // the block's slot entries' caller indices, cell order
     __shared__ uint8_t s_cl[kSortIds];    // their cell (local index)
     __shared__ double s_ored[kObsRed];
     // (cell walk) the previous step's observables, level 1, before the
@@ -842,10 +842,11 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
     __syncthreads();
     // scatter: the block's particles in cell order; a particle's sorted
     // position is its cell's start + its rank among the cell's members by
-    // particle index (deterministic in-cell order).  The block's slot
-    // entries are first gathered into LDS (loads batched per lane), so the
-    // ranks read LDS; a block with more particles than the LDS copy holds
-    // reads the slot lists from memory.
+    // their index in the caller's order, as the reference's cell fill walks
+    // the particles (femmd2D/src/nn.cpp:8-20), whatever the internal order.
+    // The block's slot entries' caller indices are first gathered into LDS
+    // (loads batched per lane), so the ranks read LDS; a block with more
+    // particles than the LDS copy holds reads the slot lists from memory.
     const bool in_lds = tot <= kSortIds;
     auto cell_of = [&](int q) {
         int lo = 0, hi = kSortCells;
@@ -870,7 +871,7 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
             for (int u = 0; u < kSortBatch; ++u) {
                 const int q = q0 + u * kSortCells;
                 if (q < tot) {
-                    s_ids[q] = v[u];
+                    s_ids[q] = a.uid[v[u]];
                     s_cl[q] = (uint8_t)cl[u];
                 }
             }
@@ -886,14 +887,17 @@ __global__ __launch_bounds__(kSortCells) void k_sort(ScanArgs a) {
             if (q < tot) {
                 const int cl = in_lds ? (int)s_cl[q] : cell_of(q);
                 const int b = s_ex[cl], m = s_ex[cl + 1] - b;
-                int i, rank = 0;
+                // (caller indices are unique within a replica, whose cells
+                // these are: a strict order)
+                const int *src = a.tmp + (size_t)(c0 + cl) * a.cap;
+                const int i = src[q - b];  // the particle (internal index)
+                int rank = 0;
                 if (in_lds) {
-                    i = s_ids[q];
-                    for (int k = 0; k < m; ++k) rank += (s_ids[b + k] < i);
+                    const int ui = s_ids[q];
+                    for (int k = 0; k < m; ++k) rank += (s_ids[b + k] < ui);
                 } else {
-                    const int *src = a.tmp + (size_t)(c0 + cl) * a.cap;
-                    i = src[q - b];
-                    for (int k = 0; k < m; ++k) rank += (src[k] < i);
+                    const int ui = a.uid[i];
+                    for (int k = 0; k < m; ++k) rank += (a.uid[src[k]] < ui);
                 }
                 ii[u] = i;
                 pos[u] = s_st[cl] + rank;
