@@ -1420,7 +1420,12 @@ __device__ __forceinline__ void list_store_slot(const ForceArgs &a, int k, int p
 // image: those next to a halo column across the periodic boundary in x, and
 // all of them when a halo row crosses it in y or the box has fewer than 5
 // cells along an axis (elsewhere a neighbour is less than 2 cells <= 0.4 L
-// away in the same periodic image, so the image shift is zero).
+// away in the same periodic image, so the image shift is zero).  Every entry
+// of such a list takes it, not only those in the halo across the boundary:
+// the centre and its neighbours in the boundary cells can wrap between two
+// builds.  A particle that wrapped is then more than a cell width minus
+// skin / 2 > rc from every centre two or more cells from the boundary, so
+// the plain difference rejects it there as the minimum image would.
 struct ImgCols {
     bool all, left, right;
     __device__ bool needs(int col, int W) const {
