@@ -150,3 +150,56 @@ def test_list_overflow_is_reported_not_faulting():
     s.clear_errors()
     assert s.device_errors() == 0
     s.close()
+
+
+@pytest.mark.parametrize("tile_cells", [0, 4])
+@pytest.mark.parametrize("skin", [0.6, 0.4])
+def test_particles_wrapping_between_list_builds(skin, tile_cells):
+    """Particles driven across the periodic edges (a column 0.05 from the
+    right edge moving right, a row 0.05 above the bottom edge moving down)
+    while the lists built before the crossing are still in use: at every
+    step the interacting sets equal the oracle's FP32 restatement.  A
+    particle of a boundary cell can wrap between two builds; the lists of
+    the boundary columns and rows take the minimum image in every entry,
+    and a particle that wrapped is farther than rc from every other centre,
+    where the plain difference rejects it as the minimum image would.
+    Tiles span the whole row (default) or 4 cells (interior tiles too)."""
+    L = 40.0
+    a = L / 32
+    rng = np.random.default_rng(5)
+    gx = 1.2 + a * np.arange(32)  # last column at L - 0.05
+    gy = 0.05 + a * np.arange(32)  # first row at 0.05
+    X, Y = np.meshgrid(gx, gy)
+    x = (X.ravel() + rng.uniform(-0.02, 0.02, X.size)).astype(np.float32)
+    y = (Y.ravel() + rng.uniform(-0.02, 0.02, Y.size)).astype(np.float32)
+    O.wrap_f32(x, y, np.float32(L), np.float32(L))
+    vx = rng.normal(0.0, 0.5, x.size).astype(np.float32)
+    vy = rng.normal(0.0, 0.5, x.size).astype(np.float32)
+    vx[X.ravel() > L - 1.0] = 2.0
+    vy[Y.ravel() < 1.0] = -2.0
+    s = _system(L, skin)
+    s.add_particles(x, y, vx, vy)
+    s.allocate_cuda()
+    if tile_cells:
+        s.set_tuning(tile_cells=tile_cells)
+    s.calc_forces()  # lists built (after a retile too) before counting
+    osys = O.OracleSystem(O.box_f32(L), O.box_f32(L), x, y)
+    r0 = s.list_stats()["rebuilds"]
+    wraps_stale = 0
+    for step in range(24):
+        cnt, s1, s2 = s.pair_sets()
+        xc, yc, _, _ = s.get_state()
+        ocnt, os1, os2 = O.pair_sets(osys.with_positions(xc, yc), 1)
+        np.testing.assert_array_equal(cnt, ocnt)
+        np.testing.assert_array_equal(s1, os1)
+        np.testing.assert_array_equal(s2, os2)
+        s.run(1)
+        xn, yn, _, _ = s.get_state()
+        wrapped = int(np.sum((np.abs(xn - xc) > L / 2) | (np.abs(yn - yc) > L / 2)))
+        if s.list_stats()["rebuilds"] == r0:
+            wraps_stale += wrapped
+    print(f"[wrap skin {skin} tiles {tile_cells}] {wraps_stale} crossings under the first "
+          f"lists, rebuilds {s.list_stats()['rebuilds'] - r0}")
+    assert wraps_stale >= 20
+    assert s.device_errors() == 0
+    s.close()
