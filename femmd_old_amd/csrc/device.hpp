@@ -46,9 +46,6 @@ struct alignas(16) RepDesc {
     float rl2_uniform; // list radius^2, (rc*s + skin)^2 with a rounding margin
     int32_t ks0, nks;  // this replica's kick spans [ks0, ks0 + nks)
     float dslack;      // rounding allowance of the displacement test
-    float rl_max;      // list mode: farthest reach of a list entry,
-                       // rc*sigma_max + 2 skin (only a particle that close
-                       // to an edge has pairs needing the minimum image)
     float im_uniform;  // 1/m shared by every particle of the replica (the
                        // kick then loads no inv_m); 0: masses differ
     int32_t pg;        // particles [pg, p1) have element or bond corners (or
@@ -57,7 +54,7 @@ struct alignas(16) RepDesc {
     float rc_list;     // list mode: the largest cutoff of the pair table (in
                        // sigma units); lists hold every pair within
                        // rc_list * sigma_ij + skin
-    int32_t pad_[2];
+    int32_t pad_[3];
 };
 
 // Per-step observables (femmd2D/src/properties.cpp:15-86, evaluated and

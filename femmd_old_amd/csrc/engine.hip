@@ -189,10 +189,6 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
         D.rc_list = rcl;
         const double rl = (double)rcl * s0 + E->skin_;
         D.rl2_uniform = (float)(rl * rl * 1.00002);
-        // a listed pair was within rc*sigma + skin at the build and each end
-        // moved at most skin / 2 since: rc*sigma_max + 2 skin bounds its
-        // separation (plus a margin for rounding)
-        D.rl_max = (float)((rcmax * hi + 2.0 * E->skin_) * 1.001 + 1e-3);
         amax = std::max(amax, (double)(D.Lx / D.ncx) * (D.Ly / D.ncy));
         rho_cp = std::max(rho_cp, 1.16 / ((double)lo * lo));
     }
