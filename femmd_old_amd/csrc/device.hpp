@@ -22,6 +22,10 @@ constexpr int kKickThreads = 256;
 constexpr int kElemThreads = 256;
 constexpr int kMaxPairTypes = 16;  // particle types of a pair table
 constexpr int kMaxListCap = 512;  // neighbour-list capacity per particle
+// Largest per-cell capacity: a row of 3 cells holds < 256 candidates, so the
+// list build buffers an entry as its one-byte offset in that row.
+constexpr int kMaxCellCap = 60;
+static_assert(3 * kMaxCellCap < 256, "one-byte list-build offsets");
 constexpr int kMaxListGroups = kMaxListCap / 4;
 // A list slot's word: length in the low bits, kListImg when the particle's
 // neighbours include a halo cell across the periodic boundary (its entries

@@ -194,7 +194,7 @@ int Engine::build(const std::vector<const HostSystem *> &reps, std::unique_ptr<E
     }
     // per-cell list capacity: twice the close-packed count of a cell
     E->rho_cp_ = rho_cp;
-    E->cap_ = std::min(60, std::max(32, (int)std::ceil((2.0 * rho_cp * amax + 4.0) / 8.0) * 8));
+    E->cap_ = std::min(kMaxCellCap, std::max(32, (int)std::ceil((2.0 * rho_cp * amax + 4.0) / 8.0) * 8));
     // neighbour-list capacity: 1.3 x the close-packed count of the list disc
     if (E->list_) {
         const double m = 1.3 * M_PI * rlmax * rlmax * rho_cp;

@@ -1553,15 +1553,18 @@ __device__ int list_rank(int n4, bool img, bool act, int capg, int *hist) {
     return act ? hist[key * nw + w] + __popcll(eq & before) : -1;
 }
 
-// List build of one staged tile through the LDS list buffer (slot k of lane
-// t at buf[k * blockDim + t]).  Each row's candidate loop runs a
-// wavefront-uniform trip count (the longest run among the lanes; a lane
-// past its own run is masked), and every candidate is written to the lane's
-// next buffer slot while only hits advance it, so the loop has no branches.
-// Reads past a lane's run stay inside the staging's headroom of 3 cells.
+// List build of one staged tile through the LDS list buffer: slot k of lane
+// t at buf[k * (blockDim + 4) + t] holds the entry's offset in its row's run
+// of candidates (< 3 cell capacities <= 180: one byte; the lane's counts at
+// the two row ends tell the row), so that four workgroups fit on a CU.  Each
+// row's candidate loop runs a wavefront-uniform trip count (the longest run
+// among the lanes; a lane past its own run is masked), and every candidate
+// is written to the lane's next buffer slot while only hits advance it, so
+// the loop has no branches.  Reads past a lane's run stay inside the
+// staging's headroom of 3 cells.
 template <bool kWrap, bool kSigma>
 __device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepDesc &R, int W,
-                                   PairLds &L, uint16_t *buf, int *hist) {
+                                   PairLds &L, uint8_t *buf, int *hist) {
     const ImgCols ic = img_cols(T, R, W);
     const int row = W + 2;
     const int c_lo = L.off[row + 1], c_hi = L.off[row + W + 1];
@@ -1569,6 +1572,7 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepD
     const int sbase = L.cstart[row + 1];
     const float Lx = R.Lx, Ly = R.Ly, hLx = R.hLx, hLy = R.hLy;
     const int bd = blockDim.x;  // held in a register across the loops
+    const int bs = bd + 4;      // buffer row stride (bytes): a slot's rows rotate over the banks
     const int tid = threadIdx.x;
     const int cap = a.nbr_cap, capm1 = a.nbr_cap - 1;
     // every lane of a wavefront takes part in each pass (uniform loop)
@@ -1588,11 +1592,13 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepD
         const int gix = gi >= 0 ? gi : -2;
         const float si = pi.w;
         int cnt = 0;
+        int jr[3], cr[2];  // each row's first candidate; the count after rows 0 and 1
         // the three staged rows, each a contiguous run of candidates
         // (cells col-1 .. col+1); the centre particle is in row 1 only
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int j0 = L.off[r * row + col - 1];
+            jr[r] = j0;
             const int len = act ? L.off[r * row + col + 2] - j0 : 0;
             const int lmax = __builtin_amdgcn_readfirstlane(wave_max_int(len));
 #pragma unroll 2
@@ -1622,10 +1628,11 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepD
                 }
                 bool hit = (t < len) & (dr2 < rl2) & (__float_as_int(pj.z) != gix);
                 if (r == 1) hit &= (j != me);
-                // (24-bit multiply: full rate; cnt <= cap <= 512, bd <= 256)
-                buf[__umul24(min(cnt, capm1), bd) + tid] = (uint16_t)(8 * j);
+                // (24-bit multiply: full rate; cnt <= cap <= 512, bs <= 260)
+                buf[__umul24(min(cnt, capm1), bs) + tid] = (uint8_t)t;
                 cnt += hit ? 1 : 0;
             }
+            if (r < 2) cr[r] = cnt;
         }
         const int m = min(cnt, cap);
         // the chunk's lists are stored longest first (list slot q = sbase +
@@ -1643,7 +1650,8 @@ __device__ void list_tile_body_buf(const ForceArgs &a, const Tile &T, const RepD
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int kk = 4 * g + t;
-                    jj[t] = kk < m ? (int)buf[__umul24(kk, bd) + tid] : 8 * a.lds_cap;
+                    const int j0 = kk < cr[0] ? jr[0] : (kk < cr[1] ? jr[1] : jr[2]);
+                    jj[t] = kk < m ? 8 * (j0 + (int)buf[__umul24(kk, bs) + tid]) : 8 * a.lds_cap;
                 }
                 list_store_group(a, g, p, jj);
             }
@@ -1667,9 +1675,9 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
     L.pt = nullptr;  // the list radius comes from the replica (rc_list)
     L.red = nullptr;
     // list buffer after the staging and its read headroom of 3 cells
-    uint16_t *buf = reinterpret_cast<uint16_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
+    uint8_t *buf = reinterpret_cast<uint8_t *>(dyn + a.lds_cap + 3 * a.cell_cap);
     // the ranking histogram after the list buffer (list_build_lds_bytes)
-    int *hist = reinterpret_cast<int *>(buf + (size_t)a.nbr_cap * blockDim.x);
+    int *hist = reinterpret_cast<int *>(buf + (size_t)a.nbr_cap * (blockDim.x + 4));
     if (a.ctl && a.any && !*a.any) return;  // gated, and no replica rebuilds
     // gated batches walk the listed tiles of the rebuilding replicas
     const int nt = a.ftile ? *a.nft : a.ntiles;
@@ -3160,7 +3168,7 @@ hipError_t launch_forces(bool sigma, bool types, bool debug, bool list, bool ene
 size_t list_build_lds_bytes(int lds_cap, int cell_cap, int nbr_cap, int block, bool buffered) {
     // staging; list buffer and the ranking histogram (list_rank)
     return (size_t)(lds_cap + 3 * cell_cap) * sizeof(float4) +
-           (buffered ? (size_t)nbr_cap * block * sizeof(uint16_t) +
+           (buffered ? (size_t)nbr_cap * (block + 4) +
                            (size_t)2 * (nbr_cap / 4 + 1) * (block / 64) * sizeof(int)
                      : 0);
 }
