@@ -203,3 +203,44 @@ def test_particles_wrapping_between_list_builds(skin, tile_cells):
     assert wraps_stale >= 20
     assert s.device_errors() == 0
     s.close()
+
+
+def test_long_candidate_runs_in_the_list_build():
+    """Cells 6.55 wide (skin 4) at hex density 1.04 hold about 45 particles,
+    so a row of three staged cells offers up to about 146 candidates: the list
+    build buffers each entry as its offset in that run, here past 127 (the
+    offsets stay below 3 cell capacities <= 180 < 256).  128-thread
+    workgroups keep the buffer in LDS.  The interacting sets at the start
+    and after steps equal the oracle's FP32 restatement."""
+    L, d = 65.5, 1.04  # 10 cells of 6.55 per axis
+    rows = 2 * int(L / (2 * d * np.sqrt(3) / 2))  # even: the stagger is periodic
+    cols = int(L / d)
+    gx, gy = np.meshgrid(np.arange(cols) * (L / cols), np.arange(rows) * (L / rows))
+    gx = gx + 0.5 * (L / cols) * (np.arange(rows)[:, None] % 2)
+    rng = np.random.default_rng(17)
+    x = (gx.ravel() + rng.uniform(-0.01, 0.01, gx.size)).astype(np.float32)
+    y = (gy.ravel() + rng.uniform(-0.01, 0.01, gy.size)).astype(np.float32)
+    O.wrap_f32(x, y, np.float32(L), np.float32(L))
+    s = _system(L, 4.0)
+    s.add_particles(x, y, rng.normal(0, 0.5, x.size), rng.normal(0, 0.5, x.size))
+    s.allocate_cuda()
+    s.set_tuning(block=128)
+    ncx, ncy, _, _, _ = s.cells()
+    counts = np.bincount(O.cell_keys(x, y, O.box_f32(L), O.box_f32(L), ncx, ncy),
+                         minlength=ncx * ncy).reshape(ncy, ncx)
+    runs = counts + np.roll(counts, 1, axis=1) + np.roll(counts, -1, axis=1)
+    print(f"[long runs] N={x.size}, {ncx}x{ncy} cells, particles per cell "
+          f"{counts.min()}..{counts.max()}, candidate runs up to {runs.max()}")
+    assert ncx == 10 and runs.max() > 128
+    osys = O.OracleSystem(O.box_f32(L), O.box_f32(L), x, y)
+    for step in range(3):
+        cnt, s1, s2 = s.pair_sets()
+        xc, yc, _, _ = s.get_state()
+        ocnt, os1, os2 = O.pair_sets(osys.with_positions(xc, yc), 1)
+        np.testing.assert_array_equal(cnt, ocnt)
+        np.testing.assert_array_equal(s1, os1)
+        np.testing.assert_array_equal(s2, os2)
+        s.run(4)
+    print(f"[long runs] lists {s.list_stats()}")
+    assert s.device_errors() == 0
+    s.close()
