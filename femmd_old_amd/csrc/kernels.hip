@@ -1556,7 +1556,8 @@ __device__ int list_rank(int n4, bool img, bool act, int capg, int *hist) {
 // List build of one staged tile through the LDS list buffer: slot k of lane
 // t at buf[k * (blockDim + 4) + t] holds the entry's offset in its row's run
 // of candidates (< 3 cell capacities <= 180: one byte; the lane's counts at
-// the two row ends tell the row), so that four workgroups fit on a CU.  Each
+// the two row ends tell the row): half the LDS of two-byte entries, so more
+// workgroups share a CU, which this latency-bound loop needs.  Each
 // row's candidate loop runs a wavefront-uniform trip count (the longest run
 // among the lanes; a lane past its own run is masked), and every candidate
 // is written to the lane's next buffer slot while only hits advance it, so
