@@ -118,7 +118,8 @@ struct alignas(16) PairEntry {
 // Work tables: one workgroup per entry.
 struct alignas(16) Tile {  // force pass, LJ role
     int32_t rep, cy, cx0;
-    int32_t w_flags;  // W in bits 0..15, bit 16 = needs minimum image
+    int32_t w_flags;  // W in bits 0..15, bit 16 = needs minimum image,
+                      // bit 17 in x, bit 18 in y
 };
 struct alignas(16) Span {  // kick / element / bond workgroups
     int32_t rep, begin, end, pad;

@@ -599,9 +599,16 @@ int Engine::build_tiles(const std::vector<int> &tile_w, std::string &err) {
                 const int w = std::min(W, D.ncx - cx0);
                 // interior tiles (no staged cell wraps, >= 5 cells per axis)
                 // never need the minimum image: |dx| < 2 cells <= 0.4 L
-                const bool interior = D.ncx >= 5 && D.ncy >= 5 && cy >= 1 && cy <= D.ncy - 2 &&
-                                      cx0 >= 1 && cx0 + w <= D.ncx - 1;
-                tiles_h_.push_back(Tile{r, cy, cx0, w | (interior ? 0 : (1 << 16))});
+                const bool big = D.ncx >= 5 && D.ncy >= 5;
+                const bool in_x = big && cx0 >= 1 && cx0 + w <= D.ncx - 1;
+                const bool in_y = big && cy >= 1 && cy <= D.ncy - 2;
+                // bit 16: some list may need the minimum image; bits 17, 18:
+                // in x, in y (a particle that wraps between list builds is
+                // then more than rc from every centre in an interior row or
+                // column, so only the edge rows and columns need an axis)
+                tiles_h_.push_back(Tile{r, cy, cx0,
+                                        w | (in_x && in_y ? 0 : (1 << 16)) | (in_x ? 0 : (1 << 17)) |
+                                            (in_y ? 0 : (1 << 18))});
             }
         }
         // staged particles of a tile at close packing (1.16 / sigma_min^2)

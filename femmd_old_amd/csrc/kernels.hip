@@ -1722,7 +1722,7 @@ __global__ __launch_bounds__(256) void k_list_build(ForceArgs a) {
 // entries, so U_i = 4 eps (A - B) + n shift = 4 eps T - 2 eps (Q - kq L)
 // and W_i = 48 eps (A - B / 2) = 48 eps T (femmd2D/src/interactionsCuda.cu:
 // 91-96).  kCount (eps = 0, or the dump): n is counted instead.
-template <bool kWrap, bool kSigma, bool kTypes, bool kEnergy, bool kUnit, bool kDebug, bool kCount,
+template <int kWrap, bool kSigma, bool kTypes, bool kEnergy, bool kUnit, bool kDebug, bool kCount,
           int kKick>
 __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int rep, int W, int tile,
                                     const PairLds &L, const float2 *lp, const float *ls,
@@ -1759,7 +1759,7 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
         // in a tile that needs the minimum image, only the wavefronts
         // holding a list flagged for it take the image path (the build
         // ranks those lists after the others); elsewhere the shift is 0
-        const bool wimg = kWrap && __ballot((nq & kListImg) != 0) != 0ull;
+        const bool wimg = kWrap != 0 && __ballot((nq & kListImg) != 0) != 0ull;
         // single type: T = sum_hit tv, Q as described above; with a type
         // table: T = sum_hit e4 (s12 - s6), Q = sum_hit e48 tv, ush = sum_hit
         // shift (femmd2D/src/interactions.cpp:347-389 per type pair)
@@ -1769,15 +1769,20 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
         // one list entry, without branches: the force of a non-interacting
         // entry (or of the padding entry) is computed and discarded by a
         // select
+        // wrapc: the axes that may need the minimum image (bit 0 x, bit 1
+        // y); an axis left out has no shift in this tile, and its plain
+        // difference has the bits the image path would give
         auto term = [&](auto wrapc, float2 pj, float sj, int tj, int jo) {
-            constexpr bool kW = decltype(wrapc)::value;
+            constexpr int kW = decltype(wrapc)::value;
+            constexpr bool kWX = (kW & 1) != 0, kWY = (kW & 2) != 0;
             const float xj = pj.x, yj = pj.y;
             const float ex = xi - xj, ey = yi - yj;
             float ox = 0.0f, oy = 0.0f;
-            if (kW) {
-                // minimum image as two selects per axis
+            if (kWX) {  // minimum image as two selects per axis
                 ox = (ex >= hLx) ? Lx : 0.0f;
                 ox = (ex < -hLx) ? -Lx : ox;
+            }
+            if (kWY) {
                 oy = (ey >= hLy) ? Ly : 0.0f;
                 oy = (ey < -hLy) ? -Ly : oy;
             }
@@ -1808,8 +1813,8 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
             if (kW) {
                 // image-exact difference: the far coordinate is shifted by L
                 // before subtracting (same bits as dx where no shift applies)
-                dxa = (ox > 0.0f) ? (xim - xj) : ((ox < 0.0f) ? (xi - (xj - Lx)) : dx);
-                dya = (oy > 0.0f) ? (yim - yj) : ((oy < 0.0f) ? (yi - (yj - Ly)) : dy);
+                if (kWX) dxa = (ox > 0.0f) ? (xim - xj) : ((ox < 0.0f) ? (xi - (xj - Lx)) : dx);
+                if (kWY) dya = (oy > 0.0f) ? (yim - yj) : ((oy < 0.0f) ? (yi - (yj - Ly)) : dy);
                 r2a = dxa * dxa + dya * dya;
             }
             const float inv = __builtin_amdgcn_rcpf(r2a);
@@ -1880,8 +1885,8 @@ __device__ void list_tile_pass_body(const ForceArgs &a, const RepDesc &R, int re
                 group(wrapc, cur);
             }
         };
-        if (wimg) groups(std::integral_constant<bool, kWrap>{});
-        else groups(std::integral_constant<bool, false>{});
+        if (wimg) groups(std::integral_constant<int, kWrap>{});
+        else groups(std::integral_constant<int, 0>{});
         const int id = a.perm[p];
         if (kTypes)
             tile_kick<kKick, true, kEnergy>(a, R, id, p, xi, yi, fx, fy, tk);
@@ -1945,19 +1950,28 @@ __device__ void list_tile_pass(const ForceArgs &a, int tile, PairLds &L, float2 
         if (kTypes) lt[a.lds_cap] = 0;
     }
     __syncthreads();
-    const bool wrapT = (T.w_flags >> 16) & 1;
+    // the axes whose minimum image the tile's lists may need: x alone for
+    // a tile at the left or right edge away from the top and bottom rows
+    // (it needs no y image), else both (bit 0 x, bit 1 y)
+    const int wm = ((T.w_flags >> 17) & 3) == 1 ? 1 : (((T.w_flags >> 16) & 1) ? 3 : 0);
 #define FM_LPB(WR, U, C) \
     list_tile_pass_body<WR, kSigma, kTypes, kEnergy, U, kDebug, C, kKick>(a, R, T.rep, W, tile, L, lp, ls, lid, lt)
     // hits are counted for the dump, and for the energy when eps = 0
     if (kDebug || (!kTypes && kEnergy && L.pt[0].e4 == 0.0f)) {
-        if (wrapT) FM_LPB(true, false, true);
-        else FM_LPB(false, false, true);
+        if (wm) FM_LPB(3, false, true);
+        else FM_LPB(0, false, true);
     } else if (!kSigma && R.s2_uniform == 1.0f) {  // (sigma^2 = 1: the same bits)
-        if (wrapT) FM_LPB(true, true, false);
-        else FM_LPB(false, true, false);
+        switch (wm) {
+            case 1: FM_LPB(1, true, false); break;
+            case 3: FM_LPB(3, true, false); break;
+            default: FM_LPB(0, true, false);
+        }
     } else {
-        if (wrapT) FM_LPB(true, false, false);
-        else FM_LPB(false, false, false);
+        switch (wm) {
+            case 1: FM_LPB(1, false, false); break;
+            case 3: FM_LPB(3, false, false); break;
+            default: FM_LPB(0, false, false);
+        }
     }
 #undef FM_LPB
 }
